@@ -1,0 +1,52 @@
+// ocean_kernels.h -- internal interface between the C ABI (ocean_capi.cpp) and
+// the gfx950 kernels (ocean_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocn {
+
+// Per-ocean spectrum constants, FP32 for the kernels plus the FP64 pieces of
+// the phase.  Filled by the host from ocn_params.
+struct SpecConst {
+    int      N;
+    float    kscale;      // 2 pi / L
+    float    Lp2;         // (V^2 / g)^2
+    float    l2;          // suppression^2
+    float    A;           // Phillips amplitude
+    float    wx, wy;      // unit wind direction
+    uint32_t key0, key1;  // Philox key (seed)
+    uint32_t ocean_id, cascade_id;
+    int      conj_mode;   // 0: reference literal (identity), 1: textbook conj
+    double   kscale_d;    // 2 pi / L
+    double   g_d;         // gravity
+};
+
+struct FrameArgs {
+    const float4* h0;
+    float2*       z1;         // N x N
+    float2*       z2;         // N/2 x N
+    const float2* tw;         // Stockham twiddles of the (N, radix) plan
+    SpecConst     sc;
+    double        t_rev;      // t / (2 pi)
+    float*        height;
+    float*        dx;
+    float*        dz;
+    float4*       normals;    // may be null
+    float*        jacobian;   // may be null
+    int           normal_mode;
+    float         lambda;
+    float         inv2d;      // N / (2 L)
+};
+
+hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s);
+hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, float2* hz, const SpecConst& sc,
+                             double t_rev, hipStream_t s);
+hipError_t launch_frame(int N, const FrameArgs& a, hipStream_t s);
+hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s);
+hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
+                          float* jacobian, int mode, float lambda, float inv2d, hipStream_t s);
+int        fft_radix();
+
+}  // namespace ocn

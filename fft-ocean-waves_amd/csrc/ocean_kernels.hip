@@ -1,0 +1,428 @@
+// ocean_kernels.hip -- gfx950 kernels of the Tessendorf ocean path.
+//
+// Reference passes (diharaw/fft-ocean-waves) and their kernels here:
+//   tilde_h0_k         (main.cpp:553-583, tilde_h0_k_cs.glsl)   -> k_tilde_h0_k, k_tilde_h0_k_noise
+//   tilde_h0_t         (main.cpp:587-608, tilde_h0_t_cs.glsl)   -> k_tilde_h0_t            (pass API)
+//   butterfly_fft      (main.cpp:612-683, butterfly_cs.glsl,
+//                       inversion_cs.glsl)                      -> k_c2c_rows + k_c2c_cols_inv (pass API)
+//   generate_normal_map(main.cpp:687-707, normal_map_cs.glsl)   -> k_normals
+//   update() frame     (main.cpp:240-244)                       -> k_cols_fused + k_rows_fused + k_normals
+//
+// Fused frame data flow (per texel, FP32):
+//   h0 (float4: h0k, h0mk) --k_cols_fused--> Z1 (N x N float2), Z2 (N/2 x N float2)
+//                          --k_rows_fused--> height, dx, dz (float)
+//                          --k_normals-----> normals (float4), Jacobian (float)
+// k_cols_fused evaluates the spectrum at texel t and at its mirror t' = (-t mod N),
+// folds each field F into its Hermitian part A = (F(t) + conj F(t'))/2 (whose
+// inverse DFT is Re of F's inverse DFT, which is all inversion_cs.glsl keeps),
+// and runs the column (ty) transforms for the half plane tx in [0, N/2].  It
+// stores the packed rows Z1 = G_h + i G_dx (full width via G(-tx) = conj G(tx))
+// and Z2 = G_dz(y) + i G_dz(y + N/2).  k_rows_fused runs the row (tx)
+// transforms: Re/Im of each result are two real outputs at once.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+#include "ocean_kernels.h"
+
+namespace ocn {
+
+// ------------------------------------------------------------------ noise
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+// (m + 1/2) * 2^-23 from the top 23 bits: exact in FP32.
+__device__ __forceinline__ float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
+
+// gauss_rnd (tilde_h0_k_cs.glsl:51-68): clamp to [0.001, 1], two Box-Muller pairs.
+__device__ __forceinline__ float4 box_muller(float n0, float n1, float n2, float n3)
+{
+    n0 = fminf(fmaxf(n0, 0.001f), 1.0f);
+    n1 = fminf(fmaxf(n1, 0.001f), 1.0f);
+    n2 = fminf(fmaxf(n2, 0.001f), 1.0f);
+    n3 = fminf(fmaxf(n3, 0.001f), 1.0f);
+    float s0, c0, s1, c1;
+    sincospif(2.0f * n0, &s0, &c0);
+    sincospif(2.0f * n2, &s1, &c1);
+    const float r0 = sqrtf(-2.0f * logf(n1));
+    const float r1 = sqrtf(-2.0f * logf(n3));
+    return make_float4(r0 * c0, r0 * s0, r1 * c1, r1 * s1);
+}
+
+// clamp(sqrt(P(k))/sqrt(2), -4000, 4000) (tilde_h0_k_cs.glsl:35-45, 76-88).
+// k_hat = k / |k|_clamped: equal to normalize(k) for k != 0, and 0 at DC where
+// normalize(0) is undefined.  pow(d, 2.0) is d*d.
+__device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int ty)
+{
+    const float kx = sc.kscale * (float)(tx - sc.N / 2);
+    const float ky = sc.kscale * (float)(ty - sc.N / 2);
+    float kmag = sqrtf(kx * kx + ky * ky);
+    if (kmag < 0.00001f) kmag = 0.00001f;
+    const float k2  = kmag * kmag;
+    const float sup = expf(-k2 * sc.l2);
+    const float d   = (kx * sc.wx + ky * sc.wy) / kmag;
+    const float P   = sc.A * expf(-1.0f / (k2 * sc.Lp2)) * (d * d) * sup / (k2 * k2);
+    return fminf(sqrtf(P) * 0.70710678118654752f, 4000.0f);
+}
+
+// ------------------------------------------------------------------ A3 tilde_h0_k
+
+// tilde_h0_k_cs.glsl:74-94 with Philox uniforms (counter = (texel, ocean,
+// cascade, 0), key = seed).  Output: float4 (h0k.re, h0k.im, h0mk.re, h0mk.im).
+__global__ __launch_bounds__(256) void k_tilde_h0_k(float4* __restrict__ h0, SpecConst sc)
+{
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y;
+    if (tx >= sc.N) return;
+    const uint32_t idx = (uint32_t)ty * (uint32_t)sc.N + (uint32_t)tx;
+    const uint4 r  = philox4x32_10(make_uint4(idx, sc.ocean_id, sc.cascade_id, 0u), make_uint2(sc.key0, sc.key1));
+    const float4 xi = box_muller(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
+    const float s  = phillips_amp(sc, tx, ty);
+    h0[idx] = make_float4(s * xi.x, s * xi.y, s * xi.z, s * xi.w);
+}
+
+// Same with the reference's four 256x256 8-bit noise images sampled NEAREST +
+// CLAMP_TO_EDGE at gid/N (tilde_h0_k_cs.glsl:53-58, main.cpp:1102-1116).
+__global__ __launch_bounds__(256) void k_tilde_h0_k_noise(float4* __restrict__ h0, const uint8_t* __restrict__ noise,
+                                                          SpecConst sc)
+{
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y;
+    if (tx >= sc.N) return;
+    int ix = (tx * 256) / sc.N, iy = (ty * 256) / sc.N;
+    ix = ix > 255 ? 255 : ix;
+    iy = iy > 255 ? 255 : iy;
+    const int o = iy * 256 + ix;
+    const float4 xi = box_muller(noise[o] / 255.0f, noise[65536 + o] / 255.0f, noise[131072 + o] / 255.0f,
+                                 noise[196608 + o] / 255.0f);
+    const float s = phillips_amp(sc, tx, ty);
+    h0[(size_t)ty * sc.N + tx] = make_float4(s * xi.x, s * xi.y, s * xi.z, s * xi.w);
+}
+
+// ------------------------------------------------------------------ A4 time evolution
+
+// h(k, t) = h0k e^{i w t} + C(h0mk) e^{-i w t}, w = sqrt(g |k|_clamped)
+// (tilde_h0_t_cs.glsl:70-110; C = identity in the reference, :42-48).
+// The phase is formed in FP64 and reduced to one turn before the FP32
+// sincos: w t reaches hundreds of radians, where FP32 rounding of w alone
+// would cost ~1e-5 rad.  Also returns the choppy multipliers kx/|k|, ky/|k|.
+__device__ __forceinline__ float2 evolve(const SpecConst& sc, float4 a, int tx, int ty, double t_rev,
+                                         float& mx, float& mz)
+{
+    const int nx = tx - sc.N / 2, ny = ty - sc.N / 2;
+    double kmag = sc.kscale_d * sqrt((double)(nx * nx + ny * ny));
+    if (kmag < 0.00001) kmag = 0.00001;
+    const double rev = sqrt(sc.g_d * kmag) * t_rev;  // w t / (2 pi)
+    const float  fr  = (float)(rev - rint(rev));
+    float s, c;
+    sincospif(2.0f * fr, &s, &c);
+    const float kinv = (float)(1.0 / kmag);
+    mx = sc.kscale * (float)nx * kinv;
+    mz = sc.kscale * (float)ny * kinv;
+    const float bi = sc.conj_mode ? -a.w : a.w;
+    // a*(c + i s) + b*(c - i s)
+    return make_float2(fmaf(a.x, c, -a.y * s) + fmaf(a.z, c, bi * s),
+                       fmaf(a.x, s, a.y * c) + fmaf(bi, c, -a.z * s));
+}
+
+// -i m * H  (tilde_h0_t_cs.glsl:113-126)
+__device__ __forceinline__ float2 choppy(float m, float2 h) { return make_float2(m * h.y, -m * h.x); }
+
+// Pass API: the three spectrum textures of tilde_h0_t (dy, dx, dz), float2 each.
+__global__ __launch_bounds__(256) void k_tilde_h0_t(const float4* __restrict__ h0, float2* __restrict__ hy,
+                                                    float2* __restrict__ hx, float2* __restrict__ hz, SpecConst sc,
+                                                    double t_rev)
+{
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y;
+    if (tx >= sc.N) return;
+    const size_t i = (size_t)ty * sc.N + tx;
+    float mx, mz;
+    const float2 h = evolve(sc, h0[i], tx, ty, t_rev, mx, mz);
+    hy[i] = h;
+    hx[i] = choppy(mx, h);
+    hz[i] = choppy(mz, h);
+}
+
+// ------------------------------------------------------------------ fused frame, pass 1
+
+// Column pass of one frame.  Block = C columns x T threads; thread (c, j)
+// transforms column tx = blockIdx.x*C + c of the half plane (tx <= N/2), rows
+// ty = j + r*T.  Three transforms (height, Dx, Dz) share the column's LDS buffer.
+template <int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void k_cols_fused(const float4* __restrict__ h0, float2* __restrict__ z1,
+                                                          float2* __restrict__ z2, const float2* __restrict__ tw,
+                                                          SpecConst sc, double t_rev)
+{
+    constexpr int T = N / E;
+    __shared__ float2 lds[C * lds_len(N)];
+    const int  c     = threadIdx.x % C;
+    const int  j     = threadIdx.x / C;
+    const int  tx    = blockIdx.x * C + c;
+    const bool valid = tx <= N / 2;
+    const int  txc   = valid ? tx : N / 2;
+    const int  mxc   = (N - txc) & (N - 1);
+
+    float2 ay[E], ax[E], az[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int ty = j + r * T;
+        const int my = (N - ty) & (N - 1);
+        const float4 p = h0[(size_t)ty * N + txc];
+        const float4 q = h0[(size_t)my * N + mxc];
+        float pmx, pmz, qmx, qmz;
+        const float2 hp = evolve(sc, p, txc, ty, t_rev, pmx, pmz);
+        const float2 hq = evolve(sc, q, mxc, my, t_rev, qmx, qmz);
+        // Hermitian part of each field: (F(t) + conj F(t')) / 2
+        ay[r] = make_float2(0.5f * (hp.x + hq.x), 0.5f * (hp.y - hq.y));
+        const float2 xp = choppy(pmx, hp), xq = choppy(qmx, hq);
+        ax[r] = make_float2(0.5f * (xp.x + xq.x), 0.5f * (xp.y - xq.y));
+        const float2 zp = choppy(pmz, hp), zq = choppy(qmz, hq);
+        az[r] = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
+    }
+
+    float2* buf = lds + c * lds_len(N);
+    fft_inverse<N, E>(ay, j, buf, tw, BlockSync{});
+    fft_inverse<N, E>(ax, j, buf, tw, BlockSync{});
+    fft_inverse<N, E>(az, j, buf, tw, BlockSync{});
+
+    if (!valid) return;
+    const bool mirror = mxc != tx;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const size_t row = (size_t)(j + r * T) * N;
+        z1[row + tx] = make_float2(ay[r].x - ax[r].y, ay[r].y + ax[r].x);
+        if (mirror) z1[row + mxc] = make_float2(ay[r].x + ax[r].y, ax[r].x - ay[r].y);
+    }
+#pragma unroll
+    for (int r = 0; r < E / 2; ++r) {
+        const size_t row = (size_t)(j + r * T) * N;
+        const float2 lo = az[r], hi = az[r + E / 2];  // rows y and y + N/2
+        z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
+        if (mirror) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
+    }
+}
+
+// ------------------------------------------------------------------ fused frame, pass 2
+
+// Row pass.  Block y handles rows y and y + N/2: group 0 transforms Z1 row y,
+// group 1 Z1 row y + N/2, group 2 Z2 row y, then applies the inversion step
+// (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).
+template <int N, int E>
+__global__ __launch_bounds__(3 * (N / E)) void k_rows_fused(const float2* __restrict__ z1,
+                                                            const float2* __restrict__ z2, float* __restrict__ height,
+                                                            float* __restrict__ dx, float* __restrict__ dz,
+                                                            const float2* __restrict__ tw)
+{
+    constexpr int T = N / E;
+    __shared__ float2 lds[3 * lds_len(N)];
+    const int g = threadIdx.x / T;
+    const int j = threadIdx.x % T;
+    const int y = blockIdx.x;
+    const float2* src = g == 0 ? z1 + (size_t)y * N : g == 1 ? z1 + (size_t)(y + N / 2) * N : z2 + (size_t)y * N;
+    float2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+    fft_inverse<N, E>(v, j, lds + g * lds_len(N), tw, BlockSync{});
+    constexpr float scale = 1.0f / ((float)N * (float)N);
+    const size_t row_lo = (size_t)y * N, row_hi = (size_t)(y + N / 2) * N;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int x = j + r * T;
+        const float s = ((x + y) & 1) ? -scale : scale;  // N/2 is even: same sign for y + N/2
+        if (g == 0) {
+            height[row_lo + x] = s * v[r].x;
+            dx[row_lo + x]     = s * v[r].y;
+        } else if (g == 1) {
+            height[row_hi + x] = s * v[r].x;
+            dx[row_hi + x]     = s * v[r].y;
+        } else {
+            dz[row_lo + x] = s * v[r].x;
+            dz[row_hi + x] = s * v[r].y;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ A7 normals (+ Jacobian)
+
+__device__ __forceinline__ float tap(const float* __restrict__ f, int N, int x, int y)
+{
+    return f[(size_t)((y + N) & (N - 1)) * N + ((x + N) & (N - 1))];
+}
+// texture() under LINEAR + REPEAT at a texel corner: 2x2 box over (x-1..x, y-1..y).
+__device__ __forceinline__ float tap_bilinear(const float* __restrict__ f, int N, int x, int y)
+{
+    return 0.25f * (tap(f, N, x - 1, y - 1) + tap(f, N, x, y - 1) + tap(f, N, x - 1, y) + tap(f, N, x, y));
+}
+
+// normal_map_cs.glsl:24-54 (Sobel on the height, ny = 1, normalize, w = 1),
+// and optionally the Jacobian J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
+// of the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), by periodic
+// central differences with spacing L/N.
+__global__ __launch_bounds__(256) void k_normals(const float* __restrict__ height, const float* __restrict__ dx,
+                                                 const float* __restrict__ dz, float4* __restrict__ normals,
+                                                 float* __restrict__ jac, int N, int mode, float lambda,
+                                                 float inv2d)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= N || y >= N) return;
+    const size_t i = (size_t)y * N + x;
+    if (normals) {
+        float z[8];
+        const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+        const int oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            z[k] = mode ? tap_bilinear(height, N, x + ox[k], y + oy[k]) : tap(height, N, x + ox[k], y + oy[k]);
+        const float nz = z[0] + 2.0f * z[1] + z[2] - z[5] - 2.0f * z[6] - z[7];
+        const float nx = z[0] + 2.0f * z[3] + z[5] - z[2] - 2.0f * z[4] - z[7];
+        const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
+        normals[i] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+    }
+    if (jac) {
+        const float dxdx = (tap(dx, N, x + 1, y) - tap(dx, N, x - 1, y)) * inv2d;
+        const float dxdz = (tap(dx, N, x, y + 1) - tap(dx, N, x, y - 1)) * inv2d;
+        const float dzdx = (tap(dz, N, x + 1, y) - tap(dz, N, x - 1, y)) * inv2d;
+        const float dzdz = (tap(dz, N, x, y + 1) - tap(dz, N, x, y - 1)) * inv2d;
+        jac[i] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+    }
+}
+
+// ------------------------------------------------------------------ A5+A6 pass API (general complex input)
+
+// Row transforms of a general complex N x N field (the horizontal butterflies).
+template <int N, int E>
+__global__ __launch_bounds__(N / E) void k_c2c_rows(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                    const float2* __restrict__ tw)
+{
+    constexpr int T = N / E;
+    __shared__ float2 lds[lds_len(N)];
+    const int j = threadIdx.x;
+    const size_t row = (size_t)blockIdx.x * N;
+    float2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = src[row + j + r * T];
+    fft_inverse<N, E>(v, j, lds, tw, BlockSync{});
+#pragma unroll
+    for (int r = 0; r < E; ++r) dst[row + j + r * T] = v[r];
+}
+
+// Column transforms + inversion: out = (-1)^(x+y) Re(.) / N^2.
+template <int N, int E, int C>
+__global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __restrict__ src, float* __restrict__ out,
+                                                            const float2* __restrict__ tw)
+{
+    constexpr int T = N / E;
+    __shared__ float2 lds[C * lds_len(N)];
+    const int c = threadIdx.x % C;
+    const int j = threadIdx.x / C;
+    const int x = blockIdx.x * C + c;
+    float2 v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = src[(size_t)(j + r * T) * N + x];
+    fft_inverse<N, E>(v, j, lds + c * lds_len(N), tw, BlockSync{});
+    constexpr float scale = 1.0f / ((float)N * (float)N);
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int y = j + r * T;
+        out[(size_t)y * N + x] = (((x + y) & 1) ? -scale : scale) * v[r].x;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <int N>
+struct Shape {
+    static constexpr int E  = 16;
+    static constexpr int T  = N / E;
+    static constexpr int CC = (512 / T) < 16 ? (512 / T) : 16;  // columns per block, pass 1
+};
+
+template <int N>
+static hipError_t launch_frame_n(const FrameArgs& a, hipStream_t s)
+{
+    using S = Shape<N>;
+    const int nblk1 = (N / 2) / S::CC + 1;  // half plane tx in [0, N/2]
+    hipLaunchKernelGGL((k_cols_fused<N, S::E, S::CC>), dim3(nblk1), dim3(S::CC * S::T), 0, s, a.h0, a.z1, a.z2,
+                       a.tw, a.sc, a.t_rev);
+    hipLaunchKernelGGL((k_rows_fused<N, S::E>), dim3(N / 2), dim3(3 * S::T), 0, s, a.z1, a.z2, a.height, a.dx, a.dz,
+                       a.tw);
+    if (a.normals || a.jacobian)
+        hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4), dim3(256), 0, s, a.height, a.dx, a.dz, a.normals,
+                           a.jacobian, N, a.normal_mode, a.lambda, a.inv2d);
+    return hipGetLastError();
+}
+
+template <int N>
+static hipError_t launch_ifft2_n(const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s)
+{
+    using S = Shape<N>;
+    hipLaunchKernelGGL((k_c2c_rows<N, S::E>), dim3(N), dim3(S::T), 0, s, src, tmp, tw);
+    hipLaunchKernelGGL((k_c2c_cols_inv<N, S::E, S::CC>), dim3(N / S::CC), dim3(S::CC * S::T), 0, s, tmp, out, tw);
+    return hipGetLastError();
+}
+
+#define OCN_DISPATCH_N(N, CALL)                     \
+    switch (N) {                                    \
+    case 64: return CALL(64);                       \
+    case 128: return CALL(128);                     \
+    case 256: return CALL(256);                     \
+    case 512: return CALL(512);                     \
+    case 1024: return CALL(1024);                   \
+    case 2048: return CALL(2048);                   \
+    case 4096: return CALL(4096);                   \
+    default: return hipErrorInvalidValue;           \
+    }
+
+hipError_t launch_frame(int N, const FrameArgs& a, hipStream_t s)
+{
+#define CALL(n) launch_frame_n<n>(a, s)
+    OCN_DISPATCH_N(N, CALL)
+#undef CALL
+}
+
+hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s)
+{
+#define CALL(n) launch_ifft2_n<n>(src, tmp, out, tw, s)
+    OCN_DISPATCH_N(N, CALL)
+#undef CALL
+}
+
+hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s)
+{
+    dim3 grid((N + 255) / 256, N);
+    if (noise) hipLaunchKernelGGL(k_tilde_h0_k_noise, grid, dim3(256), 0, s, h0, noise, sc);
+    else hipLaunchKernelGGL(k_tilde_h0_k, grid, dim3(256), 0, s, h0, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, float2* hz, const SpecConst& sc,
+                             double t_rev, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tilde_h0_t, dim3((N + 255) / 256, N), dim3(256), 0, s, h0, hy, hx, hz, sc, t_rev);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
+                          float* jacobian, int mode, float lambda, float inv2d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
+                       lambda, inv2d);
+    return hipGetLastError();
+}
+
+int fft_radix() { return 16; }
+
+}  // namespace ocn

@@ -1,0 +1,349 @@
+"""ctypes binding of libocean.so (include/ocean_c.h) plus a minimal HIP runtime
+wrapper for device buffers, streams and events.
+
+This is the Python mirror of the reference's ocean passes
+(diharaw/fft-ocean-waves src/main.cpp: tilde_h0_k :553, tilde_h0_t :587,
+butterfly_fft :612, generate_normal_map :687, generate_twiddle_factors :711,
+generate_bit_reversed_indices :733, called from init() :199 / update() :229).
+Every compute call goes through the C ABI to the gfx950 kernels; there is no
+CPU fallback: loading fails loudly when libocean.so is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libocean.so")
+
+OCN_OK, OCN_EINVAL, OCN_ENOMEM, OCN_EHIP, OCN_ESTATE, OCN_ENOTSUP = 0, -1, -2, -3, -4, -5
+OCN_CONJ_REFERENCE, OCN_CONJ_TEXTBOOK = 0, 1
+OCN_NORMALS_NEAREST, OCN_NORMALS_BILINEAR = 0, 1
+OCN_FIELD_DY, OCN_FIELD_DX, OCN_FIELD_DZ = 0, 1, 2
+
+# Every function include/ocean_c.h declares (checked by tests/test_capi.py).
+EXPORTS = [
+    "ocn_params_default", "ocn_create", "ocn_destroy", "ocn_set_params", "ocn_get_params",
+    "ocn_last_error", "ocn_status_string", "ocn_abi_version",
+    "ocn_tilde_h0_k", "ocn_tilde_h0_k_from_noise", "ocn_generate_twiddle_factors",
+    "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
+    "ocn_update", "ocn_update_into", "ocn_update_frames",
+    "ocn_get_fields", "ocn_get_spectrum",
+    "ocn_bit_reversed_indices", "ocn_twiddle_factors",
+]
+
+
+class OcnParams(ctypes.Structure):
+    _fields_ = [
+        ("N", ctypes.c_int32),
+        ("conj_mode", ctypes.c_int32),
+        ("normal_mode", ctypes.c_int32),
+        ("ocean_id", ctypes.c_uint32),
+        ("cascade_id", ctypes.c_uint32),
+        ("reserved0", ctypes.c_uint32),
+        ("seed", ctypes.c_uint64),
+        ("L", ctypes.c_double),
+        ("wind_speed", ctypes.c_double),
+        ("wind_dir_x", ctypes.c_double),
+        ("wind_dir_y", ctypes.c_double),
+        ("amplitude", ctypes.c_double),
+        ("suppression", ctypes.c_double),
+        ("choppiness", ctypes.c_double),
+        ("gravity", ctypes.c_double),
+    ]
+
+
+class OcnFields(ctypes.Structure):
+    _fields_ = [("height", ctypes.c_void_p), ("dx", ctypes.c_void_p), ("dz", ctypes.c_void_p),
+                ("normals", ctypes.c_void_p), ("jacobian", ctypes.c_void_p)]
+
+
+class OceanError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads libocean.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libocean.so is missing: build it with `make -C fft-ocean-waves_amd` "
+                               "or __graft_entry__.build(); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        P = ctypes.POINTER(OcnParams)
+        F = ctypes.POINTER(OcnFields)
+        sig = {
+            "ocn_params_default": [P],
+            "ocn_create": [P, ctypes.POINTER(vp)],
+            "ocn_destroy": [vp],
+            "ocn_set_params": [vp, P],
+            "ocn_get_params": [vp, P],
+            "ocn_abi_version": [],
+            "ocn_tilde_h0_k": [vp, vp],
+            "ocn_tilde_h0_k_from_noise": [vp, ctypes.POINTER(ctypes.c_uint8), vp],
+            "ocn_generate_twiddle_factors": [vp],
+            "ocn_tilde_h0_t": [vp, dbl, vp],
+            "ocn_butterfly_fft": [vp, i32, vp],
+            "ocn_butterfly_fft_buffers": [vp, vp, vp, vp],
+            "ocn_generate_normal_map": [vp, vp],
+            "ocn_update": [vp, dbl, vp],
+            "ocn_update_into": [vp, dbl, F, vp],
+            "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
+            "ocn_get_fields": [vp, F],
+            "ocn_get_spectrum": [vp] + [ctypes.POINTER(vp)] * 4,
+            "ocn_bit_reversed_indices": [i32, ctypes.POINTER(ctypes.c_int32)],
+            "ocn_twiddle_factors": [i32, ctypes.POINTER(ctypes.c_float)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.ocn_last_error.argtypes = []
+        L.ocn_last_error.restype = ctypes.c_char_p
+        L.ocn_status_string.argtypes = [ctypes.c_int]
+        L.ocn_status_string.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OCN_OK:
+        L = lib()
+        raise OceanError(L.ocn_status_string(rc).decode(), L.ocn_last_error().decode())
+    return rc
+
+
+def default_params():
+    p = OcnParams()
+    check(lib().ocn_params_default(ctypes.byref(p)))
+    return p
+
+
+def make_params(N=256, L=1000.0, wind_speed=40.0, wind_dir=(1.0, 1.0), amplitude=4.0, suppression=None,
+                choppiness=1.0, gravity=9.81, seed=42, ocean_id=0, cascade_id=0,
+                conj_mode=OCN_CONJ_REFERENCE, normal_mode=OCN_NORMALS_NEAREST):
+    """Parameter block; suppression defaults to L/2000 (the small-wave cutoff rule)."""
+    p = default_params()
+    p.N, p.L, p.wind_speed = int(N), float(L), float(wind_speed)
+    p.wind_dir_x, p.wind_dir_y = float(wind_dir[0]), float(wind_dir[1])
+    p.amplitude = float(amplitude)
+    p.suppression = float(L / 2000.0 if suppression is None else suppression)
+    p.choppiness, p.gravity = float(choppiness), float(gravity)
+    p.seed, p.ocean_id, p.cascade_id = int(seed), int(ocean_id), int(cascade_id)
+    p.conj_mode, p.normal_mode = int(conj_mode), int(normal_mode)
+    return p
+
+
+def bit_reversed_indices(N):
+    out = np.zeros(N, dtype=np.int32)
+    check(lib().ocn_bit_reversed_indices(N, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return out
+
+
+def twiddle_factors(N):
+    nb = int(round(np.log2(N)))
+    out = np.zeros((nb, N, 4), dtype=np.float32)
+    check(lib().ocn_twiddle_factors(N, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
+
+
+# ---------------------------------------------------------------- HIP runtime (ctypes)
+
+class Hip:
+    """The few HIP runtime calls the tests and the bench need."""
+    H2D, D2H, D2D = 1, 2, 3
+
+    def __init__(self):
+        lib()  # libocean.so pulls in the runtime
+        path = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so")
+        self.rt = ctypes.CDLL(path)
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        r = self.rt
+        for name, args in {
+            "hipMalloc": [ctypes.POINTER(vp), sz], "hipFree": [vp],
+            "hipMemcpy": [vp, vp, sz, ctypes.c_int], "hipMemset": [vp, ctypes.c_int, sz],
+            "hipDeviceSynchronize": [], "hipGetDeviceCount": [ctypes.POINTER(ctypes.c_int)],
+            "hipStreamCreate": [ctypes.POINTER(vp)], "hipStreamDestroy": [vp], "hipStreamSynchronize": [vp],
+            "hipEventCreate": [ctypes.POINTER(vp)], "hipEventDestroy": [vp], "hipEventRecord": [vp, vp],
+            "hipEventSynchronize": [vp],
+            "hipEventElapsedTime": [ctypes.POINTER(ctypes.c_float), vp, vp],
+            "hipGetLastError": [],
+        }.items():
+            fn = getattr(r, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        r.hipGetErrorString.argtypes = [ctypes.c_int]
+        r.hipGetErrorString.restype = ctypes.c_char_p
+
+    def ok(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed: %s (%d)" % (what, self.rt.hipGetErrorString(rc).decode(), rc))
+
+    def device_count(self):
+        n = ctypes.c_int(0)
+        rc = self.rt.hipGetDeviceCount(ctypes.byref(n))
+        return n.value if rc == 0 else 0
+
+    def malloc(self, nbytes):
+        p = ctypes.c_void_p()
+        self.ok(self.rt.hipMalloc(ctypes.byref(p), nbytes), "hipMalloc(%d)" % nbytes)
+        return p.value
+
+    def free(self, p):
+        self.ok(self.rt.hipFree(p), "hipFree")
+
+    def upload(self, dptr, arr):
+        a = np.ascontiguousarray(arr)
+        self.ok(self.rt.hipMemcpy(dptr, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy H2D")
+
+    def download(self, dptr, shape, dtype=np.float32):
+        out = np.empty(shape, dtype=dtype)
+        self.ok(self.rt.hipMemcpy(out.ctypes.data, dptr, out.nbytes, self.D2H), "hipMemcpy D2H")
+        return out
+
+    def memset(self, dptr, val, nbytes):
+        self.ok(self.rt.hipMemset(dptr, val, nbytes), "hipMemset")
+
+    def sync(self):
+        self.ok(self.rt.hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+    def stream(self):
+        s = ctypes.c_void_p()
+        self.ok(self.rt.hipStreamCreate(ctypes.byref(s)), "hipStreamCreate")
+        return s.value
+
+    def stream_sync(self, s):
+        self.ok(self.rt.hipStreamSynchronize(s), "hipStreamSynchronize")
+
+    def stream_destroy(self, s):
+        self.ok(self.rt.hipStreamDestroy(s), "hipStreamDestroy")
+
+    def event(self):
+        e = ctypes.c_void_p()
+        self.ok(self.rt.hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        return e.value
+
+    def record(self, ev, stream=None):
+        self.ok(self.rt.hipEventRecord(ev, stream), "hipEventRecord")
+
+    def elapsed_ms(self, e0, e1):
+        self.ok(self.rt.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = ctypes.c_float()
+        self.ok(self.rt.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+        return ms.value
+
+    def event_destroy(self, ev):
+        self.ok(self.rt.hipEventDestroy(ev), "hipEventDestroy")
+
+
+# ---------------------------------------------------------------- Ocean (reference pass structure)
+
+class Ocean:
+    """One ocean on the GPU, with the reference's passes as methods."""
+
+    def __init__(self, params, stream=None):
+        self.p = params
+        self.N = params.N
+        self.stream = stream
+        h = ctypes.c_void_p()
+        check(lib().ocn_create(ctypes.byref(params), ctypes.byref(h)))
+        self.h = h.value
+        self.hip = Hip()
+
+    def close(self):
+        if self.h:
+            check(lib().ocn_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # init() passes (main.cpp:217-220)
+    def tilde_h0_k(self):
+        check(lib().ocn_tilde_h0_k(self.h, self.stream))
+
+    def tilde_h0_k_from_noise(self, noise4):
+        n = np.ascontiguousarray(noise4, dtype=np.uint8)
+        assert n.shape == (4, 256, 256)
+        check(lib().ocn_tilde_h0_k_from_noise(self.h, n.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                              self.stream))
+
+    def generate_twiddle_factors(self):
+        check(lib().ocn_generate_twiddle_factors(self.h))
+
+    def set_params(self, params):
+        check(lib().ocn_set_params(self.h, ctypes.byref(params)))
+        self.p = params
+
+    # update() passes (main.cpp:240-244)
+    def tilde_h0_t(self, t):
+        check(lib().ocn_tilde_h0_t(self.h, float(t), self.stream))
+
+    def butterfly_fft(self, field):
+        check(lib().ocn_butterfly_fft(self.h, int(field), self.stream))
+
+    def butterfly_fft_buffers(self, src_dptr, dst_dptr):
+        check(lib().ocn_butterfly_fft_buffers(self.h, src_dptr, dst_dptr, self.stream))
+
+    def generate_normal_map(self):
+        check(lib().ocn_generate_normal_map(self.h, self.stream))
+
+    def update(self, t):
+        check(lib().ocn_update(self.h, float(t), self.stream))
+
+    def update_into(self, t, fields):
+        check(lib().ocn_update_into(self.h, float(t), ctypes.byref(fields), self.stream))
+
+    def update_frames(self, times, fields):
+        ts = np.ascontiguousarray(times, dtype=np.float64)
+        check(lib().ocn_update_frames(self.h, ts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(ts),
+                                      ctypes.byref(fields), self.stream))
+
+    # buffers
+    def fields(self):
+        f = OcnFields()
+        check(lib().ocn_get_fields(self.h, ctypes.byref(f)))
+        return f
+
+    def spectrum_ptrs(self):
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        check(lib().ocn_get_spectrum(self.h, *[ctypes.byref(q) for q in ptrs]))
+        return [q.value for q in ptrs]
+
+    def download_fields(self):
+        """Synchronises and copies the handle's outputs to host numpy arrays."""
+        self.hip.sync()
+        f, N = self.fields(), self.N
+        return {
+            "height": self.hip.download(f.height, (N, N)),
+            "dx": self.hip.download(f.dx, (N, N)),
+            "dz": self.hip.download(f.dz, (N, N)),
+            "normals": self.hip.download(f.normals, (N, N, 4)),
+            "jacobian": self.hip.download(f.jacobian, (N, N)),
+        }
+
+    def download_h0(self):
+        self.hip.sync()
+        h0 = self.spectrum_ptrs()[0]
+        return self.hip.download(h0, (self.N, self.N, 4))
+
+    def download_spectrum_t(self):
+        self.hip.sync()
+        _, hy, hx, hz = self.spectrum_ptrs()
+        N = self.N
+        return [self.hip.download(q, (N, N), np.complex64) for q in (hy, hx, hz)]

@@ -1,0 +1,197 @@
+/*
+ * ocean_c.h -- C ABI of the MI355X (gfx950) Tessendorf ocean library
+ * (libocean.so).  Plain pointers, sizes and status codes; no C++ or HIP types.
+ *
+ * The reference (diharaw/fft-ocean-waves) has no plugin API: its ocean path is
+ * six private methods of FFTOceanWaves (src/main.cpp:553-744) called from
+ * init() / update() (main.cpp:199-255), with parameters in members
+ * (main.cpp:1622-1646) and GL textures as outputs (main.cpp:1083-1099).  Each
+ * entry point below names the reference code it replaces.  A binding that
+ * drives the reference path through this ABI is shown in INTEGRATION.md.
+ *
+ * Conventions
+ *   - Every call returns OCN_OK (0) or a negative OCN_E* status; arguments are
+ *     validated on the host before any HIP call.  ocn_last_error() gives the
+ *     message of the calling thread's last failure.
+ *   - Device pointers are HIP device memory on the handle's device; `stream`
+ *     is a hipStream_t passed as void* (NULL = the null stream).  Work is
+ *     stream-ordered; no call synchronises the host except ocn_create,
+ *     ocn_set_params (buffer setup) and ocn_destroy.
+ *   - Layouts match the reference textures: texel (tx, ty) is element
+ *     ty*N + tx; tx is the first texture coordinate (world x), ty the second
+ *     (world z).  R32F fields are float[N*N]; RGBA32F fields are float[N*N*4].
+ *   - A handle is not thread-safe; distinct handles may be used from distinct
+ *     threads and streams.
+ */
+#ifndef OCEAN_C_H
+#define OCEAN_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCN_ABI_VERSION 1
+
+/* status codes */
+#define OCN_OK        0
+#define OCN_EINVAL   (-1) /* bad argument (checked before any HIP call) */
+#define OCN_ENOMEM   (-2) /* device or host allocation failed */
+#define OCN_EHIP     (-3) /* a HIP runtime call failed */
+#define OCN_ESTATE   (-4) /* call out of order (e.g. update before the spectrum exists) */
+#define OCN_ENOTSUP  (-5) /* valid request this build does not support */
+
+/* ocn_params.conj_mode: how h0(-k) enters h(k,t).  The reference's
+ * conjugate() returns its argument unchanged (tilde_h0_t_cs.glsl:42-48), so
+ * REFERENCE computes h0k e^{iwt} + h0mk e^{-iwt}; TEXTBOOK uses conj(h0mk). */
+#define OCN_CONJ_REFERENCE 0
+#define OCN_CONJ_TEXTBOOK  1
+
+/* ocn_params.normal_mode: how normal_map_cs.glsl samples the height at
+ * gid/N +- 1/N (:37-44).  The reference never sets m_dy's sampler state.
+ * NEAREST reads the periodic neighbour texels; BILINEAR reproduces LINEAR +
+ * REPEAT filtering at texel corners (a 2x2 box shifted half a texel). */
+#define OCN_NORMALS_NEAREST  0
+#define OCN_NORMALS_BILINEAR 1
+
+/* field ids for ocn_butterfly_fft (the three butterfly_fft calls, main.cpp:241-243) */
+#define OCN_FIELD_DY 0 /* height:  m_tilde_h0_t_dy -> m_dy */
+#define OCN_FIELD_DX 1 /* choppy x: m_tilde_h0_t_dx -> m_dx */
+#define OCN_FIELD_DZ 2 /* choppy z: m_tilde_h0_t_dz -> m_dz */
+
+/* Ocean parameters.  The reference keeps these as members
+ * (main.cpp:1632-1646: m_wind_speed 80, m_amplitude 2, m_suppression_factor
+ * 0.1, m_wind_direction (1,1), m_N 256, m_L 1000, m_choppiness 0.75) and as
+ * the constant g = 9.81 (tilde_h0_k_cs.glsl:26).  Noise comes from Philox4x32-10
+ * with key = seed and counter = (ty*N + tx, ocean_id, cascade_id, 0), unless
+ * ocn_tilde_h0_k_from_noise supplies the reference's noise images. */
+typedef struct ocn_params {
+    int32_t  N;            /* grid size: power of two, 64..4096 */
+    int32_t  conj_mode;    /* OCN_CONJ_* */
+    int32_t  normal_mode;  /* OCN_NORMALS_* */
+    uint32_t ocean_id;     /* Philox counter word 1 */
+    uint32_t cascade_id;   /* Philox counter word 2 */
+    uint32_t reserved0;
+    uint64_t seed;         /* Philox key */
+    double   L;            /* patch size (m), > 0 (u_L, an int in the reference) */
+    double   wind_speed;   /* V (m/s), >= 0 */
+    double   wind_dir_x;   /* wind direction; normalised by the library (main.cpp:555) */
+    double   wind_dir_y;
+    double   amplitude;    /* Phillips A, >= 0 */
+    double   suppression;  /* small-wave cutoff l (m), >= 0 (u_SuppressFactor) */
+    double   choppiness;   /* lambda; enters only the Jacobian (the reference applies it at render) */
+    double   gravity;      /* g (m/s^2), > 0 */
+} ocn_params;
+
+/* Output fields of one frame (device pointers).  height/dx/dz are unscaled
+ * displacement (the render applies (-lambda*dx, scale*height, -lambda*dz),
+ * grid_tes.glsl:60-62).  normals is RGBA32F (nx, ny, nz, 1).  jacobian is
+ * optional (NULL skips it); normals may be NULL to skip the normal map. */
+typedef struct ocn_fields {
+    float* height;   /* m_dy         (main.cpp:1097) */
+    float* dx;       /* m_dx         (main.cpp:1096) */
+    float* dz;       /* m_dz         (main.cpp:1098) */
+    float* normals;  /* m_normal_map (main.cpp:1099) */
+    float* jacobian; /* no reference counterpart: foam/folding term */
+} ocn_fields;
+
+typedef struct ocn_handle ocn_handle;
+
+/* ---------------------------------------------------------------- lifecycle */
+
+/* Fills the reference's defaults (main.cpp:1632-1646, tilde_h0_k_cs.glsl:26),
+ * seed 42. */
+int ocn_params_default(ocn_params* p);
+
+/* Allocates the device resources of one ocean on the current HIP device:
+ * create_textures (main.cpp:1083-1099) for the path's textures.  Validates p. */
+int ocn_create(const ocn_params* p, ocn_handle** out);
+
+/* Frees everything ocn_create allocated.  NULL is accepted. */
+int ocn_destroy(ocn_handle* h);
+
+/* Replaces the parameters without reallocating (N must not change).  The
+ * reference never re-runs tilde_h0_k after a GUI edit (main.cpp:352-353); here
+ * the next ocn_tilde_h0_k uses the new values. */
+int ocn_set_params(ocn_handle* h, const ocn_params* p);
+int ocn_get_params(const ocn_handle* h, ocn_params* p);
+
+const char* ocn_last_error(void);
+const char* ocn_status_string(int status);
+int         ocn_abi_version(void);
+
+/* ---------------------------------------------------------------- init passes */
+
+/* tilde_h0_k (main.cpp:553-583): Phillips spectrum x Gaussian noise -> h0k,
+ * h0(-k) (one float4 per texel: h0k.re, h0k.im, h0mk.re, h0mk.im). */
+int ocn_tilde_h0_k(ocn_handle* h, void* stream);
+
+/* tilde_h0_k with the reference's noise textures: noise_host is uint8[4][256][256],
+ * the R channel of data/noise/LDR_LLL1_{0..3}.png (main.cpp:1085-1088), sampled
+ * NEAREST + CLAMP_TO_EDGE at gid/N (tilde_h0_k_cs.glsl:53-58). */
+int ocn_tilde_h0_k_from_noise(ocn_handle* h, const uint8_t* noise_host, void* stream);
+
+/* generate_bit_reversed_indices + generate_twiddle_factors (main.cpp:711-744):
+ * builds the handle's FFT plan (twiddle tables) on the device.  ocn_create
+ * already does this; calling it again is harmless. */
+int ocn_generate_twiddle_factors(ocn_handle* h);
+
+/* ---------------------------------------------------------------- per-frame passes */
+
+/* tilde_h0_t (main.cpp:587-608): the three spectrum textures at time t
+ * (seconds, FP64; the reference uses float(glfwGetTime()), main.cpp:599). */
+int ocn_tilde_h0_t(ocn_handle* h, double t, void* stream);
+
+/* butterfly_fft(src, dst) (main.cpp:612-683) for one field: 2-D inverse DFT of
+ * that field's spectrum texture followed by the inversion pass
+ * ((-1)^(x+y) Re / N^2) into the handle's output field.  Unlike the
+ * reference, the spectrum texture is left unchanged. */
+int ocn_butterfly_fft(ocn_handle* h, int field, void* stream);
+
+/* Same on caller buffers: src is float2[N*N] (re, im) device memory, dst is
+ * float[N*N] device memory.  Uses the handle's scratch. */
+int ocn_butterfly_fft_buffers(ocn_handle* h, const void* src, float* dst, void* stream);
+
+/* generate_normal_map (main.cpp:687-707) from the handle's height field, plus
+ * the Jacobian from its dx/dz fields. */
+int ocn_generate_normal_map(ocn_handle* h, void* stream);
+
+/* ---------------------------------------------------------------- fused frame */
+
+/* One frame (main.cpp:240-244): tilde_h0_t, the three butterfly_fft calls and
+ * generate_normal_map, fused into three kernels, into the handle's fields. */
+int ocn_update(ocn_handle* h, double t, void* stream);
+
+/* Same into caller buffers (height, dx, dz required; normals, jacobian optional). */
+int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream);
+
+/* nframes frames at times t_host[0..nframes) into consecutive output slices:
+ * frame f writes out->height + f*N*N, ..., out->normals + f*N*N*4, ...
+ * (the per-frame slices of a recorded sequence). */
+int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream);
+
+/* ---------------------------------------------------------------- buffers */
+
+/* Device pointers of the handle-owned output textures (valid until destroy). */
+int ocn_get_fields(const ocn_handle* h, ocn_fields* out);
+
+/* Device pointers of the spectrum textures: h0 (float4[N*N]) and the three
+ * tilde_h0_t textures (float2[N*N] each; NULL until ocn_tilde_h0_t ran).
+ * Any out-pointer may be NULL. */
+int ocn_get_spectrum(const ocn_handle* h, void** h0, void** hy, void** hx, void** hz);
+
+/* ---------------------------------------------------------------- host utilities */
+
+/* generate_bit_reversed_indices (main.cpp:733-744): out[i] = log2N-bit reversal of i. */
+int ocn_bit_reversed_indices(int N, int32_t* out);
+
+/* The reference's twiddle texture (twiddle_factors_cs.glsl:34-69) in its
+ * layout: out[(s*N + j)*4 + {0..3}] = (Re w, Im w, idxA, idxB), s < log2N. */
+int ocn_twiddle_factors(int N, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCEAN_C_H */

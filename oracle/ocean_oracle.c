@@ -1,0 +1,454 @@
+/*
+ * ocean_oracle.c -- FP64 CPU restatement of the reference ocean path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline).  The product
+ * library never links or calls this file.
+ *
+ * What pins each function (tests/test_oracle.py):
+ *   orc_reverse_bits / orc_bit_reversed_indices : brute-force string reversal, involution.
+ *   orc_twiddle_factors                          : closed form e^{+2 pi i k/N}, index ranges,
+ *                                                  and the FFT result it produces.
+ *   orc_butterfly_fft2d + orc_inversion          : numpy.fft.ifft2 (N^2 * ifft2 == butterfly
+ *                                                  network), O(N^4) direct DFT at N<=16,
+ *                                                  single-mode spectra -> analytic cosines,
+ *                                                  committed numpy golden vectors.
+ *   orc_philox4x32_10                            : Random123 known-answer vectors and
+ *                                                  vectors from rocRAND's philox4x32_10
+ *                                                  engine compiled for the host (golden).
+ *   orc_h0 / orc_phillips_amp                    : k _|_ wind => 0, P(-k) = P(k), linear in A
+ *                                                  (below the clamp), clamp at 4000, DC = 0,
+ *                                                  Box-Muller moments, reference-noise fixture.
+ *   orc_spectrum_t                               : t = 0 reduces to h0k + h0mk; a single mode
+ *                                                  is periodic in 2 pi / omega; the choppy
+ *                                                  multipliers; conj mode.
+ *   orc_normal_map                               : flat height -> (0,1,0); an x- or z-only
+ *                                                  sinusoid gives the analytic Sobel response.
+ *   orc_jacobian                                 : zero displacement -> 1; a linear-in-x
+ *                                                  displacement field gives the closed form.
+ */
+#include "ocean_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+static const double ORC_PI = 3.14159265358979323846264338327950288;
+
+/* ------------------------------------------------------------------ A1 */
+
+/* Reverses the low nbits bits of v.  The reference does the same reversal with
+ * a byte lookup table plus masks (main.cpp:29-64); both give the nbits-bit
+ * mirror image of v for nbits <= 16. */
+uint32_t orc_reverse_bits(uint32_t v, int nbits)
+{
+    uint32_t r = 0;
+    for (int b = 0; b < nbits; ++b) {
+        r = (r << 1) | (v & 1u);
+        v >>= 1;
+    }
+    return r;
+}
+
+/* log2 N the way the reference computes it: int(log(N) / log(2)) in double
+ * (main.cpp:620, 715, 738). */
+int orc_log2n(int N)
+{
+    return (int)(log((double)N) / log(2.0));
+}
+
+/* generate_bit_reversed_indices (main.cpp:733-744). */
+int orc_bit_reversed_indices(int N, int32_t* out)
+{
+    if (N < 2 || (N & (N - 1)) || !out) return -1;
+    int nb = orc_log2n(N);
+    for (int i = 0; i < N; ++i) out[i] = (int32_t)orc_reverse_bits((uint32_t)i, nb);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ A2 */
+
+/* twiddle_factors_cs.glsl:34-69, dispatched as (log2N, N/32) at main.cpp:711-729.
+ * out[(s*N + j)*4 + {0,1,2,3}] = (Re w, Im w, idxA, idxB) for stage s, row j. */
+int orc_twiddle_factors(int N, double* out)
+{
+    if (N < 2 || (N & (N - 1)) || !out) return -1;
+    int nb = orc_log2n(N);
+    int32_t* br = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+    if (!br) return -1;
+    orc_bit_reversed_indices(N, br);
+    for (int s = 0; s < nb; ++s) {
+        double span2 = ldexp(1.0, s + 1);  /* pow(2, x.x + 1) */
+        int    span  = 1 << s;             /* butterflyspan */
+        for (int j = 0; j < N; ++j) {
+            double k   = fmod((double)j * ((double)N / span2), (double)N);   /* :37 */
+            double wre = cos(2.0 * ORC_PI * k / (double)N);                  /* :38 */
+            double wim = sin(2.0 * ORC_PI * k / (double)N);
+            int top = fmod((double)j, span2) < (double)span;                  /* :44-47 */
+            double a, b;
+            if (s == 0) {                                                     /* :50-58 */
+                if (top) { a = br[j];     b = br[j + 1]; }
+                else     { a = br[j - 1]; b = br[j]; }
+            } else {                                                          /* :60-68 */
+                if (top) { a = j;        b = j + span; }
+                else     { a = j - span; b = j; }
+            }
+            double* o = out + ((size_t)s * N + j) * 4;
+            o[0] = wre; o[1] = wim; o[2] = a; o[3] = b;
+        }
+    }
+    free(br);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ A5 */
+
+typedef struct {
+    int N, stage, dir, row0, row1;
+    const double* tw;
+    const double* in;
+    double* out;
+} bfly_job;
+
+/* One butterfly dispatch (butterfly_cs.glsl:54-91 horizontal, :95-132 vertical):
+ * out = in[a] + w * in[b] with (w, a, b) = twiddle[stage][x or y]. */
+static void* bfly_run(void* arg)
+{
+    const bfly_job* J = (const bfly_job*)arg;
+    const int N = J->N;
+    for (int y = J->row0; y < J->row1; ++y) {
+        for (int x = 0; x < N; ++x) {
+            const double* d = J->tw + ((size_t)J->stage * N + (J->dir == 0 ? x : y)) * 4;
+            int a = (int)d[2], b = (int)d[3];
+            size_t ip, iq;
+            if (J->dir == 0) { ip = (size_t)y * N + a; iq = (size_t)y * N + b; }
+            else             { ip = (size_t)a * N + x; iq = (size_t)b * N + x; }
+            double pr = J->in[2 * ip], pi = J->in[2 * ip + 1];
+            double qr = J->in[2 * iq], qi = J->in[2 * iq + 1];
+            double wr = d[0], wi = d[1];
+            size_t o = (size_t)y * N + x;
+            J->out[2 * o]     = pr + (wr * qr - wi * qi);
+            J->out[2 * o + 1] = pi + (wr * qi + wi * qr);
+        }
+    }
+    return NULL;
+}
+
+/* butterfly_fft (main.cpp:612-664): log2N horizontal stages then log2N
+ * vertical stages, ping-ponging between data (pingpong0) and scratch
+ * (pingpong1).  After the even number of stages the result is back in data,
+ * which is where inversion reads it (main.cpp:667-680 with u_PingPong = 0). */
+int orc_butterfly_fft2d(int N, const double* tw, double* data, double* scratch, int nthreads)
+{
+    if (N < 2 || (N & (N - 1)) || !tw || !data || !scratch) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    int nb = orc_log2n(N);
+    int pingpong = 0;
+    pthread_t th[256];
+    bfly_job jobs[256];
+    for (int dir = 0; dir < 2; ++dir) {
+        for (int s = 0; s < nb; ++s) {
+            const double* in  = pingpong == 0 ? data : scratch;
+            double*       out = pingpong == 0 ? scratch : data;
+            int per = (N + nthreads - 1) / nthreads;
+            int nt = 0;
+            for (int t = 0; t < nthreads; ++t) {
+                int r0 = t * per, r1 = r0 + per > N ? N : r0 + per;
+                if (r0 >= r1) break;
+                jobs[t] = (bfly_job){N, s, dir, r0, r1, tw, in, out};
+                nt++;
+            }
+            if (nt == 1) {
+                bfly_run(&jobs[0]);
+            } else {
+                for (int t = 1; t < nt; ++t) pthread_create(&th[t], NULL, bfly_run, &jobs[t]);
+                bfly_run(&jobs[0]);
+                for (int t = 1; t < nt; ++t) pthread_join(th[t], NULL);
+            }
+            pingpong ^= 1;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ A6 */
+
+/* inversion_cs.glsl:25-43: perm = (-1)^(x+y); out = perm * Re(in) / N^2. */
+void orc_inversion(int N, const double* data, double* out)
+{
+    const double inv = 1.0 / ((double)N * (double)N);
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            size_t i = (size_t)y * N + x;
+            double perm = ((x + y) & 1) ? -1.0 : 1.0;
+            out[i] = perm * (data[2 * i] * inv);
+        }
+}
+
+/* ------------------------------------------------------------------ A7 */
+
+static inline double tap_nearest(int N, const double* h, int x, int y)
+{
+    x = ((x % N) + N) % N;  /* GL_REPEAT */
+    y = ((y % N) + N) % N;
+    return h[(size_t)y * N + x];
+}
+
+/* texture(s_HeightMap, (x, y)/N) under LINEAR + REPEAT: the sample point sits
+ * on a texel corner, so the bilinear weights are all 1/4 over texels
+ * (x-1..x, y-1..y). */
+static inline double tap_bilinear(int N, const double* h, int x, int y)
+{
+    return 0.25 * (tap_nearest(N, h, x - 1, y - 1) + tap_nearest(N, h, x, y - 1) +
+                   tap_nearest(N, h, x - 1, y) + tap_nearest(N, h, x, y));
+}
+
+/* normal_map_cs.glsl:24-54.  Taps z0..z7 at (x-1,y-1) (x,y-1) (x+1,y-1)
+ * (x-1,y) (x+1,y) (x-1,y+1) (x,y+1) (x+1,y+1); Sobel; ny = 1; normalize. */
+void orc_normal_map(int N, const double* h, int mode, double* out)
+{
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            double z[8];
+            static const int off[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
+                                          {1, 0},   {-1, 1}, {0, 1}, {1, 1}};
+            for (int i = 0; i < 8; ++i)
+                z[i] = mode == 1 ? tap_bilinear(N, h, x + off[i][0], y + off[i][1])
+                                 : tap_nearest(N, h, x + off[i][0], y + off[i][1]);
+            double nz = z[0] + 2.0 * z[1] + z[2] - z[5] - 2.0 * z[6] - z[7];  /* :49 */
+            double nx = z[0] + 2.0 * z[3] + z[5] - z[2] - 2.0 * z[4] - z[7];  /* :50 */
+            double ny = 1.0;                                                   /* :51 */
+            double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+            double* o = out + ((size_t)y * N + x) * 4;
+            o[0] = nx * inv; o[1] = ny * inv; o[2] = nz * inv; o[3] = 1.0;     /* :53 */
+        }
+}
+
+/* ------------------------------------------------------------------ noise */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the
+ * published multipliers and Weyl key increments. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Uniform in (0, 1) from the top 23 bits: (m + 1/2) * 2^-23.  Every value is
+ * exactly representable in FP32, so the GPU and this oracle start Box-Muller
+ * from bit-identical uniforms. */
+double orc_u01(uint32_t x)
+{
+    return ((double)(x >> 9) + 0.5) * (1.0 / 8388608.0);
+}
+
+/* ------------------------------------------------------------------ A3 */
+
+static void wind_unit(const orc_params* p, double* wx, double* wy)
+{
+    /* glm::normalize(m_wind_direction) (main.cpp:555) */
+    double n = sqrt(p->wind_dir_x * p->wind_dir_x + p->wind_dir_y * p->wind_dir_y);
+    *wx = p->wind_dir_x / n;
+    *wy = p->wind_dir_y / n;
+}
+
+static void wave_vector(const orc_params* p, int tx, int ty, double* kx, double* ky, double* kmag)
+{
+    /* tilde_h0_k_cs.glsl:76-82 / tilde_h0_t_cs.glsl:72-77 */
+    double x = (double)tx - (double)p->N / 2.0;
+    double y = (double)ty - (double)p->N / 2.0;
+    *kx = 2.0 * ORC_PI * x / p->L;
+    *ky = 2.0 * ORC_PI * y / p->L;
+    double m = sqrt(*kx * *kx + *ky * *ky);
+    if (m < 0.00001) m = 0.00001;
+    *kmag = m;
+}
+
+/* philips_power_spectrum + suppression_factor + clamp(sqrt(P)/sqrt(2))
+ * (tilde_h0_k_cs.glsl:35-45, 84-88).  k_hat = k / |k|_clamped, which equals
+ * normalize(k) for every k != 0 and makes the DC term 0 instead of the NaN
+ * that normalize(0) gives.  pow(d, 2.0) is d*d. */
+double orc_phillips_amp(const orc_params* p, int tx, int ty)
+{
+    double kx, ky, kmag, wx, wy;
+    wave_vector(p, tx, ty, &kx, &ky, &kmag);
+    wind_unit(p, &wx, &wy);
+    double g   = p->gravity;
+    double Lp  = p->wind_speed * p->wind_speed / g;                      /* :78 */
+    double k2  = kmag * kmag;                                            /* :84 */
+    double sup = exp(-k2 * p->suppression * p->suppression);             /* :37 */
+    double d   = (kx * wx + ky * wy) / kmag;
+    double P   = p->amplitude * exp(-1.0 / (k2 * Lp * Lp)) * (d * d) * sup / (k2 * k2); /* :44 */
+    double s   = sqrt(P) / sqrt(2.0);                                    /* :87-88 */
+    if (s > 4000.0) s = 4000.0;
+    if (s < -4000.0) s = -4000.0;
+    return s;
+}
+
+/* gauss_rnd (tilde_h0_k_cs.glsl:51-68): four uniforms clamped to [0.001, 1],
+ * two Box-Muller pairs. */
+static void box_muller(const double u[4], double xi[4])
+{
+    double n0 = u[0], n1 = u[1], n2 = u[2], n3 = u[3];
+    n0 = n0 < 0.001 ? 0.001 : (n0 > 1.0 ? 1.0 : n0);
+    n1 = n1 < 0.001 ? 0.001 : (n1 > 1.0 ? 1.0 : n1);
+    n2 = n2 < 0.001 ? 0.001 : (n2 > 1.0 ? 1.0 : n2);
+    n3 = n3 < 0.001 ? 0.001 : (n3 > 1.0 ? 1.0 : n3);
+    double a0 = 2.0 * ORC_PI * n0, r0 = sqrt(-2.0 * log(n1));
+    double a1 = 2.0 * ORC_PI * n2, r1 = sqrt(-2.0 * log(n3));
+    xi[0] = r0 * cos(a0); xi[1] = r0 * sin(a0);
+    xi[2] = r1 * cos(a1); xi[3] = r1 * sin(a1);
+}
+
+static int check_params(const orc_params* p)
+{
+    if (!p || p->N < 2 || (p->N & (p->N - 1))) return -1;
+    if (!(p->L > 0.0) || !(p->gravity > 0.0)) return -1;
+    return 0;
+}
+
+/* tilde_h0_k_cs.glsl:74-94 with Philox noise: counter = (texel index,
+ * ocean_id, cascade_id, 0), key = seed.  h0k = s * xi.xy, h0mk = s * xi.zw
+ * (P(-k) = P(k), so one amplitude serves both). */
+int orc_h0(const orc_params* p, double* h0)
+{
+    if (check_params(p) || !h0) return -1;
+    const int N = p->N;
+    uint32_t key[2] = {(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};
+    for (int ty = 0; ty < N; ++ty)
+        for (int tx = 0; tx < N; ++tx) {
+            uint32_t ctr[4] = {(uint32_t)(ty * N + tx), p->ocean_id, p->cascade_id, 0u}, r[4];
+            orc_philox4x32_10(ctr, key, r);
+            double u[4] = {orc_u01(r[0]), orc_u01(r[1]), orc_u01(r[2]), orc_u01(r[3])}, xi[4];
+            box_muller(u, xi);
+            double s = orc_phillips_amp(p, tx, ty);
+            double* o = h0 + ((size_t)ty * N + tx) * 4;
+            o[0] = s * xi[0]; o[1] = s * xi[1]; o[2] = s * xi[2]; o[3] = s * xi[3];
+        }
+    return 0;
+}
+
+/* Same, with the reference's noise textures: noise_i = texture(noise_i, gid/N).r
+ * with NEAREST + CLAMP_TO_EDGE on a 256x256 8-bit UNORM image
+ * (tilde_h0_k_cs.glsl:53-58, main.cpp:1085-1088, 1102-1116).  Image row 0 is
+ * texture row 0. */
+int orc_h0_from_noise(const orc_params* p, const uint8_t* noise4, double* h0)
+{
+    if (check_params(p) || !h0 || !noise4) return -1;
+    const int N = p->N;
+    for (int ty = 0; ty < N; ++ty)
+        for (int tx = 0; tx < N; ++tx) {
+            int ix = (int)floor((double)tx / N * 256.0), iy = (int)floor((double)ty / N * 256.0);
+            ix = ix < 0 ? 0 : (ix > 255 ? 255 : ix);
+            iy = iy < 0 ? 0 : (iy > 255 ? 255 : iy);
+            double u[4], xi[4];
+            for (int c = 0; c < 4; ++c) u[c] = noise4[(size_t)c * 65536 + (size_t)iy * 256 + ix] / 255.0;
+            box_muller(u, xi);
+            double s = orc_phillips_amp(p, tx, ty);
+            double* o = h0 + ((size_t)ty * N + tx) * 4;
+            o[0] = s * xi[0]; o[1] = s * xi[1]; o[2] = s * xi[2]; o[3] = s * xi[3];
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ A4 */
+
+/* tilde_h0_t_cs.glsl:70-131.  omega = sqrt(g |k|); H = h0k e^{i w t} +
+ * C(h0mk) e^{-i w t} where C is the identity in conj_mode 0 (the reference's
+ * conjugate() returns its argument unchanged, :42-48) and complex conjugation
+ * in conj_mode 1.  Hx = -i (kx/|k|) H, Hz = -i (ky/|k|) H. */
+int orc_spectrum_t(const orc_params* p, const double* h0, double t,
+                   double* hy, double* hx, double* hz)
+{
+    if (check_params(p) || !h0) return -1;
+    const int N = p->N;
+    for (int ty = 0; ty < N; ++ty)
+        for (int tx = 0; tx < N; ++tx) {
+            double kx, ky, kmag;
+            wave_vector(p, tx, ty, &kx, &ky, &kmag);
+            double w = sqrt(p->gravity * kmag);                /* :79 */
+            size_t i = (size_t)ty * N + tx;
+            const double* a = h0 + i * 4;
+            double ar = a[0], ai = a[1], br = a[2], bi = a[3];
+            if (p->conj_mode == 1) bi = -bi;
+            double c = cos(w * t), s = sin(w * t);
+            double Hr = (ar * c - ai * s) + (br * c + bi * s);  /* :110 */
+            double Hi = (ar * s + ai * c) + (bi * c - br * s);
+            double mx = kx / kmag, mz = ky / kmag;
+            if (hy) { hy[2 * i] = Hr; hy[2 * i + 1] = Hi; }
+            if (hx) { hx[2 * i] = mx * Hi; hx[2 * i + 1] = -mx * Hr; }  /* :113-118 */
+            if (hz) { hz[2 * i] = mz * Hi; hz[2 * i + 1] = -mz * Hr; }  /* :121-126 */
+        }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ A10 */
+
+/* J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 (dDx/dz)(dDz/dx) for the world
+ * displacement (-l Dx, -l Dz) applied at render (grid_tes.glsl:60-62);
+ * derivatives are periodic central differences with spacing L/N. */
+void orc_jacobian(int N, double L, double lambda, const double* dx, const double* dz, double* out)
+{
+    const double inv2d = (double)N / (2.0 * L);
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            double dxdx = (tap_nearest(N, dx, x + 1, y) - tap_nearest(N, dx, x - 1, y)) * inv2d;
+            double dxdz = (tap_nearest(N, dx, x, y + 1) - tap_nearest(N, dx, x, y - 1)) * inv2d;
+            double dzdx = (tap_nearest(N, dz, x + 1, y) - tap_nearest(N, dz, x - 1, y)) * inv2d;
+            double dzdz = (tap_nearest(N, dz, x, y + 1) - tap_nearest(N, dz, x, y - 1)) * inv2d;
+            out[(size_t)y * N + x] =
+                (1.0 - lambda * dxdx) * (1.0 - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+        }
+}
+
+/* ------------------------------------------------------------------ frame */
+
+/* update() (main.cpp:240-244): tilde_h0_t, butterfly_fft for dy, dx, dz,
+ * generate_normal_map; plus the Jacobian. */
+int orc_frame(const orc_params* p, const double* h0, double t,
+              double* height, double* dx, double* dz, double* normals, double* jacobian,
+              int nthreads)
+{
+    if (check_params(p) || !h0) return -1;
+    const int N = p->N;
+    const size_t nn = (size_t)N * N;
+    double* tw  = (double*)malloc(sizeof(double) * 4 * (size_t)orc_log2n(N) * N);
+    double* f[3];
+    for (int i = 0; i < 3; ++i) f[i] = (double*)malloc(sizeof(double) * 2 * nn);
+    double* scratch = (double*)malloc(sizeof(double) * 2 * nn);
+    double* hbuf = (double*)malloc(sizeof(double) * nn);
+    double* xbuf = (double*)malloc(sizeof(double) * nn);
+    double* zbuf = (double*)malloc(sizeof(double) * nn);
+    int rc = -1;
+    if (!tw || !f[0] || !f[1] || !f[2] || !scratch || !hbuf || !xbuf || !zbuf) goto done;
+    orc_twiddle_factors(N, tw);
+    orc_spectrum_t(p, h0, t, f[0], f[1], f[2]);
+    double* outs[3] = {hbuf, xbuf, zbuf};
+    for (int i = 0; i < 3; ++i) {
+        orc_butterfly_fft2d(N, tw, f[i], scratch, nthreads);
+        orc_inversion(N, f[i], outs[i]);
+    }
+    if (height) memcpy(height, hbuf, sizeof(double) * nn);
+    if (dx) memcpy(dx, xbuf, sizeof(double) * nn);
+    if (dz) memcpy(dz, zbuf, sizeof(double) * nn);
+    if (normals) orc_normal_map(N, hbuf, p->normal_mode, normals);
+    if (jacobian) orc_jacobian(N, p->L, p->choppiness, xbuf, zbuf, jacobian);
+    rc = 0;
+done:
+    free(tw); free(f[0]); free(f[1]); free(f[2]); free(scratch);
+    free(hbuf); free(xbuf); free(zbuf);
+    return rc;
+}

@@ -1,0 +1,251 @@
+"""GPU parity: the HIP library (through the C ABI) against the FP64 oracle on
+identical seeded inputs.  Tolerances are in ocean_testutil (rel-L2 <= 1e-5,
+max-abs <= 1e-4 * max|field|) unless a test states its own with the reason."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import pyocean
+from ocean_testutil import (assert_field_close, both_params, c1_kwargs, c2_kwargs, max_abs, rel_l2)
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = pyocean.Hip()
+    assert h.device_count() >= 1, "no HIP device visible"
+    return h
+
+
+def _noise():
+    path = os.path.join(os.path.dirname(__file__), "golden", "noise_ldr_lll1_r.bin")
+    return np.fromfile(path, dtype=np.uint8).reshape(4, 256, 256)
+
+
+# ---------------------------------------------------------------- A3 tilde_h0_k
+
+@pytest.mark.parametrize("N", [64, 256, 1024])
+def test_h0_philox_matches_oracle(hip, N):
+    kw = dict(c1_kwargs(), N=N)
+    lp, op = both_params(**kw)
+    _say("h0 philox N=%d" % N)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        got = oc.download_h0()
+    ref = oracle.h0(op)
+    # FP32 Box-Muller/Phillips from bit-identical uniforms: a few ulp.
+    assert_field_close("h0", got, ref, rel_tol=2e-6, abs_rel_tol=1e-5)
+    # the DC texel is exactly zero (P(DC) = 0)
+    assert np.all(got[N // 2, N // 2] == 0.0)
+
+
+@pytest.mark.parametrize("N", [256, 512, 128])
+def test_h0_reference_noise_matches_oracle(hip, N):
+    kw = dict(N=N, L=1000.0, wind_speed=80.0, wind_dir=(1.0, 1.0), amplitude=2.0, suppression=0.1)
+    lp, op = both_params(**kw)
+    noise = _noise()
+    _say("h0 noise N=%d" % N)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k_from_noise(noise)
+        got = oc.download_h0()
+    ref = oracle.h0_from_noise(op, noise)
+    assert_field_close("h0(noise)", got, ref, rel_tol=2e-6, abs_rel_tol=1e-5)
+
+
+# ---------------------------------------------------------------- A4 tilde_h0_t
+
+@pytest.mark.parametrize("conj", [0, 1])
+@pytest.mark.parametrize("t", [0.0, 1.0, 9.99])
+def test_spectrum_t_matches_oracle(hip, conj, t):
+    kw = dict(N=256, L=30.0, wind_speed=12.0, wind_dir=(0.3, -1.0), amplitude=4.0, conj_mode=conj)
+    lp, op = both_params(**kw)
+    _say("spectrum_t conj=%d t=%g" % (conj, t))
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        h0 = oc.download_h0().astype(np.float64)
+        oc.tilde_h0_t(t)
+        got = oc.download_spectrum_t()
+    ref = oracle.spectrum_t(op, h0, t)
+    for name, g, r in zip(("hy", "hx", "hz"), got, ref):
+        assert_field_close(name, np.stack([g.real, g.imag]), np.stack([r.real, r.imag]), rel_tol=2e-6,
+                           abs_rel_tol=1e-5)
+
+
+# ---------------------------------------------------------------- A5+A6 butterfly_fft
+
+@pytest.mark.parametrize("N", [64, 128, 256, 512, 1024, 2048])
+def test_butterfly_fft_buffers_matches_oracle(hip, N):
+    rng = np.random.default_rng(N)
+    x = (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))).astype(np.complex64)
+    lp, _ = both_params(N=N)
+    _say("butterfly_fft N=%d" % N)
+    with pyocean.Ocean(lp) as oc:
+        src = hip.malloc(x.nbytes)
+        dst = hip.malloc(N * N * 4)
+        try:
+            hip.upload(src, x)
+            oc.butterfly_fft_buffers(src, dst)
+            hip.sync()
+            got = hip.download(dst, (N, N))
+        finally:
+            hip.free(src)
+            hip.free(dst)
+    ref = oracle.inversion(oracle.butterfly_fft2d(x.astype(np.complex128), nthreads=NTHREADS))
+    assert_field_close("ifft2+inversion", got, ref)
+
+
+# ---------------------------------------------------------------- A7 normals (+ Jacobian)
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_normal_map_matches_oracle_on_same_height(hip, mode):
+    kw = dict(c1_kwargs(), normal_mode=mode)
+    lp, op = both_params(**kw)
+    _say("normal map mode=%d" % mode)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.update(1.0)
+        f = oc.download_fields()
+    n_ref = oracle.normal_map(f["height"].astype(np.float64), mode)
+    assert_field_close("normals", f["normals"], n_ref)
+    j_ref = oracle.jacobian(op.N, op.L, op.choppiness, f["dx"].astype(np.float64), f["dz"].astype(np.float64))
+    assert_field_close("jacobian", f["jacobian"], j_ref)
+
+
+# ---------------------------------------------------------------- whole frame vs oracle
+
+FRAME_CASES = [
+    ("c1", c1_kwargs(), 1.0),
+    ("c2_t0.5", c2_kwargs(), 0.5),
+    ("c2_t9.98", c2_kwargs(), 599 / 60.0),
+    ("n64", dict(N=64, L=100.0, wind_speed=10.0, wind_dir=(1.0, 0.2), amplitude=3.0, choppiness=0.9), 2.0),
+    ("n128_textbook", dict(N=128, L=250.0, wind_speed=20.0, wind_dir=(-1.0, 0.5), amplitude=4.0,
+                           conj_mode=1), 3.0),
+    ("n512_cascade", dict(N=512, L=250.0, wind_speed=17.0, wind_dir=(0.1, 1.0), amplitude=4.0,
+                          choppiness=1.2, seed=43, ocean_id=1, cascade_id=1), 5.0),
+    ("n1024_small_patch", dict(N=1024, L=30.0, wind_speed=30.0, wind_dir=(1.0, -1.0), amplitude=4.0,
+                               choppiness=1.5, seed=106, ocean_id=63, cascade_id=2), 10.0),
+    ("n256_bilinear", dict(c1_kwargs(), normal_mode=1), 4.0),
+]
+
+
+def _frame_tol(name):
+    return name
+
+
+@pytest.mark.parametrize("name,kw,t", FRAME_CASES, ids=[c[0] for c in FRAME_CASES])
+def test_frame_matches_oracle(hip, name, kw, t):
+    lp, op = both_params(**kw)
+    _say("frame %s N=%d t=%g" % (name, op.N, t))
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.update(t)
+        got = oc.download_fields()
+    ref = oracle.frame(op, oracle.h0(op), t, nthreads=NTHREADS)
+    for fld in ("height", "dx", "dz"):
+        r, m = assert_field_close(fld, got[fld], ref[fld])
+        _say("  %s rel-L2 %.2e max-abs %.2e (max|ref| %.3e)" % (fld, r, m, np.abs(ref[fld]).max()))
+    r, m = assert_field_close("normals", got["normals"], ref["normals"])
+    _say("  normals rel-L2 %.2e max-abs %.2e" % (r, m))
+    r, m = assert_field_close("jacobian", got["jacobian"], ref["jacobian"])
+    _say("  jacobian rel-L2 %.2e max-abs %.2e" % (r, m))
+
+
+def test_unfused_passes_match_fused_update(hip):
+    """tilde_h0_t + 3 x butterfly_fft + generate_normal_map (the reference's pass
+    structure) against the fused update of the same frame."""
+    lp, op = both_params(**c1_kwargs())
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.update(1.5)
+        fused = oc.download_fields()
+        oc.tilde_h0_t(1.5)
+        for f in (pyocean.OCN_FIELD_DY, pyocean.OCN_FIELD_DX, pyocean.OCN_FIELD_DZ):
+            oc.butterfly_fft(f)
+        oc.generate_normal_map()
+        passes = oc.download_fields()
+    for fld in ("height", "dx", "dz", "normals", "jacobian"):
+        assert_field_close(fld, passes[fld], fused[fld])
+
+
+def test_update_frames_equals_per_frame_update(hip):
+    kw = dict(c2_kwargs(), N=256)
+    lp, _ = both_params(**kw)
+    N, F = 256, 5
+    times = np.arange(F) / 60.0 + 0.25
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        bufs = {k: hip.malloc(F * N * N * (16 if k == "normals" else 4))
+                for k in ("height", "dx", "dz", "normals", "jacobian")}
+        try:
+            fl = pyocean.OcnFields(bufs["height"], bufs["dx"], bufs["dz"], bufs["normals"], bufs["jacobian"])
+            oc.update_frames(times, fl)
+            hip.sync()
+            batch = {k: hip.download(v, (F, N, N, 4) if k == "normals" else (F, N, N)) for k, v in bufs.items()}
+        finally:
+            for v in bufs.values():
+                hip.free(v)
+        for f, t in enumerate(times):
+            oc.update(t)
+            one = oc.download_fields()
+            for k in batch:
+                np.testing.assert_array_equal(batch[k][f], one[k])
+
+
+# ---------------------------------------------------------------- full sizes: properties
+
+def _direct_points(op, h0, t, pts):
+    """h, dx, dz at selected texels by the direct centred inverse DFT of the
+    oracle spectrum (inversion_cs.glsl semantics), O(N^2) per point."""
+    hy, hx, hz = oracle.spectrum_t(op, h0, t)
+    N = op.N
+    n = np.arange(N)
+    out = []
+    for (x, y) in pts:
+        ex = np.exp(2j * np.pi * n * x / N)
+        ey = np.exp(2j * np.pi * n * y / N)
+        sgn = -1.0 if (x + y) & 1 else 1.0
+        vals = []
+        for f in (hy, hx, hz):
+            vals.append(sgn * np.real(ey @ f @ ex) / (N * N))
+        out.append(vals)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("N", [2048, 4096])
+def test_full_size_point_samples_and_parseval(hip, N):
+    kw = dict(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(0.5), math.sin(0.5)), amplitude=4.0,
+              choppiness=1.3)
+    lp, op = both_params(**kw)
+    t = 7.25
+    _say("full size N=%d" % N)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.update(t)
+        got = oc.download_fields()
+    h0 = oracle.h0(op)
+    rng = np.random.default_rng(N)
+    pts = [(int(a), int(b)) for a, b in rng.integers(0, N, size=(12, 2))] + [(0, 0), (N - 1, N - 1)]
+    ref = _direct_points(op, h0, t, pts)
+    for i, (x, y) in enumerate(pts):
+        for k, fld in enumerate(("height", "dx", "dz")):
+            scale = np.abs(got[fld]).max()
+            assert abs(got[fld][y, x] - ref[i, k]) <= 1e-4 * scale, (fld, x, y, got[fld][y, x], ref[i, k])
+    # Parseval: sum h^2 = sum |A|^2 / N^2 with A the Hermitian part of the height spectrum
+    hy, _, _ = oracle.spectrum_t(op, h0, t)
+    mirror = np.roll(np.roll(hy[::-1, ::-1], 1, axis=0), 1, axis=1)
+    A = 0.5 * (hy + np.conj(mirror))
+    e_ref = np.sum(np.abs(A) ** 2) / (N * N)
+    e_got = np.sum(got["height"].astype(np.float64) ** 2)
+    assert abs(e_got - e_ref) <= 1e-5 * e_ref, (e_got, e_ref)
