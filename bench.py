@@ -1,0 +1,209 @@
+"""Benchmark of the Tessendorf ocean path on one MI355X.
+
+Workload: the plain N=1024 ocean (L=2000 m, wind 25 m/s at 30 deg, A=4,
+lambda=1.3, l=L/2000, Philox seed 42), frames at t_f = f/60 s, each frame's
+height, Dx, Dz, normals and Jacobian written into its own output slice (600
+slices, 19.3 GiB, reused cyclically for more than 600 steps).  A step is one
+frame: spectrum evolution, three inverse 2-D FFTs with the inversion step,
+normals and Jacobian, all on the GPU with inputs resident in HBM.
+
+Prints one JSON line (metric, value in frames/s, roofline of the dominant
+kernel measured with HIP events, CPU baseline of the FP64 oracle).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+  python bench.py --mode copy      (device-to-device copy used to calibrate
+                                    rocprofv3 FETCH_SIZE / WRITE_SIZE)
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fft-ocean-waves_amd"))
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E
+FRAMES = 600
+N = 1024
+
+# Canonical (algorithmic) bytes per texel.  Whole frame: h0 pair in (16) +
+# one intermediate round trip of three complex FP32 fields (24 + 24) + height,
+# Dx, Dz (12) + normals (16) + Jacobian (4) = 96.  Per kernel of the fused
+# frame: columns = 16 + 24, rows = 24 + 12, normals = 16 + 4 (plus re-reading
+# height/Dx/Dz, which the canonical count does not credit).
+CANON_FRAME = 96
+CANON_STAGE = {"k_cols_fused": 40, "k_rows_fused": 36, "k_normals": 20}
+
+
+def workload_params():
+    import pyocean
+    th = math.radians(30.0)
+    return pyocean.make_params(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(th), math.sin(th)),
+                               amplitude=4.0, suppression=1.0, choppiness=1.3, seed=42)
+
+
+def cpu_baseline(seconds):
+    """The FP64 oracle (a scalar C port of the reference shaders) timed on
+    this host: whole N=1024 frames, one thread, until `seconds` have passed."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    th = math.radians(30.0)
+    op = oracle.make_params(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(th), math.sin(th)),
+                            amplitude=4.0, suppression=1.0, choppiness=1.3, seed=42)
+    h0 = oracle.h0(op)
+    oracle.frame(op, h0, 0.0, nthreads=1)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        oracle.frame(op, h0, n / 60.0, nthreads=1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": round(n / el, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": "%d whole N=1024 frames (spectrum, 3 radix-2 butterfly IFFTs + inversion, normals, "
+                      "Jacobian) of the FP64 oracle in %.1f s, 1 thread" % (n, el)}
+
+
+def run_copy(hip):
+    """Device-to-device copies of 256 MiB (rocprofv3 counter calibration)."""
+    nbytes = 256 << 20
+    a, b = hip.malloc(nbytes), hip.malloc(nbytes)
+    hip.memset(a, 1, nbytes)
+    for _ in range(5):
+        hip.ok(hip.rt.hipMemcpy(b, a, nbytes, hip.D2D), "hipMemcpy D2D")
+    hip.sync()
+    hip.free(a)
+    hip.free(b)
+    print(json.dumps({"mode": "copy", "bytes_per_copy": nbytes, "copies": 5}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--mode", choices=["frames", "copy"], default="frames")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the ocean path runs on a single MI355X); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import pyocean
+    hip = pyocean.Hip()
+    if hip.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    if args.mode == "copy":
+        run_copy(hip)
+        return
+
+    nn = N * N
+    slots = min(FRAMES, max(1, args.steps))
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    bufs = {k: hip.malloc(slots * nn * b) for k, b in sizes.items()}
+    stream = hip.stream()
+    oc = pyocean.Ocean(workload_params(), stream=stream)
+    oc.tilde_h0_k()
+    hip.stream_sync(stream)
+
+    def fields(slot):
+        return pyocean.OcnFields(*[bufs[k] + slot * nn * sizes[k] for k in ("height", "dx", "dz", "normals",
+                                                                            "jacobian")])
+
+    slot_fields = [fields(s) for s in range(slots)]
+
+    def frames(f0, count):
+        # consecutive frames go to consecutive slots; one ABI call per run of slots
+        f = f0
+        while f < f0 + count:
+            s = f % slots
+            n = min(slots - s, f0 + count - f)
+            oc.update_frames(np.arange(f, f + n, dtype=np.float64) / 60.0, slot_fields[s])
+            f += n
+
+    frames(0, args.warmup)
+    hip.stream_sync(stream)
+    hip.sync()
+
+    e0, e1 = hip.event(), hip.event()
+    t_wall = time.perf_counter()
+    hip.record(e0, stream)
+    frames(0, args.steps)
+    hip.record(e1, stream)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.sync()
+    t_wall = time.perf_counter() - t_wall
+    fps = args.steps / (ms / 1e3)
+
+    # Per-kernel durations (HIP events on the launch stream, each kernel
+    # launched alone back to back over the same slots).
+    stage_ms = {}
+    reps = max(50, min(args.steps, 1000))
+    for name, stage in (("k_cols_fused", pyocean.OCN_STAGE_COLUMNS), ("k_rows_fused", pyocean.OCN_STAGE_ROWS),
+                        ("k_normals", pyocean.OCN_STAGE_NORMALS)):
+        for f in range(10):
+            oc.update_stage(f / 60.0, slot_fields[f % slots], stage)
+        hip.record(e0, stream)
+        for f in range(reps):
+            oc.update_stage(f / 60.0, slot_fields[f % slots], stage)
+        hip.record(e1, stream)
+        stage_ms[name] = hip.elapsed_ms(e0, e1) / reps
+    dominant = max(stage_ms, key=stage_ms.get)
+    dom_bytes = CANON_STAGE[dominant] * nn
+    achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
+    frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
+
+    oc.close()
+    for v in bufs.values():
+        hip.free(v)
+    hip.stream_destroy(stream)
+
+    cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
+    out = {
+        "metric": "ocean-frames/s + HBM GB/s (% peak) at N=1024; batched & fwd+bwd vs plain on GPU",
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 6),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox4x32-10 seed 42 + Box-Muller h0, generated on the GPU)",
+        "config": {
+            "workload": "plain single ocean N=1024x1024, L=2000 m, wind 25 m/s at 30 deg, A=4, lambda=1.3, "
+                        "l=1 m, frames t=f/60 s; height/Dx/Dz/normals/Jacobian per frame into %d slices" % slots,
+            "N": N, "frames_in_sequence": FRAMES, "output_slices": slots,
+        },
+        "frame": {
+            "canonical_bytes_per_frame": CANON_FRAME * nn,
+            "canonical_GBps": round(frame_gbps, 1),
+            "hbm_frac": round(frame_gbps / HBM_PEAK_GBPS, 4),
+            "wall_s_timed_region": round(t_wall, 3),
+            "kernel_ms": {k: round(v, 5) for k, v in stage_ms.items()},
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": dom_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

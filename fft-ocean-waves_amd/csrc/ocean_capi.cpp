@@ -381,14 +381,22 @@ int ocn_generate_normal_map(ocn_handle* h, void* stream)
     return e == hipSuccess ? OCN_OK : hip_fail(e, "generate_normal_map launch");
 }
 
-int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
+int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, void* stream)
 {
     int rc = check_handle(h);
-    if (rc || (rc = check_fields(out)) || (rc = ready_for_frame(h, t)) || (rc = set_device(h))) return rc;
+    if (rc || (rc = check_fields(out)) || (rc = ready_for_frame(h, t))) return rc;
+    if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
+        return fail(OCN_EINVAL, "stage %d is not a combination of OCN_STAGE_* bits", stage);
+    if ((rc = set_device(h))) return rc;
     ocn::FrameArgs a{};
     frame_args(h, t, out, &a);
-    hipError_t e = ocn::launch_frame(h->p.N, a, (hipStream_t)stream);
+    hipError_t e = ocn::launch_frame(h->p.N, a, stage, (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "update launch");
+}
+
+int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
+{
+    return ocn_update_stage(h, t, out, OCN_STAGE_ALL, stream);
 }
 
 int ocn_update(ocn_handle* h, double t, void* stream)

@@ -43,7 +43,9 @@ struct FrameArgs {
 hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s);
 hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, float2* hz, const SpecConst& sc,
                              double t_rev, hipStream_t s);
-hipError_t launch_frame(int N, const FrameArgs& a, hipStream_t s);
+// stages: bit 0 = spectrum + column transforms, bit 1 = row transforms +
+// inversion, bit 2 = normal map + Jacobian (7 = whole frame).
+hipError_t launch_frame(int N, const FrameArgs& a, int stages, hipStream_t s);
 hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s);
 hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s);

@@ -351,15 +351,17 @@ struct Shape {
 };
 
 template <int N>
-static hipError_t launch_frame_n(const FrameArgs& a, hipStream_t s)
+static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
 {
     using S = Shape<N>;
     const int nblk1 = (N / 2) / S::CC + 1;  // half plane tx in [0, N/2]
-    hipLaunchKernelGGL((k_cols_fused<N, S::E, S::CC>), dim3(nblk1), dim3(S::CC * S::T), 0, s, a.h0, a.z1, a.z2,
-                       a.tw, a.sc, a.t_rev);
-    hipLaunchKernelGGL((k_rows_fused<N, S::E>), dim3(N / 2), dim3(3 * S::T), 0, s, a.z1, a.z2, a.height, a.dx, a.dz,
-                       a.tw);
-    if (a.normals || a.jacobian)
+    if (stages & 1)
+        hipLaunchKernelGGL((k_cols_fused<N, S::E, S::CC>), dim3(nblk1), dim3(S::CC * S::T), 0, s, a.h0, a.z1, a.z2,
+                           a.tw, a.sc, a.t_rev);
+    if (stages & 2)
+        hipLaunchKernelGGL((k_rows_fused<N, S::E>), dim3(N / 2), dim3(3 * S::T), 0, s, a.z1, a.z2, a.height, a.dx,
+                           a.dz, a.tw);
+    if ((stages & 4) && (a.normals || a.jacobian))
         hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4), dim3(256), 0, s, a.height, a.dx, a.dz, a.normals,
                            a.jacobian, N, a.normal_mode, a.lambda, a.inv2d);
     return hipGetLastError();
@@ -386,9 +388,9 @@ static hipError_t launch_ifft2_n(const float2* src, float2* tmp, float* out, con
     default: return hipErrorInvalidValue;           \
     }
 
-hipError_t launch_frame(int N, const FrameArgs& a, hipStream_t s)
+hipError_t launch_frame(int N, const FrameArgs& a, int stages, hipStream_t s)
 {
-#define CALL(n) launch_frame_n<n>(a, s)
+#define CALL(n) launch_frame_n<n>(a, stages, s)
     OCN_DISPATCH_N(N, CALL)
 #undef CALL
 }

@@ -20,6 +20,7 @@ OCN_OK, OCN_EINVAL, OCN_ENOMEM, OCN_EHIP, OCN_ESTATE, OCN_ENOTSUP = 0, -1, -2, -
 OCN_CONJ_REFERENCE, OCN_CONJ_TEXTBOOK = 0, 1
 OCN_NORMALS_NEAREST, OCN_NORMALS_BILINEAR = 0, 1
 OCN_FIELD_DY, OCN_FIELD_DX, OCN_FIELD_DZ = 0, 1, 2
+OCN_STAGE_COLUMNS, OCN_STAGE_ROWS, OCN_STAGE_NORMALS, OCN_STAGE_ALL = 1, 2, 4, 7
 
 # Every function include/ocean_c.h declares (checked by tests/test_capi.py).
 EXPORTS = [
@@ -27,7 +28,7 @@ EXPORTS = [
     "ocn_last_error", "ocn_status_string", "ocn_abi_version",
     "ocn_tilde_h0_k", "ocn_tilde_h0_k_from_noise", "ocn_generate_twiddle_factors",
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
-    "ocn_update", "ocn_update_into", "ocn_update_frames",
+    "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
     "ocn_get_fields", "ocn_get_spectrum",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
@@ -94,6 +95,7 @@ def lib():
             "ocn_generate_normal_map": [vp, vp],
             "ocn_update": [vp, dbl, vp],
             "ocn_update_into": [vp, dbl, F, vp],
+            "ocn_update_stage": [vp, dbl, F, i32, vp],
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_get_fields": [vp, F],
             "ocn_get_spectrum": [vp] + [ctypes.POINTER(vp)] * 4,
@@ -308,6 +310,9 @@ class Ocean:
 
     def update_into(self, t, fields):
         check(lib().ocn_update_into(self.h, float(t), ctypes.byref(fields), self.stream))
+
+    def update_stage(self, t, fields, stage):
+        check(lib().ocn_update_stage(self.h, float(t), ctypes.byref(fields), int(stage), self.stream))
 
     def update_frames(self, times, fields):
         ts = np.ascontiguousarray(times, dtype=np.float64)

@@ -167,6 +167,18 @@ int ocn_update(ocn_handle* h, double t, void* stream);
 /* Same into caller buffers (height, dx, dz required; normals, jacobian optional). */
 int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream);
 
+/* One kernel (or a subset) of the fused frame, for profiling: stage is an
+ * OR of OCN_STAGE_COLUMNS (spectrum + Hermitian fold + column transforms into
+ * the handle's intermediate), OCN_STAGE_ROWS (row transforms + inversion into
+ * height/dx/dz) and OCN_STAGE_NORMALS (normal map + Jacobian from
+ * height/dx/dz).  OCN_STAGE_ALL is ocn_update_into.  No reference counterpart:
+ * the reference's passes are the unfused entry points above. */
+#define OCN_STAGE_COLUMNS 1
+#define OCN_STAGE_ROWS    2
+#define OCN_STAGE_NORMALS 4
+#define OCN_STAGE_ALL     7
+int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, void* stream);
+
 /* nframes frames at times t_host[0..nframes) into consecutive output slices:
  * frame f writes out->height + f*N*N, ..., out->normals + f*N*N*4, ...
  * (the per-frame slices of a recorded sequence). */

@@ -1,0 +1,116 @@
+"""CPU tests of the C ABI (include/ocean_c.h, libocean.so): the library loads,
+exports every declared entry point, rejects bad arguments before any HIP call,
+and its host utilities (bit reversal, the reference-layout twiddle table)
+match the oracle.  No compute call needs a GPU here."""
+import ctypes
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import pyocean
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ocean_c.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\s*\*)\s+(ocn_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = pyocean.lib()
+    names = _declared()
+    assert len(names) >= 20
+    assert sorted(pyocean.EXPORTS) == names
+    for n in names:
+        assert hasattr(L, n), n
+    dyn = subprocess.run(["nm", "-D", "--defined-only", pyocean.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (ocn_\w+)", dyn))
+    assert set(names) <= exported
+    assert L.ocn_abi_version() == 1
+
+
+def test_params_default_matches_reference_members():
+    p = pyocean.default_params()
+    # main.cpp:1632-1646 and tilde_h0_k_cs.glsl:26
+    assert (p.N, p.L, p.wind_speed, p.amplitude, p.suppression, p.choppiness, p.gravity) == \
+        (256, 1000.0, 80.0, 2.0, 0.1, 0.75, 9.81)
+    assert (p.wind_dir_x, p.wind_dir_y) == (1.0, 1.0)
+    assert p.conj_mode == pyocean.OCN_CONJ_REFERENCE and p.normal_mode == pyocean.OCN_NORMALS_NEAREST
+
+
+def test_struct_layout_matches_header():
+    # the ctypes mirror of ocn_params / ocn_fields must match the C layout
+    assert ctypes.sizeof(pyocean.OcnParams) == 6 * 4 + 8 + 8 * 8
+    assert pyocean.OcnParams.seed.offset == 24 and pyocean.OcnParams.L.offset == 32
+    assert ctypes.sizeof(pyocean.OcnFields) == 5 * 8
+
+
+BAD = [
+    ("N", 100), ("N", 32), ("N", 8192), ("N", 0), ("L", 0.0), ("L", float("nan")), ("wind_speed", -1.0),
+    ("wind_speed", float("inf")), ("amplitude", -0.5), ("suppression", -1.0), ("choppiness", float("nan")),
+    ("gravity", 0.0), ("conj_mode", 2), ("normal_mode", 7),
+]
+
+
+@pytest.mark.parametrize("field,value", BAD)
+def test_create_rejects_bad_params_before_hip(field, value):
+    p = pyocean.make_params(N=256)
+    setattr(p, field, value)
+    h = ctypes.c_void_p()
+    rc = pyocean.lib().ocn_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == pyocean.OCN_EINVAL
+    assert h.value is None
+    assert len(pyocean.lib().ocn_last_error()) > 0
+
+
+def test_zero_wind_direction_rejected():
+    p = pyocean.make_params(N=256, wind_dir=(0.0, 0.0))
+    h = ctypes.c_void_p()
+    assert pyocean.lib().ocn_create(ctypes.byref(p), ctypes.byref(h)) == pyocean.OCN_EINVAL
+
+
+def test_null_handles_and_pointers_rejected():
+    L = pyocean.lib()
+    assert L.ocn_create(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_tilde_h0_k(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_tilde_h0_t(None, 1.0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_butterfly_fft(None, 0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_update(None, 0.0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_update_into(None, 0.0, None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_generate_normal_map(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_get_fields(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_destroy(None) == pyocean.OCN_OK
+    assert L.ocn_params_default(None) == pyocean.OCN_EINVAL
+    assert L.ocn_bit_reversed_indices(256, None) == pyocean.OCN_EINVAL
+    assert L.ocn_twiddle_factors(100, None) == pyocean.OCN_EINVAL
+    assert L.ocn_status_string(pyocean.OCN_ESTATE) == b"OCN_ESTATE"
+
+
+@pytest.mark.parametrize("N", [2, 8, 256, 1024, 4096])
+def test_bit_reversed_indices_match_oracle(N):
+    assert np.array_equal(pyocean.bit_reversed_indices(N), oracle.bit_reversed_indices(N))
+
+
+@pytest.mark.parametrize("N", [8, 256, 1024])
+def test_reference_twiddle_table_matches_oracle(N):
+    got = pyocean.twiddle_factors(N)
+    ref = oracle.twiddle_factors(N)
+    assert np.array_equal(got[..., 2:], ref[..., 2:].astype(np.float32))   # butterfly indices exact
+    assert np.abs(got[..., :2] - ref[..., :2]).max() <= 2.0 ** -24          # FP64 twiddles rounded once
+
+
+def test_python_binding_fails_loudly_without_library(tmp_path, monkeypatch):
+    import importlib
+    import pyocean as po
+    monkeypatch.setattr(po, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(po, "_lib", None)
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        po.lib()
+    importlib.reload(po)
