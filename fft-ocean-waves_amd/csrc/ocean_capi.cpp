@@ -1,15 +1,20 @@
 // ocean_capi.cpp -- the extern "C" boundary (include/ocean_c.h) over the
 // gfx950 kernels.  Host-side validation, the handle's device buffers, FFT
-// plans and launch sequencing.  Citations name the reference code each entry
-// point replaces (diharaw/fft-ocean-waves, src/main.cpp).
+// plans, schedules (with the autotuner) and launch sequencing.  Citations
+// name the reference code each entry point replaces (diharaw/fft-ocean-waves,
+// src/main.cpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ocean_c.h"
@@ -89,35 +94,65 @@ ocn::SpecConst spec_const(const ocn_params& p)
     ocn::SpecConst s{};
     const double n  = std::sqrt(p.wind_dir_x * p.wind_dir_x + p.wind_dir_y * p.wind_dir_y);
     const double Lp = p.wind_speed * p.wind_speed / p.gravity;
+    const double ks = 2.0 * kPi / p.L;
     s.N          = p.N;
-    s.kscale     = (float)(2.0 * kPi / p.L);
+    s.kscale     = (float)ks;
     s.Lp2        = (float)(Lp * Lp);
     s.l2         = (float)(p.suppression * p.suppression);
     s.A          = (float)p.amplitude;
     s.wx         = (float)(p.wind_dir_x / n);
     s.wy         = (float)(p.wind_dir_y / n);
+    s.g          = (float)p.gravity;
     s.key0       = (uint32_t)p.seed;
     s.key1       = (uint32_t)(p.seed >> 32);
     s.ocean_id   = p.ocean_id;
     s.cascade_id = p.cascade_id;
     s.conj_mode  = p.conj_mode;
-    s.kscale_d   = 2.0 * kPi / p.L;
-    s.g_d        = p.gravity;
+    s.k2_d       = ks * ks;
+    s.g2_d       = p.gravity * p.gravity;
     return s;
+}
+
+// Schedules the autotuner found, per (device, N, frames-per-launch class).
+std::mutex                                          g_sched_mu;
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_sched;
+
+int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn::kMaxFrames; }
+
+// Built-in schedule: the fastest (column variant, row variant) measured on
+// MI355X for one frame per launch; the autotuner may replace it.
+std::pair<int, int> default_schedule(int N)
+{
+    int cv;
+    switch (N) {
+    case 64: cv = 1; break;    // radix 16, 2 columns
+    case 128: cv = 2; break;   // radix 16, 4 columns
+    case 256: cv = 3; break;   // radix 16, 8 columns
+    case 512: cv = 3; break;
+    case 1024: cv = 3; break;
+    case 2048: cv = 2; break;
+    default: cv = 1; break;    // 4096
+    }
+    if (!ocn::schedule_valid(N, cv, 0)) cv = 0;
+    return {cv, 0};
 }
 
 }  // namespace
 
 struct ocn_handle {
     ocn_params p{};
-    int        device = 0;
+    int        device   = 0;
     bool       h0_ready = false;
-    float4*    h0 = nullptr;      // N*N
-    float2*    z1 = nullptr;      // N*N: fused-pass intermediate / butterfly scratch
-    float2*    z2 = nullptr;      // N*N/2
-    float2*    tw = nullptr;      // Stockham twiddles
+    int        zcap     = 0;        // frames the intermediate buffers hold
+    int        cols_variant = 0;
+    int        rows_variant = 0;
+    bool       tuned = false;       // schedule set by ocn_autotune / ocn_set_schedule
+    float4*    h0 = nullptr;        // N*N
+    float2*    z1 = nullptr;        // zcap*N*N: intermediate; frame 0 doubles as butterfly scratch
+    float2*    z2 = nullptr;        // zcap*N*N/2
+    float2*    tw[3] = {nullptr, nullptr, nullptr};  // Stockham twiddles, radix 4 / 8 / 16
     float2*    spec[3] = {nullptr, nullptr, nullptr};  // tilde_h0_t textures (lazy)
-    uint8_t*   noise = nullptr;   // 4*256*256 (lazy)
+    uint8_t*   noise = nullptr;     // 4*256*256 (lazy)
     float*     height = nullptr;
     float*     dx = nullptr;
     float*     dz = nullptr;
@@ -129,8 +164,8 @@ namespace {
 
 void free_all(ocn_handle* h)
 {
-    void* ptrs[] = {h->h0, h->z1, h->z2, h->tw, h->spec[0], h->spec[1], h->spec[2], h->noise,
-                    h->height, h->dx, h->dz, h->jac, h->normals};
+    void* ptrs[] = {h->h0, h->z1, h->z2, h->tw[0], h->tw[1], h->tw[2], h->spec[0], h->spec[1], h->spec[2],
+                    h->noise, h->height, h->dx, h->dz, h->jac, h->normals};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
 }
@@ -160,35 +195,84 @@ int dalloc(T** p, size_t count, const char* what)
 
 int upload_twiddles(ocn_handle* h)
 {
-    std::vector<float> t = stockham_twiddles(h->p.N, ocn::fft_radix());
-    if (h->tw) (void)hipFree(h->tw);
-    h->tw = nullptr;
-    int rc = dalloc(&h->tw, t.size() / 2 + 1, "twiddles");
-    if (rc) return rc;
-    if (!t.empty()) {
-        hipError_t e = hipMemcpy(h->tw, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles)");
+    const int radices[3] = {4, 8, 16};
+    for (int i = 0; i < 3; ++i) {
+        std::vector<float> t = stockham_twiddles(h->p.N, radices[i]);
+        if (h->tw[i]) (void)hipFree(h->tw[i]);
+        h->tw[i] = nullptr;
+        int rc = dalloc(&h->tw[i], t.size() / 2 + 1, "twiddles");
+        if (rc) return rc;
+        if (!t.empty()) {
+            hipError_t e = hipMemcpy(h->tw[i], t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpy(twiddles)");
+        }
     }
     return OCN_OK;
 }
 
-int frame_args(const ocn_handle* h, double t, const ocn_fields* out, ocn::FrameArgs* a)
+// Grows the intermediate buffers to hold `frames` frames (stream-ordered
+// users are finished first: hipFree synchronises).
+int ensure_zcap(ocn_handle* h, int frames)
 {
-    a->h0          = h->h0;
-    a->z1          = h->z1;
-    a->z2          = h->z2;
-    a->tw          = h->tw;
-    a->sc          = spec_const(h->p);
-    a->t_rev       = t / (2.0 * kPi);
-    a->height      = out->height;
-    a->dx          = out->dx;
-    a->dz          = out->dz;
-    a->normals     = reinterpret_cast<float4*>(out->normals);
-    a->jacobian    = out->jacobian;
-    a->normal_mode = h->p.normal_mode;
-    a->lambda      = (float)h->p.choppiness;
-    a->inv2d       = (float)((double)h->p.N / (2.0 * h->p.L));
+    if (frames <= h->zcap) return OCN_OK;
+    const size_t nn = (size_t)h->p.N * h->p.N;
+    float2 *z1 = nullptr, *z2 = nullptr;
+    int rc;
+    if ((rc = dalloc(&z1, nn * frames, "z1"))) return rc;
+    if ((rc = dalloc(&z2, nn / 2 * frames, "z2"))) {
+        (void)hipFree(z1);
+        return rc;
+    }
+    if (h->z1) (void)hipFree(h->z1);
+    if (h->z2) (void)hipFree(h->z2);
+    h->z1   = z1;
+    h->z2   = z2;
+    h->zcap = frames;
     return OCN_OK;
+}
+
+void pick_schedule(ocn_handle* h, int nframes, int* cv, int* rv)
+{
+    if (h->tuned) {
+        *cv = h->cols_variant;
+        *rv = h->rows_variant;
+        return;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        auto it = g_sched.find({h->device, h->p.N, frames_class(nframes)});
+        if (it != g_sched.end()) {
+            *cv = it->second.first;
+            *rv = it->second.second;
+            return;
+        }
+    }
+    *cv = h->cols_variant;
+    *rv = h->rows_variant;
+}
+
+void frame_args(const ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride,
+                ocn::FrameArgs* a)
+{
+    const size_t nn = (size_t)h->p.N * h->p.N;
+    a->h0 = h->h0;
+    a->z1 = h->z1;
+    a->z2 = h->z2;
+    for (int i = 0; i < 3; ++i) a->tw[i] = h->tw[i];
+    a->sc = spec_const(h->p);
+    for (int f = 0; f < nframes; ++f) a->fb.t_rev[f] = t[f] / (2.0 * kPi);
+    a->fb.z1_stride  = nn;
+    a->fb.z2_stride  = nn / 2;
+    a->fb.out_stride = out_stride;
+    a->nframes       = nframes;
+    a->height        = out->height;
+    a->dx            = out->dx;
+    a->dz            = out->dz;
+    a->normals       = reinterpret_cast<float4*>(out->normals);
+    a->jacobian      = out->jacobian;
+    a->normal_mode   = h->p.normal_mode;
+    a->lambda        = (float)h->p.choppiness;
+    a->inv2d         = (float)((double)h->p.N / (2.0 * h->p.L));
 }
 
 int ready_for_frame(const ocn_handle* h, double t)
@@ -203,6 +287,20 @@ int check_fields(const ocn_fields* out)
     if (!out) return fail(OCN_EINVAL, "fields is NULL");
     if (!out->height || !out->dx || !out->dz) return fail(OCN_EINVAL, "height, dx and dz are required");
     return OCN_OK;
+}
+
+// Launches `nframes` (<= kMaxFrames) frames with the given stages.
+int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride, int stages,
+               int cv, int rv, hipStream_t s)
+{
+    int rc;
+    if ((rc = ensure_zcap(h, nframes))) return rc;
+    ocn::FrameArgs a{};
+    frame_args(h, t, nframes, out, out_stride, &a);
+    a.cols_variant = cv;
+    a.rows_variant = rv;
+    hipError_t e = ocn::launch_frame(h->p.N, a, stages, s);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "update launch");
 }
 
 }  // namespace
@@ -261,11 +359,14 @@ int ocn_create(const ocn_params* p, ocn_handle** out)
         delete h;
         return hip_fail(e, "hipGetDevice");
     }
+    const auto sch = default_schedule(p->N);
+    h->cols_variant = sch.first;
+    h->rows_variant = sch.second;
     const size_t nn = (size_t)p->N * p->N;
-    if ((rc = dalloc(&h->h0, nn, "h0")) || (rc = dalloc(&h->z1, nn, "z1")) || (rc = dalloc(&h->z2, nn / 2, "z2")) ||
-        (rc = dalloc(&h->height, nn, "height")) || (rc = dalloc(&h->dx, nn, "dx")) ||
-        (rc = dalloc(&h->dz, nn, "dz")) || (rc = dalloc(&h->jac, nn, "jacobian")) ||
-        (rc = dalloc(&h->normals, nn, "normals")) || (rc = upload_twiddles(h))) {
+    if ((rc = dalloc(&h->h0, nn, "h0")) || (rc = ensure_zcap(h, 1)) || (rc = dalloc(&h->height, nn, "height")) ||
+        (rc = dalloc(&h->dx, nn, "dx")) || (rc = dalloc(&h->dz, nn, "dz")) ||
+        (rc = dalloc(&h->jac, nn, "jacobian")) || (rc = dalloc(&h->normals, nn, "normals")) ||
+        (rc = upload_twiddles(h))) {
         free_all(h);
         delete h;
         return rc;
@@ -354,8 +455,11 @@ int ocn_butterfly_fft_buffers(ocn_handle* h, const void* src, float* dst, void* 
     int rc = check_handle(h);
     if (rc) return rc;
     if (!src || !dst) return fail(OCN_EINVAL, "src and dst are required");
-    if (src == (const void*)h->z1 || (const void*)dst == (const void*)h->z1)
-        return fail(OCN_EINVAL, "src/dst alias the handle's scratch");
+    const size_t zbytes = sizeof(float2) * (size_t)h->p.N * h->p.N * h->zcap;
+    auto in_z1 = [&](const void* q) {
+        return (const char*)q >= (const char*)h->z1 && (const char*)q < (const char*)h->z1 + zbytes;
+    };
+    if (in_z1(src) || in_z1(dst)) return fail(OCN_EINVAL, "src/dst alias the handle's scratch");
     if ((rc = set_device(h))) return rc;
     hipError_t e = ocn::launch_ifft2(h->p.N, (const float2*)src, h->z1, dst, h->tw, (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "butterfly_fft launch");
@@ -388,10 +492,9 @@ int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, 
     if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
         return fail(OCN_EINVAL, "stage %d is not a combination of OCN_STAGE_* bits", stage);
     if ((rc = set_device(h))) return rc;
-    ocn::FrameArgs a{};
-    frame_args(h, t, out, &a);
-    hipError_t e = ocn::launch_frame(h->p.N, a, stage, (hipStream_t)stream);
-    return e == hipSuccess ? OCN_OK : hip_fail(e, "update launch");
+    int cv, rv;
+    pick_schedule(h, 1, &cv, &rv);
+    return run_frames(h, &t, 1, out, 0, stage, cv, rv, (hipStream_t)stream);
 }
 
 int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
@@ -414,13 +517,101 @@ int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const oc
     if (!t_host || nframes < 0) return fail(OCN_EINVAL, "t_host must be non-NULL and nframes >= 0");
     for (int f = 0; f < nframes; ++f)
         if ((rc = ready_for_frame(h, t_host[f]))) return rc;
+    if (nframes == 0) return OCN_OK;
+    if ((rc = set_device(h))) return rc;
     const size_t nn = (size_t)h->p.N * h->p.N;
-    for (int f = 0; f < nframes; ++f) {
-        ocn_fields s{out->height + f * nn, out->dx + f * nn, out->dz + f * nn,
-                     out->normals ? out->normals + f * nn * 4 : nullptr,
-                     out->jacobian ? out->jacobian + f * nn : nullptr};
-        if ((rc = ocn_update_into(h, t_host[f], &s, stream))) return rc;
+    int cv, rv;
+    pick_schedule(h, nframes, &cv, &rv);
+    for (int f0 = 0; f0 < nframes; f0 += ocn::kMaxFrames) {
+        const int nf = std::min(ocn::kMaxFrames, nframes - f0);
+        ocn_fields s{out->height + f0 * nn, out->dx + f0 * nn, out->dz + f0 * nn,
+                     out->normals ? out->normals + f0 * nn * 4 : nullptr,
+                     out->jacobian ? out->jacobian + f0 * nn : nullptr};
+        if ((rc = run_frames(h, t_host + f0, nf, &s, nn, OCN_STAGE_ALL, cv, rv, (hipStream_t)stream))) return rc;
     }
+    return OCN_OK;
+}
+
+int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!ocn::schedule_valid(h->p.N, cols_variant, rows_variant))
+        return fail(OCN_EINVAL, "schedule (%d, %d) is not available for N=%d", cols_variant, rows_variant, h->p.N);
+    h->cols_variant = cols_variant;
+    h->rows_variant = rows_variant;
+    h->tuned        = true;
+    return OCN_OK;
+}
+
+int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* rows_variant, int* cols_radix,
+                     int* cols_per_block, int* rows_radix)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    int cv, rv;
+    pick_schedule(const_cast<ocn_handle*>(h), nframes, &cv, &rv);
+    int E, C;
+    ocn::cols_variant_shape(cv, &E, &C);
+    if (cols_variant) *cols_variant = cv;
+    if (rows_variant) *rows_variant = rv;
+    if (cols_radix) *cols_radix = E;
+    if (cols_per_block) *cols_per_block = C;
+    if (rows_radix) *rows_radix = ocn::rows_variant_radix(rv);
+    return OCN_OK;
+}
+
+int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (nframes < 1 || nframes > ocn::kMaxFrames || reps < 1 || reps > 10000)
+        return fail(OCN_EINVAL, "nframes must be in [1, %d] and reps in [1, 10000]", ocn::kMaxFrames);
+    if ((rc = set_device(h)) || (rc = ensure_zcap(h, nframes))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t nn = (size_t)h->p.N * h->p.N;
+    float* outb = nullptr;
+    if ((rc = dalloc(&outb, nn * 3 * nframes, "autotune outputs"))) return rc;
+    ocn_fields out{outb, outb + nn * nframes, outb + 2 * nn * nframes, nullptr, nullptr};
+    std::vector<double> t(nframes);
+    for (int f = 0; f < nframes; ++f) t[f] = 0.5 + f / 60.0;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    float best = 1e30f;
+    int   bc = -1, br = -1;
+    for (int cv = 0; cv < ocn::kNumColsVariants && rc == OCN_OK; ++cv)
+        for (int rv = 0; rv < ocn::kNumRowsVariants && rc == OCN_OK; ++rv) {
+            if (!ocn::schedule_valid(h->p.N, cv, rv)) continue;
+            const int stages = OCN_STAGE_COLUMNS | OCN_STAGE_ROWS;
+            if ((rc = run_frames(h, t.data(), nframes, &out, nn, stages, cv, rv, s))) break;  // warm-up
+            (void)hipEventRecord(e0, s);
+            for (int i = 0; i < reps && rc == OCN_OK; ++i)
+                rc = run_frames(h, t.data(), nframes, &out, nn, stages, cv, rv, s);
+            (void)hipEventRecord(e1, s);
+            hipError_t e = hipEventSynchronize(e1);
+            if (e != hipSuccess) {
+                rc = hip_fail(e, "autotune");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                bc   = cv;
+                br   = rv;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(outb);
+    if (rc) return rc;
+    if (bc < 0) return fail(OCN_ENOTSUP, "no schedule available for N=%d", h->p.N);
+    {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        g_sched[{h->device, h->p.N, frames_class(nframes)}] = {bc, br};
+    }
+    h->tuned = false;  // use the tuned table entry for this frame count class
     return OCN_OK;
 }
 

@@ -3,9 +3,14 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace ocn {
+
+constexpr int kMaxFrames        = 32;  // frames per launch of the fused path
+constexpr int kNumColsVariants  = 7;   // column-pass (radix, columns/block) schedules
+constexpr int kNumRowsVariants  = 2;   // row-pass radix schedules
 
 // Per-ocean spectrum constants, FP32 for the kernels plus the FP64 pieces of
 // the phase.  Filled by the host from ocn_params.
@@ -16,20 +21,33 @@ struct SpecConst {
     float    l2;          // suppression^2
     float    A;           // Phillips amplitude
     float    wx, wy;      // unit wind direction
+    float    g;           // gravity
     uint32_t key0, key1;  // Philox key (seed)
     uint32_t ocean_id, cascade_id;
     int      conj_mode;   // 0: reference literal (identity), 1: textbook conj
-    double   kscale_d;    // 2 pi / L
-    double   g_d;         // gravity
+    double   k2_d;        // (2 pi / L)^2
+    double   g2_d;        // g^2
+};
+
+// Frames of one launch: times and the element strides between frames of the
+// intermediate and output buffers.
+struct FrameBatch {
+    double t_rev[kMaxFrames];  // t / (2 pi) per frame
+    size_t z1_stride;          // float2 elements
+    size_t z2_stride;          // float2 elements
+    size_t out_stride;         // float elements of height/dx/dz/jacobian (normals: 4x)
 };
 
 struct FrameArgs {
     const float4* h0;
-    float2*       z1;         // N x N
-    float2*       z2;         // N/2 x N
-    const float2* tw;         // Stockham twiddles of the (N, radix) plan
+    float2*       z1;         // nframes x N x N
+    float2*       z2;         // nframes x N/2 x N
+    const float2* tw[3];      // Stockham twiddles for radix 4, 8, 16
     SpecConst     sc;
-    double        t_rev;      // t / (2 pi)
+    FrameBatch    fb;
+    int           nframes;
+    int           cols_variant;
+    int           rows_variant;
     float*        height;
     float*        dx;
     float*        dz;
@@ -46,9 +64,11 @@ hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, fl
 // stages: bit 0 = spectrum + column transforms, bit 1 = row transforms +
 // inversion, bit 2 = normal map + Jacobian (7 = whole frame).
 hipError_t launch_frame(int N, const FrameArgs& a, int stages, hipStream_t s);
-hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s);
+hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* const* tw, hipStream_t s);
 hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s);
-int        fft_radix();
+bool       schedule_valid(int N, int cols_variant, int rows_variant);
+void       cols_variant_shape(int v, int* E, int* C);
+int        rows_variant_radix(int v);
 
 }  // namespace ocn

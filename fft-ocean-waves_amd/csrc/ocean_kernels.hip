@@ -6,18 +6,18 @@
 //   butterfly_fft      (main.cpp:612-683, butterfly_cs.glsl,
 //                       inversion_cs.glsl)                      -> k_c2c_rows + k_c2c_cols_inv (pass API)
 //   generate_normal_map(main.cpp:687-707, normal_map_cs.glsl)   -> k_normals
-//   update() frame     (main.cpp:240-244)                       -> k_cols_fused + k_rows_fused + k_normals
+//   update() frame     (main.cpp:240-244)                       -> k_cols + k_rows + k_normals
 //
 // Fused frame data flow (per texel, FP32):
 //   h0 (float4: h0k, h0mk) --k_cols_fused--> Z1 (N x N float2), Z2 (N/2 x N float2)
-//                          --k_rows_fused--> height, dx, dz (float)
+//                          --k_rows--> height, dx, dz (float)
 //                          --k_normals-----> normals (float4), Jacobian (float)
-// k_cols_fused evaluates the spectrum at texel t and at its mirror t' = (-t mod N),
+// k_cols evaluates the spectrum at texel t and at its mirror t' = (-t mod N),
 // folds each field F into its Hermitian part A = (F(t) + conj F(t'))/2 (whose
 // inverse DFT is Re of F's inverse DFT, which is all inversion_cs.glsl keeps),
 // and runs the column (ty) transforms for the half plane tx in [0, N/2].  It
 // stores the packed rows Z1 = G_h + i G_dx (full width via G(-tx) = conj G(tx))
-// and Z2 = G_dz(y) + i G_dz(y + N/2).  k_rows_fused runs the row (tx)
+// and Z2 = G_dz(y) + i G_dz(y + N/2).  k_rows runs the row (tx)
 // transforms: Re/Im of each result are two real outputs at once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -112,28 +112,29 @@ __global__ __launch_bounds__(256) void k_tilde_h0_k_noise(float4* __restrict__ h
 
 // ------------------------------------------------------------------ A4 time evolution
 
-// h(k, t) = h0k e^{i w t} + C(h0mk) e^{-i w t}, w = sqrt(g |k|_clamped)
-// (tilde_h0_t_cs.glsl:70-110; C = identity in the reference, :42-48).
-// The phase is formed in FP64 and reduced to one turn before the FP32
-// sincos: w t reaches hundreds of radians, where FP32 rounding of w alone
-// would cost ~1e-5 rad.  Also returns the choppy multipliers kx/|k|, ky/|k|.
-__device__ __forceinline__ float2 evolve(const SpecConst& sc, float4 a, int tx, int ty, double t_rev,
-                                         float& mx, float& mz)
+// Dispersion phase w t / (2 pi) with w = sqrt(g |k|_clamped) (tilde_h0_t_cs.glsl:77-79),
+// as a fraction of a turn in [-1/2, 1/2].  w t reaches hundreds of radians,
+// where FP32 rounding of w alone would cost ~1e-5 rad, so w is refined in FP64:
+// an FP32 estimate w0 plus one Newton step on w^4 = g^2 |k|^2 (|k|^2 is exact
+// from the integer wave numbers), then the product with t / (2 pi) is reduced
+// to one turn before the FP32 sincos.  m = nx^2 + ny^2.
+__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
 {
-    const int nx = tx - sc.N / 2, ny = ty - sc.N / 2;
-    double kmag = sc.kscale_d * sqrt((double)(nx * nx + ny * ny));
-    if (kmag < 0.00001) kmag = 0.00001;
-    const double rev = sqrt(sc.g_d * kmag) * t_rev;  // w t / (2 pi)
-    const float  fr  = (float)(rev - rint(rev));
-    float s, c;
-    sincospif(2.0f * fr, &s, &c);
-    const float kinv = (float)(1.0 / kmag);
-    mx = sc.kscale * (float)nx * kinv;
-    mz = sc.kscale * (float)ny * kinv;
-    const float bi = sc.conj_mode ? -a.w : a.w;
-    // a*(c + i s) + b*(c - i s)
-    return make_float2(fmaf(a.x, c, -a.y * s) + fmaf(a.z, c, bi * s),
-                       fmaf(a.x, s, a.y * c) + fmaf(bi, c, -a.z * s));
+    kf = fmaxf(sc.kscale * sqrtf((float)m), 0.00001f);
+    const float  w0 = sqrtf(sc.g * kf);
+    const double Q  = fmax(sc.k2_d * (double)m, 1e-10) * sc.g2_d;  // g^2 |k|_clamped^2
+    const double w  = (double)w0;
+    const double w2 = w * w;
+    const double w1 = w - (w2 * w2 - Q) * (double)(0.25f / (w0 * w0 * w0));
+    const double rev = w1 * t_rev;
+    return (float)(rev - rint(rev));
+}
+
+// a e^{i th} + C(b) e^{-i th} for h = (a, b) = (h0k, h0mk) (tilde_h0_t_cs.glsl:94-110).
+__device__ __forceinline__ float2 evolve_amp(const SpecConst& sc, float4 h, float s, float c)
+{
+    const float bi = sc.conj_mode ? -h.w : h.w;
+    return make_float2(fmaf(h.x, c, -h.y * s) + fmaf(h.z, c, bi * s), fmaf(h.x, s, h.y * c) + fmaf(bi, c, -h.z * s));
 }
 
 // -i m * H  (tilde_h0_t_cs.glsl:113-126)
@@ -147,32 +148,63 @@ __global__ __launch_bounds__(256) void k_tilde_h0_t(const float4* __restrict__ h
     const int tx = blockIdx.x * blockDim.x + threadIdx.x;
     const int ty = blockIdx.y;
     if (tx >= sc.N) return;
-    const size_t i = (size_t)ty * sc.N + tx;
-    float mx, mz;
-    const float2 h = evolve(sc, h0[i], tx, ty, t_rev, mx, mz);
+    const size_t i  = (size_t)ty * sc.N + tx;
+    const int    nx = tx - sc.N / 2, ny = ty - sc.N / 2;
+    float kf, s, c;
+    const float fr = phase_turns(sc, nx * nx + ny * ny, t_rev, kf);
+    sincospif(2.0f * fr, &s, &c);
+    const float2 h = evolve_amp(sc, h0[i], s, c);
     hy[i] = h;
-    hx[i] = choppy(mx, h);
-    hz[i] = choppy(mz, h);
+    hx[i] = choppy(sc.kscale * (float)nx / kf, h);
+    hz[i] = choppy(sc.kscale * (float)ny / kf, h);
+}
+
+// Hermitian parts (F(t) + conj F(t'))/2 of the three spectra at texel t and its
+// mirror t' = -t mod N.  Both share |k| (n' = -n except on the Nyquist row or
+// column, where n' = n = -N/2), hence one phase and one sincos for the pair.
+// p = h0 at t, q = h0 at t'; (nx, ny) and (qx, qy) are their centred indices.
+__device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
+                                          double t_rev, float2& ay, float2& ax, float2& az)
+{
+    float kf, s, c;
+    const float fr = phase_turns(sc, nx * nx + ny * ny, t_rev, kf);
+    sincospif(2.0f * fr, &s, &c);
+    const float  kinv = sc.kscale / kf;
+    const float2 H = evolve_amp(sc, p, s, c);
+    const float2 G = evolve_amp(sc, q, s, c);
+    const float mx = (float)nx * kinv, mz = (float)ny * kinv;
+    const float gx = (float)qx * kinv, gz = (float)qy * kinv;
+    ay = make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
+    // -i m H = (m H.y, -m H.x);  conj(-i g G) = (g G.y, g G.x)
+    ax = make_float2(0.5f * (mx * H.y + gx * G.y), 0.5f * (gx * G.x - mx * H.x));
+    az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
 }
 
 // ------------------------------------------------------------------ fused frame, pass 1
 
-// Column pass of one frame.  Block = C columns x T threads; thread (c, j)
-// transforms column tx = blockIdx.x*C + c of the half plane (tx <= N/2), rows
-// ty = j + r*T.  Three transforms (height, Dx, Dz) share the column's LDS buffer.
+// Column pass.  Grid = (column strips, frames).  Block = C columns x T threads;
+// thread (c, j) transforms column tx = blockIdx.x*C + c of the half plane
+// (tx <= N/2), rows ty = j + r*T; consecutive threads read consecutive columns,
+// so a wave's loads cover 64/C rows of C adjacent texels.  The three
+// transforms (height, Dx, Dz) share the column's LDS buffer.
 template <int N, int E, int C>
-__global__ __launch_bounds__(C*(N / E)) void k_cols_fused(const float4* __restrict__ h0, float2* __restrict__ z1,
-                                                          float2* __restrict__ z2, const float2* __restrict__ tw,
-                                                          SpecConst sc, double t_rev)
+__global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h0, float2* __restrict__ z1,
+                                                    float2* __restrict__ z2, const float2* __restrict__ tw,
+                                                    SpecConst sc, FrameBatch fb)
 {
     constexpr int T = N / E;
     __shared__ float2 lds[C * lds_len(N)];
+    const int  f     = blockIdx.y;
+    const double t_rev = fb.t_rev[f];
+    z1 += (size_t)f * fb.z1_stride;
+    z2 += (size_t)f * fb.z2_stride;
     const int  c     = threadIdx.x % C;
     const int  j     = threadIdx.x / C;
     const int  tx    = blockIdx.x * C + c;
     const bool valid = tx <= N / 2;
     const int  txc   = valid ? tx : N / 2;
     const int  mxc   = (N - txc) & (N - 1);
+    const int  nx = txc - N / 2, qx = mxc - N / 2;
 
     float2 ay[E], ax[E], az[E];
 #pragma unroll
@@ -181,15 +213,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols_fused(const float4* __restri
         const int my = (N - ty) & (N - 1);
         const float4 p = h0[(size_t)ty * N + txc];
         const float4 q = h0[(size_t)my * N + mxc];
-        float pmx, pmz, qmx, qmz;
-        const float2 hp = evolve(sc, p, txc, ty, t_rev, pmx, pmz);
-        const float2 hq = evolve(sc, q, mxc, my, t_rev, qmx, qmz);
-        // Hermitian part of each field: (F(t) + conj F(t')) / 2
-        ay[r] = make_float2(0.5f * (hp.x + hq.x), 0.5f * (hp.y - hq.y));
-        const float2 xp = choppy(pmx, hp), xq = choppy(qmx, hq);
-        ax[r] = make_float2(0.5f * (xp.x + xq.x), 0.5f * (xp.y - xq.y));
-        const float2 zp = choppy(pmz, hp), zq = choppy(qmz, hq);
-        az[r] = make_float2(0.5f * (zp.x + zq.x), 0.5f * (zp.y - zq.y));
+        pair_fold(sc, p, q, nx, ty - N / 2, qx, my - N / 2, t_rev, ay[r], ax[r], az[r]);
     }
 
     float2* buf = lds + c * lds_len(N);
@@ -216,17 +240,24 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols_fused(const float4* __restri
 
 // ------------------------------------------------------------------ fused frame, pass 2
 
-// Row pass.  Block y handles rows y and y + N/2: group 0 transforms Z1 row y,
-// group 1 Z1 row y + N/2, group 2 Z2 row y, then applies the inversion step
-// (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).
+// Row pass.  Grid = (N/2, frames).  Block y handles rows y and y + N/2: group 0
+// transforms Z1 row y, group 1 Z1 row y + N/2, group 2 Z2 row y, then applies
+// the inversion step (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).
 template <int N, int E>
-__global__ __launch_bounds__(3 * (N / E)) void k_rows_fused(const float2* __restrict__ z1,
-                                                            const float2* __restrict__ z2, float* __restrict__ height,
-                                                            float* __restrict__ dx, float* __restrict__ dz,
-                                                            const float2* __restrict__ tw)
+__global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__ z1, const float2* __restrict__ z2,
+                                                      float* __restrict__ height, float* __restrict__ dx,
+                                                      float* __restrict__ dz, const float2* __restrict__ tw,
+                                                      FrameBatch fb)
 {
     constexpr int T = N / E;
     __shared__ float2 lds[3 * lds_len(N)];
+    const int f = blockIdx.y;
+    z1 += (size_t)f * fb.z1_stride;
+    z2 += (size_t)f * fb.z2_stride;
+    const size_t o = (size_t)f * fb.out_stride;
+    height += o;
+    dx += o;
+    dz += o;
     const int g = threadIdx.x / T;
     const int j = threadIdx.x % T;
     const int y = blockIdx.x;
@@ -270,14 +301,21 @@ __device__ __forceinline__ float tap_bilinear(const float* __restrict__ f, int N
 // and optionally the Jacobian J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
 // of the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), by periodic
 // central differences with spacing L/N.
+// Grid = (N/64, N/4, frames); frame f reads and writes at offset f*stride.
 __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ height, const float* __restrict__ dx,
                                                  const float* __restrict__ dz, float4* __restrict__ normals,
                                                  float* __restrict__ jac, int N, int mode, float lambda,
-                                                 float inv2d)
+                                                 float inv2d, size_t stride)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= N || y >= N) return;
+    const size_t fo = (size_t)blockIdx.z * stride;
+    height += fo;
+    dx += fo;
+    dz += fo;
+    if (normals) normals += fo;
+    if (jac) jac += fo;
     const size_t i = (size_t)y * N + x;
     if (normals) {
         float z[8];
@@ -343,40 +381,109 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 
 // ------------------------------------------------------------------ launchers
 
-template <int N>
-struct Shape {
-    static constexpr int E  = 16;
-    static constexpr int T  = N / E;
-    static constexpr int CC = (512 / T) < 16 ? (512 / T) : 16;  // columns per block, pass 1
+// Column-pass schedules (radix E, columns per block C); the autotuner picks one
+// per N.  Row pass: radix E, three transforms per block.
+struct ColsVariant {
+    int E, C;
 };
+constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1}, {16, 2}, {16, 4}, {16, 8}, {8, 2}, {8, 4}, {8, 8}};
+constexpr int         kRowsRadix[kNumRowsVariants]    = {16, 8};
+
+template <int N, int E, int C>
+constexpr bool cols_fits()
+{
+    return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+}
+template <int N, int E>
+constexpr bool rows_fits()
+{
+    return N >= 2 * E && 3 * (N / E) <= 1024;
+}
+
+constexpr int radix_slot(int E) { return E == 16 ? 2 : E == 8 ? 1 : 0; }
+
+template <int N, int V>
+static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
+{
+    constexpr int E = kColsVariants[V].E, C = kColsVariants[V].C;
+    if constexpr (!cols_fits<N, E, C>()) {
+        return hipErrorInvalidValue;
+    } else {
+        hipLaunchKernelGGL((k_cols<N, E, C>), dim3((N / 2) / C + 1, a.nframes), dim3(C * (N / E)), 0, s, a.h0, a.z1,
+                           a.z2, a.tw[radix_slot(E)], a.sc, a.fb);
+        return hipGetLastError();
+    }
+}
+
+template <int N, int V>
+static hipError_t launch_rows_v(const FrameArgs& a, hipStream_t s)
+{
+    constexpr int E = kRowsRadix[V];
+    if constexpr (!rows_fits<N, E>()) {
+        return hipErrorInvalidValue;
+    } else {
+        hipLaunchKernelGGL((k_rows<N, E>), dim3(N / 2, a.nframes), dim3(3 * (N / E)), 0, s, a.z1, a.z2, a.height,
+                           a.dx, a.dz, a.tw[radix_slot(E)], a.fb);
+        return hipGetLastError();
+    }
+}
 
 template <int N>
 static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
 {
-    using S = Shape<N>;
-    const int nblk1 = (N / 2) / S::CC + 1;  // half plane tx in [0, N/2]
-    if (stages & 1)
-        hipLaunchKernelGGL((k_cols_fused<N, S::E, S::CC>), dim3(nblk1), dim3(S::CC * S::T), 0, s, a.h0, a.z1, a.z2,
-                           a.tw, a.sc, a.t_rev);
-    if (stages & 2)
-        hipLaunchKernelGGL((k_rows_fused<N, S::E>), dim3(N / 2), dim3(3 * S::T), 0, s, a.z1, a.z2, a.height, a.dx,
-                           a.dz, a.tw);
-    if ((stages & 4) && (a.normals || a.jacobian))
-        hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4), dim3(256), 0, s, a.height, a.dx, a.dz, a.normals,
-                           a.jacobian, N, a.normal_mode, a.lambda, a.inv2d);
-    return hipGetLastError();
+    hipError_t e = hipSuccess;
+    if (stages & 1) {
+        switch (a.cols_variant) {
+        case 0: e = launch_cols_v<N, 0>(a, s); break;
+        case 1: e = launch_cols_v<N, 1>(a, s); break;
+        case 2: e = launch_cols_v<N, 2>(a, s); break;
+        case 3: e = launch_cols_v<N, 3>(a, s); break;
+        case 4: e = launch_cols_v<N, 4>(a, s); break;
+        case 5: e = launch_cols_v<N, 5>(a, s); break;
+        case 6: e = launch_cols_v<N, 6>(a, s); break;
+        default: e = hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
+    }
+    if (stages & 2) {
+        switch (a.rows_variant) {
+        case 0: e = launch_rows_v<N, 0>(a, s); break;
+        case 1: e = launch_rows_v<N, 1>(a, s); break;
+        default: e = hipErrorInvalidValue;
+        }
+        if (e != hipSuccess) return e;
+    }
+    if ((stages & 4) && (a.normals || a.jacobian)) {
+        hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4, a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
+                           a.normals, a.jacobian, N, a.normal_mode, a.lambda, a.inv2d, a.fb.out_stride);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 template <int N>
-static hipError_t launch_ifft2_n(const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s)
+static bool variant_ok_n(int cols_v, int rows_v)
 {
-    using S = Shape<N>;
-    hipLaunchKernelGGL((k_c2c_rows<N, S::E>), dim3(N), dim3(S::T), 0, s, src, tmp, tw);
-    hipLaunchKernelGGL((k_c2c_cols_inv<N, S::E, S::CC>), dim3(N / S::CC), dim3(S::CC * S::T), 0, s, tmp, out, tw);
+    constexpr bool c[kNumColsVariants] = {
+        cols_fits<N, kColsVariants[0].E, kColsVariants[0].C>(), cols_fits<N, kColsVariants[1].E, kColsVariants[1].C>(),
+        cols_fits<N, kColsVariants[2].E, kColsVariants[2].C>(), cols_fits<N, kColsVariants[3].E, kColsVariants[3].C>(),
+        cols_fits<N, kColsVariants[4].E, kColsVariants[4].C>(), cols_fits<N, kColsVariants[5].E, kColsVariants[5].C>(),
+        cols_fits<N, kColsVariants[6].E, kColsVariants[6].C>()};
+    constexpr bool r[kNumRowsVariants] = {rows_fits<N, kRowsRadix[0]>(), rows_fits<N, kRowsRadix[1]>()};
+    return cols_v >= 0 && cols_v < kNumColsVariants && rows_v >= 0 && rows_v < kNumRowsVariants && c[cols_v] &&
+           r[rows_v];
+}
+
+template <int N>
+static hipError_t launch_ifft2_n(const float2* src, float2* tmp, float* out, const float2* const* tw, hipStream_t s)
+{
+    constexpr int E = 16, T = N / E, C = (512 / T) < 16 ? (512 / T) : 16;
+    hipLaunchKernelGGL((k_c2c_rows<N, E>), dim3(N), dim3(T), 0, s, src, tmp, tw[radix_slot(E)]);
+    hipLaunchKernelGGL((k_c2c_cols_inv<N, E, C>), dim3(N / C), dim3(C * T), 0, s, tmp, out, tw[radix_slot(E)]);
     return hipGetLastError();
 }
 
-#define OCN_DISPATCH_N(N, CALL)                     \
+#define OCN_DISPATCH_N(N, CALL, BAD)                \
     switch (N) {                                    \
     case 64: return CALL(64);                       \
     case 128: return CALL(128);                     \
@@ -385,20 +492,36 @@ static hipError_t launch_ifft2_n(const float2* src, float2* tmp, float* out, con
     case 1024: return CALL(1024);                   \
     case 2048: return CALL(2048);                   \
     case 4096: return CALL(4096);                   \
-    default: return hipErrorInvalidValue;           \
+    default: return BAD;                            \
     }
 
 hipError_t launch_frame(int N, const FrameArgs& a, int stages, hipStream_t s)
 {
+    if (a.nframes < 1 || a.nframes > kMaxFrames) return hipErrorInvalidValue;
 #define CALL(n) launch_frame_n<n>(a, stages, s)
-    OCN_DISPATCH_N(N, CALL)
+    OCN_DISPATCH_N(N, CALL, hipErrorInvalidValue)
 #undef CALL
 }
 
-hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* tw, hipStream_t s)
+bool schedule_valid(int N, int cols_variant, int rows_variant)
+{
+#define CALL(n) variant_ok_n<n>(cols_variant, rows_variant)
+    OCN_DISPATCH_N(N, CALL, false)
+#undef CALL
+}
+
+void cols_variant_shape(int v, int* E, int* C)
+{
+    *E = kColsVariants[v].E;
+    *C = kColsVariants[v].C;
+}
+
+int rows_variant_radix(int v) { return kRowsRadix[v]; }
+
+hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* const* tw, hipStream_t s)
 {
 #define CALL(n) launch_ifft2_n<n>(src, tmp, out, tw, s)
-    OCN_DISPATCH_N(N, CALL)
+    OCN_DISPATCH_N(N, CALL, hipErrorInvalidValue)
 #undef CALL
 }
 
@@ -420,11 +543,9 @@ hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, fl
 hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
-                       lambda, inv2d);
+    hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4, 1), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
+                       lambda, inv2d, (size_t)0);
     return hipGetLastError();
 }
-
-int fft_radix() { return 16; }
 
 }  // namespace ocn

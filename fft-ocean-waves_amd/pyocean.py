@@ -29,6 +29,7 @@ EXPORTS = [
     "ocn_tilde_h0_k", "ocn_tilde_h0_k_from_noise", "ocn_generate_twiddle_factors",
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
+    "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
     "ocn_get_fields", "ocn_get_spectrum",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
@@ -97,6 +98,9 @@ def lib():
             "ocn_update_into": [vp, dbl, F, vp],
             "ocn_update_stage": [vp, dbl, F, i32, vp],
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
+            "ocn_autotune": [vp, i32, i32, vp],
+            "ocn_set_schedule": [vp, i32, i32],
+            "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 5,
             "ocn_get_fields": [vp, F],
             "ocn_get_spectrum": [vp] + [ctypes.POINTER(vp)] * 4,
             "ocn_bit_reversed_indices": [i32, ctypes.POINTER(ctypes.c_int32)],
@@ -318,6 +322,19 @@ class Ocean:
         ts = np.ascontiguousarray(times, dtype=np.float64)
         check(lib().ocn_update_frames(self.h, ts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(ts),
                                       ctypes.byref(fields), self.stream))
+
+    # schedules
+    def autotune(self, nframes=1, reps=20):
+        check(lib().ocn_autotune(self.h, int(nframes), int(reps), self.stream))
+
+    def set_schedule(self, cols_variant, rows_variant):
+        check(lib().ocn_set_schedule(self.h, int(cols_variant), int(rows_variant)))
+
+    def schedule(self, nframes=1):
+        v = [ctypes.c_int() for _ in range(5)]
+        check(lib().ocn_get_schedule(self.h, int(nframes), *[ctypes.byref(x) for x in v]))
+        keys = ("cols_variant", "rows_variant", "cols_radix", "cols_per_block", "rows_radix")
+        return dict(zip(keys, [x.value for x in v]))
 
     # buffers
     def fields(self):

@@ -184,6 +184,23 @@ int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, 
  * (the per-frame slices of a recorded sequence). */
 int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream);
 
+/* ---------------------------------------------------------------- schedules */
+
+/* The fused frame runs a column pass and a row pass whose shapes (FFT radix,
+ * columns per workgroup) have several compiled variants per N.  No reference
+ * counterpart: the reference has one fixed 32x32 dispatch shape (main.cpp:18).
+ *
+ * ocn_autotune times every variant available for this N on the handle's
+ * device (nframes frames per launch, reps launches each, on `stream`, which
+ * it synchronises) and records the fastest for (device, N, frames-per-launch
+ * class); later launches of any handle with that N use it.  ocn_set_schedule
+ * pins a variant pair on this handle; ocn_get_schedule reports the pair used
+ * for launches of nframes frames and its shape (any out-pointer may be NULL). */
+int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream);
+int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant);
+int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* rows_variant, int* cols_radix,
+                     int* cols_per_block, int* rows_radix);
+
 /* ---------------------------------------------------------------- buffers */
 
 /* Device pointers of the handle-owned output textures (valid until destroy). */

@@ -140,10 +140,6 @@ FRAME_CASES = [
 ]
 
 
-def _frame_tol(name):
-    return name
-
-
 @pytest.mark.parametrize("name,kw,t", FRAME_CASES, ids=[c[0] for c in FRAME_CASES])
 def test_frame_matches_oracle(hip, name, kw, t):
     lp, op = both_params(**kw)
@@ -249,3 +245,68 @@ def test_full_size_point_samples_and_parseval(hip, N):
     e_ref = np.sum(np.abs(A) ** 2) / (N * N)
     e_got = np.sum(got["height"].astype(np.float64) ** 2)
     assert abs(e_got - e_ref) <= 1e-5 * e_ref, (e_got, e_ref)
+
+
+# ---------------------------------------------------------------- schedules
+
+@pytest.mark.parametrize("N", [64, 256, 1024])
+def test_every_schedule_variant_matches_oracle(hip, N):
+    kw = dict(c2_kwargs(), N=N, L=N * 2.0)
+    lp, op = both_params(**kw)
+    t = 3.5
+    ref = oracle.frame(op, oracle.h0(op), t, nthreads=NTHREADS)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        tried = 0
+        for cv in range(7):
+            for rv in range(2):
+                try:
+                    oc.set_schedule(cv, rv)
+                except pyocean.OceanError:
+                    continue
+                tried += 1
+                _say("schedule N=%d cols=%d rows=%d %s" % (N, cv, rv, oc.schedule()))
+                oc.update(t)
+                got = oc.download_fields()
+                for fld in ("height", "dx", "dz", "normals", "jacobian"):
+                    assert_field_close("%s[%d,%d]" % (fld, cv, rv), got[fld], ref[fld])
+        assert tried >= 4
+
+
+def test_multiframe_launch_matches_oracle_per_frame(hip):
+    kw = dict(c2_kwargs(), N=256, L=500.0)
+    lp, op = both_params(**kw)
+    N, F = 256, 40  # more than one launch of frames
+    times = 0.3 + np.arange(F) / 60.0
+    h0 = oracle.h0(op)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        bufs = {k: hip.malloc(F * N * N * (16 if k == "normals" else 4))
+                for k in ("height", "dx", "dz", "normals", "jacobian")}
+        try:
+            fl = pyocean.OcnFields(bufs["height"], bufs["dx"], bufs["dz"], bufs["normals"], bufs["jacobian"])
+            oc.update_frames(times, fl)
+            hip.sync()
+            got = {k: hip.download(v, (F, N, N, 4) if k == "normals" else (F, N, N)) for k, v in bufs.items()}
+        finally:
+            for v in bufs.values():
+                hip.free(v)
+    for f in (0, 17, 31, 32, 39):
+        ref = oracle.frame(op, h0, times[f], nthreads=NTHREADS)
+        for fld in ("height", "dx", "dz", "normals", "jacobian"):
+            assert_field_close("%s[f=%d]" % (fld, f), got[fld][f], ref[fld])
+
+
+def test_autotune_picks_a_valid_schedule_and_stays_correct(hip):
+    lp, op = both_params(**c1_kwargs())
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.autotune(nframes=1, reps=3)
+        sch = oc.schedule(1)
+        _say("autotuned N=256: %s" % sch)
+        assert sch["cols_radix"] in (8, 16) and sch["rows_radix"] in (8, 16)
+        oc.update(1.0)
+        got = oc.download_fields()
+    ref = oracle.frame(op, oracle.h0(op), 1.0, nthreads=NTHREADS)
+    for fld in ("height", "dx", "dz", "normals", "jacobian"):
+        assert_field_close(fld, got[fld], ref[fld])
