@@ -119,22 +119,30 @@ std::map<std::tuple<int, int, int>, std::pair<int, int>> g_sched;
 
 int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn::kMaxFrames; }
 
-// Built-in schedule: the fastest (column variant, row variant) measured on
-// MI355X for one frame per launch; the autotuner may replace it.
-std::pair<int, int> default_schedule(int N)
+// Built-in schedules: the fastest (column variant, row variant) measured on
+// MI355X per N for one frame per launch and for batches of frames; the
+// autotuner may replace them.  Column variants: 0 (16,1) 1 (16,2) 2 (16,4)
+// 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block); row
+// variants: 0 radix 16, 1 radix 8.
+std::pair<int, int> default_schedule(int N, int nframes)
 {
-    int cv;
+    const bool batch = nframes > 1;
+    int cv, rv = 1;
     switch (N) {
-    case 64: cv = 1; break;    // radix 16, 2 columns
-    case 128: cv = 2; break;   // radix 16, 4 columns
-    case 256: cv = 3; break;   // radix 16, 8 columns
-    case 512: cv = 3; break;
-    case 1024: cv = 3; break;
-    case 2048: cv = 2; break;
-    default: cv = 1; break;    // 4096
+    case 64: cv = 4; break;
+    case 128: cv = 4; break;
+    case 256: cv = batch ? 6 : 4; break;
+    case 512: cv = 5; break;
+    case 1024: cv = batch ? 2 : 5; break;
+    case 2048: cv = 4; break;
+    default: cv = 4; rv = 0; break;  // 4096: radix-8 rows would need 1536 threads
     }
-    if (!ocn::schedule_valid(N, cv, 0)) cv = 0;
-    return {cv, 0};
+    if (!ocn::schedule_valid(N, cv, rv)) {
+        cv = 0;
+        rv = 0;
+        while (cv < ocn::kNumColsVariants && !ocn::schedule_valid(N, cv, rv)) ++cv;
+    }
+    return {cv, rv};
 }
 
 }  // namespace
@@ -247,8 +255,9 @@ void pick_schedule(ocn_handle* h, int nframes, int* cv, int* rv)
             return;
         }
     }
-    *cv = h->cols_variant;
-    *rv = h->rows_variant;
+    const auto d = default_schedule(h->p.N, nframes);
+    *cv = d.first;
+    *rv = d.second;
 }
 
 void frame_args(const ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride,
@@ -264,6 +273,7 @@ void frame_args(const ocn_handle* h, const double* t, int nframes, const ocn_fie
     a->fb.z1_stride  = nn;
     a->fb.z2_stride  = nn / 2;
     a->fb.out_stride = out_stride;
+    a->fb.nframes    = nframes;
     a->nframes       = nframes;
     a->height        = out->height;
     a->dx            = out->dx;
@@ -359,7 +369,7 @@ int ocn_create(const ocn_params* p, ocn_handle** out)
         delete h;
         return hip_fail(e, "hipGetDevice");
     }
-    const auto sch = default_schedule(p->N);
+    const auto sch = default_schedule(p->N, 1);
     h->cols_variant = sch.first;
     h->rows_variant = sch.second;
     const size_t nn = (size_t)p->N * p->N;
@@ -485,36 +495,14 @@ int ocn_generate_normal_map(ocn_handle* h, void* stream)
     return e == hipSuccess ? OCN_OK : hip_fail(e, "generate_normal_map launch");
 }
 
-int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, void* stream)
-{
-    int rc = check_handle(h);
-    if (rc || (rc = check_fields(out)) || (rc = ready_for_frame(h, t))) return rc;
-    if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
-        return fail(OCN_EINVAL, "stage %d is not a combination of OCN_STAGE_* bits", stage);
-    if ((rc = set_device(h))) return rc;
-    int cv, rv;
-    pick_schedule(h, 1, &cv, &rv);
-    return run_frames(h, &t, 1, out, 0, stage, cv, rv, (hipStream_t)stream);
-}
-
-int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
-{
-    return ocn_update_stage(h, t, out, OCN_STAGE_ALL, stream);
-}
-
-int ocn_update(ocn_handle* h, double t, void* stream)
-{
-    int rc = check_handle(h);
-    if (rc) return rc;
-    ocn_fields f{h->height, h->dx, h->dz, reinterpret_cast<float*>(h->normals), h->jac};
-    return ocn_update_into(h, t, &f, stream);
-}
-
-int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream)
+int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, int stage,
+                     void* stream)
 {
     int rc = check_handle(h);
     if (rc || (rc = check_fields(out))) return rc;
     if (!t_host || nframes < 0) return fail(OCN_EINVAL, "t_host must be non-NULL and nframes >= 0");
+    if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
+        return fail(OCN_EINVAL, "stage %d is not a combination of OCN_STAGE_* bits", stage);
     for (int f = 0; f < nframes; ++f)
         if ((rc = ready_for_frame(h, t_host[f]))) return rc;
     if (nframes == 0) return OCN_OK;
@@ -527,9 +515,27 @@ int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const oc
         ocn_fields s{out->height + f0 * nn, out->dx + f0 * nn, out->dz + f0 * nn,
                      out->normals ? out->normals + f0 * nn * 4 : nullptr,
                      out->jacobian ? out->jacobian + f0 * nn : nullptr};
-        if ((rc = run_frames(h, t_host + f0, nf, &s, nn, OCN_STAGE_ALL, cv, rv, (hipStream_t)stream))) return rc;
+        if ((rc = run_frames(h, t_host + f0, nf, &s, nn, stage, cv, rv, (hipStream_t)stream))) return rc;
     }
     return OCN_OK;
+}
+
+int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
+{
+    return ocn_update_stage(h, &t, 1, out, OCN_STAGE_ALL, stream);
+}
+
+int ocn_update(ocn_handle* h, double t, void* stream)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    ocn_fields f{h->height, h->dx, h->dz, reinterpret_cast<float*>(h->normals), h->jac};
+    return ocn_update_into(h, t, &f, stream);
+}
+
+int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream)
+{
+    return ocn_update_stage(h, t_host, nframes, out, OCN_STAGE_ALL, stream);
 }
 
 int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant)
@@ -633,6 +639,28 @@ int ocn_get_spectrum(const ocn_handle* h, void** h0, void** hy, void** hx, void*
     if (hx) *hx = h->spec[1];
     if (hz) *hz = h->spec[2];
     return OCN_OK;
+}
+
+int ocn_copy_fields_to_host(ocn_handle* h, float* height, float* dx, float* dz, float* normals, float* jacobian,
+                            void* stream)
+{
+    int rc = check_handle(h);
+    if (rc || (rc = set_device(h))) return rc;
+    const size_t nn = (size_t)h->p.N * h->p.N;
+    hipStream_t  s  = (hipStream_t)stream;
+    const struct {
+        float*      dst;
+        const void* src;
+        size_t      bytes;
+    } copies[] = {{height, h->height, nn * 4}, {dx, h->dx, nn * 4}, {dz, h->dz, nn * 4},
+                  {normals, h->normals, nn * 16}, {jacobian, h->jac, nn * 4}};
+    for (const auto& c : copies) {
+        if (!c.dst) continue;
+        hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(fields)");
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipStreamSynchronize");
 }
 
 int ocn_bit_reversed_indices(int N, int32_t* out)

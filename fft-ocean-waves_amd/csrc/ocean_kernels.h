@@ -36,6 +36,7 @@ struct FrameBatch {
     size_t z1_stride;          // float2 elements
     size_t z2_stride;          // float2 elements
     size_t out_stride;         // float elements of height/dx/dz/jacobian (normals: 4x)
+    int    nframes;            // frames in this launch (<= kMaxFrames)
 };
 
 struct FrameArgs {

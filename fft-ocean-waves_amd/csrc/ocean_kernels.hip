@@ -180,27 +180,54 @@ __device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 
     az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
 }
 
+// ------------------------------------------------------------------ XCD-aware block order
+
+// MI355X spreads the workgroups of a launch round-robin over its 8 XCDs
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  This maps the
+// hardware block index to a logical index such that XCD x processes the
+// contiguous logical range [x*q + min(x, r), ...) in order (q = n / 8,
+// r = n % 8): consecutive logical blocks share an L2.  It is a bijection on
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_logical(int b, int n)
+{
+    constexpr int X = 8;
+    const int q = n / X, r = n % X, x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+}
+
 // ------------------------------------------------------------------ fused frame, pass 1
 
-// Column pass.  Grid = (column strips, frames).  Block = C columns x T threads;
-// thread (c, j) transforms column tx = blockIdx.x*C + c of the half plane
-// (tx <= N/2), rows ty = j + r*T; consecutive threads read consecutive columns,
-// so a wave's loads cover 64/C rows of C adjacent texels.  The three
-// transforms (height, Dx, Dz) share the column's LDS buffer.
+// Column strips per block group: G strips of C columns fill 128 B of a Z row.
+template <int C>
+__host__ __device__ constexpr int strip_group() { return C >= 16 ? 1 : 16 / C; }
+
+// Column pass.  1-D grid of (strip groups x frames x G) blocks.  Block = C
+// columns x T threads; thread (c, j) transforms column tx = strip*C + c of the
+// half plane (tx <= N/2), rows ty = j + r*T; consecutive threads read
+// consecutive columns, so a wave's loads cover 64/C rows of C adjacent texels.
+// The three transforms (height, Dx, Dz) share the column's LDS buffer.
+// Block order (after the XCD remap): group-major, then frame, then strip, so a
+// group's h0 columns (the same for every frame) and the Z lines its G strips
+// fill together are served by one XCD's L2.
 template <int N, int E, int C>
 __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h0, float2* __restrict__ z1,
                                                     float2* __restrict__ z2, const float2* __restrict__ tw,
                                                     SpecConst sc, FrameBatch fb)
 {
     constexpr int T = N / E;
+    constexpr int G = strip_group<C>();
     __shared__ float2 lds[C * lds_len(N)];
-    const int  f     = blockIdx.y;
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int rest  = L / G;
+    const int f     = rest % fb.nframes;
+    const int strip = (rest / fb.nframes) * G + L % G;
+    if (strip > (N / 2) / C) return;  // padding of the last group (whole block)
     const double t_rev = fb.t_rev[f];
     z1 += (size_t)f * fb.z1_stride;
     z2 += (size_t)f * fb.z2_stride;
     const int  c     = threadIdx.x % C;
     const int  j     = threadIdx.x / C;
-    const int  tx    = blockIdx.x * C + c;
+    const int  tx    = strip * C + c;
     const bool valid = tx <= N / 2;
     const int  txc   = valid ? tx : N / 2;
     const int  mxc   = (N - txc) & (N - 1);
@@ -301,16 +328,22 @@ __device__ __forceinline__ float tap_bilinear(const float* __restrict__ f, int N
 // and optionally the Jacobian J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
 // of the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), by periodic
 // central differences with spacing L/N.
-// Grid = (N/64, N/4, frames); frame f reads and writes at offset f*stride.
+// 1-D grid of (N/64) x (N/4) x frames blocks of 64 x 4 texels; frame f reads
+// and writes at offset f*stride.  After the XCD remap, blocks of neighbouring
+// rows run on the same XCD, so the stencil's halo rows come from its L2.
 __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ height, const float* __restrict__ dx,
                                                  const float* __restrict__ dz, float4* __restrict__ normals,
                                                  float* __restrict__ jac, int N, int mode, float lambda,
                                                  float inv2d, size_t stride)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int L  = xcd_logical(blockIdx.x, gridDim.x);
+    const int bx = L % (N / 64);
+    const int by = (L / (N / 64)) % (N / 4);
+    const int f  = L / ((N / 64) * (N / 4));
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by * 4 + (threadIdx.x >> 6);
     if (x >= N || y >= N) return;
-    const size_t fo = (size_t)blockIdx.z * stride;
+    const size_t fo = (size_t)f * stride;
     height += fo;
     dx += fo;
     dz += fo;
@@ -409,7 +442,10 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
     if constexpr (!cols_fits<N, E, C>()) {
         return hipErrorInvalidValue;
     } else {
-        hipLaunchKernelGGL((k_cols<N, E, C>), dim3((N / 2) / C + 1, a.nframes), dim3(C * (N / E)), 0, s, a.h0, a.z1,
+        constexpr int G       = strip_group<C>();
+        constexpr int strips  = (N / 2) / C + 1;  // half plane tx in [0, N/2]
+        constexpr int groups  = (strips + G - 1) / G;
+        hipLaunchKernelGGL((k_cols<N, E, C>), dim3(groups * G * a.nframes), dim3(C * (N / E)), 0, s, a.h0, a.z1,
                            a.z2, a.tw[radix_slot(E)], a.sc, a.fb);
         return hipGetLastError();
     }
@@ -454,7 +490,7 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         if (e != hipSuccess) return e;
     }
     if ((stages & 4) && (a.normals || a.jacobian)) {
-        hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4, a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
+        hipLaunchKernelGGL(k_normals, dim3((N / 64) * (N / 4) * a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
                            a.normals, a.jacobian, N, a.normal_mode, a.lambda, a.inv2d, a.fb.out_stride);
         e = hipGetLastError();
     }
@@ -543,7 +579,7 @@ hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, fl
 hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_normals, dim3(N / 64, N / 4, 1), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
+    hipLaunchKernelGGL(k_normals, dim3((N / 64) * (N / 4)), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
                        lambda, inv2d, (size_t)0);
     return hipGetLastError();
 }

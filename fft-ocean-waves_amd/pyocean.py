@@ -30,7 +30,7 @@ EXPORTS = [
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
     "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
-    "ocn_get_fields", "ocn_get_spectrum",
+    "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
 
@@ -96,13 +96,14 @@ def lib():
             "ocn_generate_normal_map": [vp, vp],
             "ocn_update": [vp, dbl, vp],
             "ocn_update_into": [vp, dbl, F, vp],
-            "ocn_update_stage": [vp, dbl, F, i32, vp],
+            "ocn_update_stage": [vp, ctypes.POINTER(ctypes.c_double), i32, F, i32, vp],
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_autotune": [vp, i32, i32, vp],
             "ocn_set_schedule": [vp, i32, i32],
             "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 5,
             "ocn_get_fields": [vp, F],
             "ocn_get_spectrum": [vp] + [ctypes.POINTER(vp)] * 4,
+            "ocn_copy_fields_to_host": [vp] * 7,
             "ocn_bit_reversed_indices": [i32, ctypes.POINTER(ctypes.c_int32)],
             "ocn_twiddle_factors": [i32, ctypes.POINTER(ctypes.c_float)],
         }
@@ -315,8 +316,10 @@ class Ocean:
     def update_into(self, t, fields):
         check(lib().ocn_update_into(self.h, float(t), ctypes.byref(fields), self.stream))
 
-    def update_stage(self, t, fields, stage):
-        check(lib().ocn_update_stage(self.h, float(t), ctypes.byref(fields), int(stage), self.stream))
+    def update_stage(self, times, fields, stage):
+        ts = np.ascontiguousarray(np.atleast_1d(times), dtype=np.float64)
+        check(lib().ocn_update_stage(self.h, ts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(ts),
+                                     ctypes.byref(fields), int(stage), self.stream))
 
     def update_frames(self, times, fields):
         ts = np.ascontiguousarray(times, dtype=np.float64)

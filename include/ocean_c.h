@@ -167,7 +167,8 @@ int ocn_update(ocn_handle* h, double t, void* stream);
 /* Same into caller buffers (height, dx, dz required; normals, jacobian optional). */
 int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream);
 
-/* One kernel (or a subset) of the fused frame, for profiling: stage is an
+/* One kernel (or a subset) of the fused frames, for profiling, with the
+ * frame layout of ocn_update_frames: stage is an
  * OR of OCN_STAGE_COLUMNS (spectrum + Hermitian fold + column transforms into
  * the handle's intermediate), OCN_STAGE_ROWS (row transforms + inversion into
  * height/dx/dz) and OCN_STAGE_NORMALS (normal map + Jacobian from
@@ -177,7 +178,8 @@ int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream
 #define OCN_STAGE_ROWS    2
 #define OCN_STAGE_NORMALS 4
 #define OCN_STAGE_ALL     7
-int ocn_update_stage(ocn_handle* h, double t, const ocn_fields* out, int stage, void* stream);
+int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, int stage,
+                     void* stream);
 
 /* nframes frames at times t_host[0..nframes) into consecutive output slices:
  * frame f writes out->height + f*N*N, ..., out->normals + f*N*N*4, ...
@@ -210,6 +212,14 @@ int ocn_get_fields(const ocn_handle* h, ocn_fields* out);
  * tilde_h0_t textures (float2[N*N] each; NULL until ocn_tilde_h0_t ran).
  * Any out-pointer may be NULL. */
 int ocn_get_spectrum(const ocn_handle* h, void** h0, void** hy, void** hx, void** hz);
+
+/* Copies the handle's output textures to host memory after the work queued on
+ * `stream` has finished (the call synchronises that stream).  Each pointer may
+ * be NULL to skip that field; sizes are N*N floats (normals: N*N*4).  No
+ * reference counterpart (the reference never reads its textures back); for
+ * callers without a HIP runtime of their own. */
+int ocn_copy_fields_to_host(ocn_handle* h, float* height, float* dx, float* dz, float* normals, float* jacobian,
+                            void* stream);
 
 /* ---------------------------------------------------------------- host utilities */
 

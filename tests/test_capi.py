@@ -114,3 +114,16 @@ def test_python_binding_fails_loudly_without_library(tmp_path, monkeypatch):
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         po.lib()
     importlib.reload(po)
+
+
+CPP_EXAMPLE = os.path.join(ROOT, "tests", "cpp", "ocean_api_example")
+
+
+def test_cpp_api_rejects_bad_params():
+    """The C++ host API (include/ocean.hpp) turns the ABI's status codes into
+    ocean::Error; these checks run before any HIP call."""
+    if not os.path.exists(CPP_EXAMPLE):
+        pytest.skip("C++ example not built (make -C tests/cpp)")
+    r = subprocess.run([CPP_EXAMPLE, "--validate"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok ") == 4

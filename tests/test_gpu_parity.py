@@ -310,3 +310,27 @@ def test_autotune_picks_a_valid_schedule_and_stays_correct(hip):
     ref = oracle.frame(op, oracle.h0(op), 1.0, nthreads=NTHREADS)
     for fld in ("height", "dx", "dz", "normals", "jacobian"):
         assert_field_close(fld, got[fld], ref[fld])
+
+
+def test_cpp_host_api_frame_matches_oracle(hip, tmp_path):
+    """One frame through the C++ host API (tests/cpp/ocean_api_example, a
+    child process), fused and pass by pass, against the oracle."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "ocean_api_example")
+    assert os.path.exists(exe), "build with __graft_entry__.build()"
+    kw = c1_kwargs()
+    out = tmp_path / "frame.bin"
+    args = [exe, str(out), "256", "1000", "40", "1", "1", "4", "0.5", "1.0", "42", "1.0"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    _say(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
+    N = 256
+    a = np.fromfile(out, dtype=np.float32)
+    assert a.size == N * N * 8
+    got = {"height": a[:N * N].reshape(N, N), "dx": a[N * N:2 * N * N].reshape(N, N),
+           "dz": a[2 * N * N:3 * N * N].reshape(N, N), "normals": a[3 * N * N:7 * N * N].reshape(N, N, 4),
+           "jacobian": a[7 * N * N:].reshape(N, N)}
+    op = oracle.make_params(**kw)
+    ref = oracle.frame(op, oracle.h0(op), 1.0, nthreads=NTHREADS)
+    for fld in got:
+        assert_field_close(fld, got[fld], ref[fld])
