@@ -33,10 +33,12 @@ N = 1024
 # Canonical (algorithmic) bytes per texel.  Whole frame: h0 pair in (16) +
 # one intermediate round trip of three complex FP32 fields (24 + 24) + height,
 # Dx, Dz (12) + normals (16) + Jacobian (4) = 96.  Per kernel of the fused
-# frame: columns = 16 + 24, rows = 24 + 12, normals = 16 + 4 (plus re-reading
-# height/Dx/Dz, which the canonical count does not credit).
+# frame: columns = 16 + 24; rows = 24 + 12 with a separate normal-map kernel
+# (16 + 4, plus re-reading height/Dx/Dz, which the canonical count does not
+# credit), or 24 + 12 + 16 + 4 = 56 when the row kernel also writes the
+# normals and the Jacobian.
 CANON_FRAME = 96
-CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20}
+CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20, "k_rows_nrm": 56}
 LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
 
 
@@ -149,8 +151,13 @@ def main():
     lf = min(LAUNCH_FRAMES, slots)
     tl = np.arange(lf, dtype=np.float64) / 60.0
     reps = 40
-    for name, stage in (("k_cols", pyocean.OCN_STAGE_COLUMNS), ("k_rows", pyocean.OCN_STAGE_ROWS),
-                        ("k_normals", pyocean.OCN_STAGE_NORMALS)):
+    sched = oc.schedule(lf)
+    fused = sched["rows_band"] > 0  # normal map + Jacobian inside the row kernel
+    stages = [("k_cols_lds" if sched["cols_variant"] >= 7 else "k_cols", pyocean.OCN_STAGE_COLUMNS),
+              ("k_rows_nrm" if fused else "k_rows", pyocean.OCN_STAGE_ROWS)]
+    if not fused:
+        stages.append(("k_normals", pyocean.OCN_STAGE_NORMALS))
+    for name, stage in stages:
         for _ in range(3):
             oc.update_stage(tl, slot_fields[0], stage)
         hip.record(e0, stream)
@@ -159,7 +166,7 @@ def main():
         hip.record(e1, stream)
         stage_ms[name] = hip.elapsed_ms(e0, e1) / reps
     dominant = max(stage_ms, key=stage_ms.get)
-    dom_bytes = CANON_STAGE[dominant] * nn * lf
+    dom_bytes = CANON_STAGE[dominant.replace("k_cols_lds", "k_cols")] * nn * lf
     achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
 
@@ -194,6 +201,7 @@ def main():
             "wall_s_timed_region": round(t_wall, 3),
             "kernel_ms_per_launch": {k: round(v, 5) for k, v in stage_ms.items()},
             "frames_per_launch": lf,
+            "schedule": sched,
         },
         "roofline": {
             "kernel": dominant,

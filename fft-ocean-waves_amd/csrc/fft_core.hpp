@@ -146,6 +146,19 @@ struct BlockSync {
     __device__ __forceinline__ void operator()() const { __syncthreads(); }
 };
 
+// For transforms whose thread group lies inside one wavefront (T <= 64 and
+// groups aligned to the wave): a wave's LDS operations execute in program
+// order, so ordering only has to hold in the compiler; the fences also make
+// it wait for this wave's outstanding LDS accesses.
+struct WaveSync {
+    __device__ __forceinline__ void operator()() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+};
+
 // One Stockham stage of radix R with stride NS on the E registers of thread j.
 // Virtual butterfly b (0 <= b < E/R) has index jj = j + b*T and owns registers
 // b + r*(E/R).  tw points at this stage's table: tw[k*(R-1) + r-1] =

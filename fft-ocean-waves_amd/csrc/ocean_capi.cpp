@@ -122,20 +122,22 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 // Built-in schedules: the fastest (column variant, row variant) measured on
 // MI355X per N for one frame per launch and for batches of frames; the
 // autotuner may replace them.  Column variants: 0 (16,1) 1 (16,2) 2 (16,4)
-// 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block); row
-// variants: 0 radix 16, 1 radix 8.
+// 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block) of k_cols,
+// 7..9 k_cols_lds with 1, 2, 4 columns per block.  Row variants: 0 radix 16,
+// 1 radix 8 (each followed by the normal-map kernel), 2..5 the fused
+// row + normals kernel with bands of 4, 8, 16, 32 rows.
 std::pair<int, int> default_schedule(int N, int nframes)
 {
     const bool batch = nframes > 1;
-    int cv, rv = 1;
+    int cv, rv;
     switch (N) {
-    case 64: cv = 4; break;
-    case 128: cv = 4; break;
-    case 256: cv = batch ? 6 : 4; break;
-    case 512: cv = 5; break;
-    case 1024: cv = batch ? 2 : 5; break;
-    case 2048: cv = 4; break;
-    default: cv = 4; rv = 0; break;  // 4096: radix-8 rows would need 1536 threads
+    case 64: cv = 4; rv = batch ? 2 : 1; break;
+    case 128: cv = 4; rv = batch ? 2 : 1; break;
+    case 256: cv = batch ? 9 : 7; rv = batch ? 2 : 1; break;
+    case 512: cv = batch ? 5 : 8; rv = batch ? 3 : 1; break;
+    case 1024: cv = 9; rv = batch ? 5 : 1; break;
+    case 2048: cv = 4; rv = batch ? 5 : 0; break;
+    default: cv = 4; rv = batch ? 5 : 0; break;  // 4096: radix-8 rows would need 1536 threads
     }
     if (!ocn::schedule_valid(N, cv, rv)) {
         cv = 0;
@@ -551,7 +553,7 @@ int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant)
 }
 
 int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* rows_variant, int* cols_radix,
-                     int* cols_per_block, int* rows_radix)
+                     int* cols_per_block, int* rows_radix, int* rows_band)
 {
     int rc = check_handle(h);
     if (rc) return rc;
@@ -564,6 +566,7 @@ int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* r
     if (cols_radix) *cols_radix = E;
     if (cols_per_block) *cols_per_block = C;
     if (rows_radix) *rows_radix = ocn::rows_variant_radix(rv);
+    if (rows_band) *rows_band = h->p.normal_mode == OCN_NORMALS_NEAREST ? ocn::rows_variant_band(rv) : 0;
     return OCN_OK;
 }
 
@@ -577,8 +580,9 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
     hipStream_t s = (hipStream_t)stream;
     const size_t nn = (size_t)h->p.N * h->p.N;
     float* outb = nullptr;
-    if ((rc = dalloc(&outb, nn * 3 * nframes, "autotune outputs"))) return rc;
-    ocn_fields out{outb, outb + nn * nframes, outb + 2 * nn * nframes, nullptr, nullptr};
+    if ((rc = dalloc(&outb, nn * 8 * nframes, "autotune outputs"))) return rc;
+    ocn_fields out{outb, outb + nn * nframes, outb + 2 * nn * nframes, outb + 3 * nn * nframes,
+                   outb + 7 * nn * nframes};
     std::vector<double> t(nframes);
     for (int f = 0; f < nframes; ++f) t[f] = 0.5 + f / 60.0;
     hipEvent_t e0, e1;
@@ -589,7 +593,7 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
     for (int cv = 0; cv < ocn::kNumColsVariants && rc == OCN_OK; ++cv)
         for (int rv = 0; rv < ocn::kNumRowsVariants && rc == OCN_OK; ++rv) {
             if (!ocn::schedule_valid(h->p.N, cv, rv)) continue;
-            const int stages = OCN_STAGE_COLUMNS | OCN_STAGE_ROWS;
+            const int stages = OCN_STAGE_ALL;
             if ((rc = run_frames(h, t.data(), nframes, &out, nn, stages, cv, rv, s))) break;  // warm-up
             (void)hipEventRecord(e0, s);
             for (int i = 0; i < reps && rc == OCN_OK; ++i)

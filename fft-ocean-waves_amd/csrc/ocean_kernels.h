@@ -9,8 +9,8 @@
 namespace ocn {
 
 constexpr int kMaxFrames        = 32;  // frames per launch of the fused path
-constexpr int kNumColsVariants  = 7;   // column-pass (radix, columns/block) schedules
-constexpr int kNumRowsVariants  = 2;   // row-pass radix schedules
+constexpr int kNumColsVariants  = 10;  // column-pass (kernel, radix, columns/block) schedules
+constexpr int kNumRowsVariants  = 6;   // row-pass schedules (radix, fused band)
 
 // Per-ocean spectrum constants, FP32 for the kernels plus the FP64 pieces of
 // the phase.  Filled by the host from ocn_params.
@@ -71,5 +71,6 @@ hipError_t launch_normals(int N, const float* height, const float* dx, const flo
 bool       schedule_valid(int N, int cols_variant, int rows_variant);
 void       cols_variant_shape(int v, int* E, int* C);
 int        rows_variant_radix(int v);
+int        rows_variant_band(int v);  // 0: separate normal-map kernel; R: fused bands of R rows
 
 }  // namespace ocn

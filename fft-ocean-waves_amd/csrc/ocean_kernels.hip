@@ -17,10 +17,12 @@
 // inverse DFT is Re of F's inverse DFT, which is all inversion_cs.glsl keeps),
 // and runs the column (ty) transforms for the half plane tx in [0, N/2].  It
 // stores the packed rows Z1 = G_h + i G_dx (full width via G(-tx) = conj G(tx))
-// and Z2 = G_dz(y) + i G_dz(y + N/2).  k_rows runs the row (tx)
+// and Z2 = G_dz(2p) + i G_dz(2p + 1).  k_rows runs the row (tx)
 // transforms: Re/Im of each result are two real outputs at once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "fft_core.hpp"
 #include "ocean_kernels.h"
@@ -120,14 +122,33 @@ __global__ __launch_bounds__(256) void k_tilde_h0_k_noise(float4* __restrict__ h
 // to one turn before the FP32 sincos.  m = nx^2 + ny^2.
 __device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
 {
-    kf = fmaxf(sc.kscale * sqrtf((float)m), 0.00001f);
-    const float  w0 = sqrtf(sc.g * kf);
+    kf = fmaxf(sc.kscale * __builtin_amdgcn_sqrtf((float)m), 0.00001f);
+    const float  w0 = __builtin_amdgcn_sqrtf(sc.g * kf);
     const double Q  = fmax(sc.k2_d * (double)m, 1e-10) * sc.g2_d;  // g^2 |k|_clamped^2
     const double w  = (double)w0;
     const double w2 = w * w;
-    const double w1 = w - (w2 * w2 - Q) * (double)(0.25f / (w0 * w0 * w0));
+    const double w1 = w - (w2 * w2 - Q) * (double)(0.25f * __builtin_amdgcn_rcpf(w0 * w0 * w0));
     const double rev = w1 * t_rev;
     return (float)(rev - rint(rev));
+}
+
+// sin and cos of 2 pi fr for fr in [-1/2, 1/2]: reduction by quarter turns
+// q = rint(4 fr) (the remainder r = fr - q/4 is exact), Taylor polynomials of
+// sin and cos on |2 pi r| <= pi/4 to degree 9 and 10 (truncation < 2e-9),
+// then rotation by q quarter turns.
+__device__ __forceinline__ void sincos_turns(float fr, float& s, float& c)
+{
+    const float q  = rintf(4.0f * fr);
+    const float th = (fr - 0.25f * q) * 6.28318530717958647692f;
+    const float t2 = th * th;
+    const float sp = th * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192239858907e-6f, -1.98412698412698413e-4f),
+                                                 8.33333333333333333e-3f), -1.66666666666666667e-1f), 1.0f);
+    const float cp = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192239858907e-7f, 2.48015873015873016e-5f),
+                                                       -1.38888888888888889e-3f), 4.16666666666666667e-2f), -0.5f),
+                          1.0f);
+    const int qi = (int)q & 3;
+    s = qi == 0 ? sp : qi == 1 ? cp : qi == 2 ? -sp : -cp;
+    c = qi == 0 ? cp : qi == 1 ? -sp : qi == 2 ? -cp : sp;
 }
 
 // a e^{i th} + C(b) e^{-i th} for h = (a, b) = (h0k, h0mk) (tilde_h0_t_cs.glsl:94-110).
@@ -151,12 +172,12 @@ __global__ __launch_bounds__(256) void k_tilde_h0_t(const float4* __restrict__ h
     const size_t i  = (size_t)ty * sc.N + tx;
     const int    nx = tx - sc.N / 2, ny = ty - sc.N / 2;
     float kf, s, c;
-    const float fr = phase_turns(sc, nx * nx + ny * ny, t_rev, kf);
-    sincospif(2.0f * fr, &s, &c);
-    const float2 h = evolve_amp(sc, h0[i], s, c);
+    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kf), s, c);
+    const float  kinv = sc.kscale * __builtin_amdgcn_rcpf(kf);
+    const float2 h    = evolve_amp(sc, h0[i], s, c);
     hy[i] = h;
-    hx[i] = choppy(sc.kscale * (float)nx / kf, h);
-    hz[i] = choppy(sc.kscale * (float)ny / kf, h);
+    hx[i] = choppy((float)nx * kinv, h);
+    hz[i] = choppy((float)ny * kinv, h);
 }
 
 // Hermitian parts (F(t) + conj F(t'))/2 of the three spectra at texel t and its
@@ -167,9 +188,8 @@ __device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 
                                           double t_rev, float2& ay, float2& ax, float2& az)
 {
     float kf, s, c;
-    const float fr = phase_turns(sc, nx * nx + ny * ny, t_rev, kf);
-    sincospif(2.0f * fr, &s, &c);
-    const float  kinv = sc.kscale / kf;
+    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kf), s, c);
+    const float  kinv = sc.kscale * __builtin_amdgcn_rcpf(kf);
     const float2 H = evolve_amp(sc, p, s, c);
     const float2 G = evolve_amp(sc, q, s, c);
     const float mx = (float)nx * kinv, mz = (float)ny * kinv;
@@ -247,6 +267,12 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
     fft_inverse<N, E>(ay, j, buf, tw, BlockSync{});
     fft_inverse<N, E>(ax, j, buf, tw, BlockSync{});
     fft_inverse<N, E>(az, j, buf, tw, BlockSync{});
+    // G_dz back through LDS in natural order: Z2 pairs adjacent rows 2p, 2p+1,
+    // which sit in different threads.
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = az[r];
+    __syncthreads();
 
     if (!valid) return;
     const bool mirror = mxc != tx;
@@ -258,18 +284,100 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
     }
 #pragma unroll
     for (int r = 0; r < E / 2; ++r) {
-        const size_t row = (size_t)(j + r * T) * N;
-        const float2 lo = az[r], hi = az[r + E / 2];  // rows y and y + N/2
+        const int    p   = j + r * T;
+        const size_t row = (size_t)p * N;
+        const float2 lo = buf[lds_pad(2 * p)], hi = buf[lds_pad(2 * p + 1)];  // rows 2p and 2p+1
         z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
         if (mirror) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
     }
 }
 
+// Column pass, LDS-staged variant (requires T = N/16 <= 64).  Same work and
+// block order as k_cols.  Phase A: every thread evaluates Hermitian parts of
+// the block's C columns (consecutive threads take consecutive columns, then
+// rows) into per-(field, column) LDS buffers.  Phase B: each of the 3C
+// transforms runs inside one T-thread group that lies in a single wave, with
+// wave-level ordering only, and leaves its result in natural order in the same
+// buffer.  Phase C: every thread stores Z1/Z2 rows.  Block = 3C*T threads.
+template <int N, int C>
+__global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __restrict__ h0,
+                                                               float2* __restrict__ z1, float2* __restrict__ z2,
+                                                               const float2* __restrict__ tw, SpecConst sc,
+                                                               FrameBatch fb)
+{
+    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
+    constexpr int G = strip_group<C>();
+    static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
+    __shared__ float2 lds[3 * C * LEN];
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int rest  = L / G;
+    const int f     = rest % fb.nframes;
+    const int strip = (rest / fb.nframes) * G + L % G;
+    if (strip > (N / 2) / C) return;  // padding of the last group (whole block)
+    const double t_rev = fb.t_rev[f];
+    z1 += (size_t)f * fb.z1_stride;
+    z2 += (size_t)f * fb.z2_stride;
+
+    constexpr int ITERS = (C * N + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        if (k < C * N) {
+            const int c   = k % C, ty = k / C;
+            const int txc = min(strip * C + c, N / 2);
+            const int mxc = (N - txc) & (N - 1);
+            const int my  = (N - ty) & (N - 1);
+            float2 ay, ax, az;
+            pair_fold(sc, h0[(size_t)ty * N + txc], h0[(size_t)my * N + mxc], txc - N / 2, ty - N / 2, mxc - N / 2,
+                      my - N / 2, t_rev, ay, ax, az);
+            lds[(0 * C + c) * LEN + lds_pad(ty)] = ay;
+            lds[(1 * C + c) * LEN + lds_pad(ty)] = ax;
+            lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
+        }
+    }
+    __syncthreads();
+    {
+        const int g = threadIdx.x / T, j = threadIdx.x % T;
+        float2* buf = lds + g * LEN;
+        float2  v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+        fft_inverse<N, E>(v, j, buf, tw, WaveSync{});
+        WaveSync{}();
+#pragma unroll
+        for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        if (k < C * N) {
+            const int c = k % C, y = k / C;
+            const int tx = strip * C + c;
+            if (tx <= N / 2) {
+                const int    mxc = (N - tx) & (N - 1);
+                const float2 gy  = lds[(0 * C + c) * LEN + lds_pad(y)];
+                const float2 gx  = lds[(1 * C + c) * LEN + lds_pad(y)];
+                const size_t row = (size_t)y * N;
+                z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
+                if (mxc != tx) z1[row + mxc] = make_float2(gy.x + gx.y, gx.x - gy.y);
+                if (y < N / 2) {  // Z2 row y pairs G_dz rows 2y and 2y+1
+                    const float2 lo = lds[(2 * C + c) * LEN + lds_pad(2 * y)];
+                    const float2 hi = lds[(2 * C + c) * LEN + lds_pad(2 * y + 1)];
+                    z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
+                    if (mxc != tx) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
+                }
+            }
+        }
+    }
+}
+
 // ------------------------------------------------------------------ fused frame, pass 2
 
-// Row pass.  Grid = (N/2, frames).  Block y handles rows y and y + N/2: group 0
-// transforms Z1 row y, group 1 Z1 row y + N/2, group 2 Z2 row y, then applies
-// the inversion step (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).
+// Row pass.  Grid = (N/2, frames).  Block p handles rows 2p and 2p+1: group 0
+// transforms Z1 row 2p, group 1 Z1 row 2p+1, group 2 Z2 row p (whose real and
+// imaginary parts are Dz of rows 2p and 2p+1), then applies the inversion step
+// (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).
 template <int N, int E>
 __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__ z1, const float2* __restrict__ z2,
                                                       float* __restrict__ height, float* __restrict__ dx,
@@ -287,87 +395,262 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
     dz += o;
     const int g = threadIdx.x / T;
     const int j = threadIdx.x % T;
-    const int y = blockIdx.x;
-    const float2* src = g == 0 ? z1 + (size_t)y * N : g == 1 ? z1 + (size_t)(y + N / 2) * N : z2 + (size_t)y * N;
+    const int p = blockIdx.x;
+    const float2* src = g == 0 ? z1 + (size_t)(2 * p) * N : g == 1 ? z1 + (size_t)(2 * p + 1) * N : z2 + (size_t)p * N;
     float2 v[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
     fft_inverse<N, E>(v, j, lds + g * lds_len(N), tw, BlockSync{});
     constexpr float scale = 1.0f / ((float)N * (float)N);
-    const size_t row_lo = (size_t)y * N, row_hi = (size_t)(y + N / 2) * N;
+    const size_t row0 = (size_t)(2 * p) * N, row1 = row0 + N;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        const int x = j + r * T;
-        const float s = ((x + y) & 1) ? -scale : scale;  // N/2 is even: same sign for y + N/2
+        const int   x = j + r * T;
+        const float s = (x & 1) ? -scale : scale;  // sign of row 2p; row 2p+1 has the opposite sign
         if (g == 0) {
-            height[row_lo + x] = s * v[r].x;
-            dx[row_lo + x]     = s * v[r].y;
+            height[row0 + x] = s * v[r].x;
+            dx[row0 + x]     = s * v[r].y;
         } else if (g == 1) {
-            height[row_hi + x] = s * v[r].x;
-            dx[row_hi + x]     = s * v[r].y;
+            height[row1 + x] = -s * v[r].x;
+            dx[row1 + x]     = -s * v[r].y;
         } else {
-            dz[row_lo + x] = s * v[r].x;
-            dz[row_hi + x] = s * v[r].y;
+            dz[row0 + x] = s * v[r].x;
+            dz[row1 + x] = -s * v[r].y;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ fused row pass + normals + Jacobian
+
+// Row pass fused with the normal map (nearest taps) and the Jacobian.  A group
+// of T = N/E threads owns a band of R rows [y0, y0 + R) (y0 even) and walks
+// it with a three-row window: for each new row it transforms Z1 (height + i Dx)
+// and, on even rows, the Z2 row holding Dz of that row and the next; rows
+// y-1, y and y+1 stay in registers (thread j holds x = j + r*T).  The
+// x-neighbours of the stencil pass through the group's LDS buffer.  The band's
+// halo rows y0-1 and y0+R are transformed as well (periodic).  Sobel taps and
+// Jacobian as in k_normals.  Groups lie in one wave when T <= 64 (wave-level
+// ordering), otherwise a block is one group.  1-D grid of ceil(bands*frames /
+// groups per block) blocks; after the XCD remap, neighbouring bands (which
+// share halo rows) run on the same XCD.
+template <int N, int E, int R>
+__global__ __launch_bounds__(N / E >= 64 ? N / E : 64) void k_rows_nrm(
+    const float2* __restrict__ z1, const float2* __restrict__ z2, float* __restrict__ height,
+    float* __restrict__ dx, float* __restrict__ dz, float4* __restrict__ normals, float* __restrict__ jac,
+    const float2* __restrict__ tw, FrameBatch fb, float lambda, float inv2d)
+{
+    constexpr int T   = N / E;
+    constexpr int GPB = T >= 64 ? 1 : 64 / T;  // groups per block
+    constexpr int NB  = N / R;                  // bands per frame
+    static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    __shared__ float2 lds[GPB * lds_len(N)];
+    const int g    = threadIdx.x / T;
+    const int j    = threadIdx.x % T;
+    const int L    = xcd_logical(blockIdx.x, gridDim.x) * GPB + g;
+    const int band = L % NB;
+    const int f    = L / NB;
+    if (f >= fb.nframes) return;  // whole group (a wave part when T <= 64): padding of the last block
+    z1 += (size_t)f * fb.z1_stride;
+    z2 += (size_t)f * fb.z2_stride;
+    const size_t fo = (size_t)f * fb.out_stride;
+    height += fo;
+    dx += fo;
+    dz += fo;
+    if (normals) normals += fo;
+    if (jac) jac += fo;
+    float2* buf = lds + g * lds_len(N);
+    constexpr float scale = 1.0f / ((float)N * (float)N);
+    const int y0 = band * R;
+
+    // inverse transform of one Z row into v (natural order, thread j holds x = j + r*T)
+    auto fft_row = [&](const float2* src, float2 (&v)[E]) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+    };
+    // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y
+    auto sgn = [&](int y) { return ((j + y) & 1) ? -scale : scale; };
+
+    float hA[E], xA[E], zA[E], hB[E], xB[E], zB[E], zN[E];
+    float2 v[E];
+    {
+        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the imaginary part of pair yA >> 1
+        fft_row(z2 + (size_t)(yA >> 1) * N, v);
+        const float sa = sgn(yA);
+#pragma unroll
+        for (int r = 0; r < E; ++r) zA[r] = sa * v[r].y;
+        fft_row(z1 + (size_t)yA * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            hA[r] = sa * v[r].x;
+            xA[r] = sa * v[r].y;
+        }
+        const float sb = sgn(y0);
+        fft_row(z2 + (size_t)(y0 >> 1) * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            zB[r] = sb * v[r].x;
+            zN[r] = -sb * v[r].y;  // Dz of row y0 + 1 (opposite sign)
+        }
+        fft_row(z1 + (size_t)y0 * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            hB[r] = sb * v[r].x;
+            xB[r] = sb * v[r].y;
+        }
+    }
+
+#pragma unroll 1
+    for (int k = 0; k < R; ++k) {
+        const int   y  = y0 + k;
+        const int   yC = (y + 1) & (N - 1);
+        const float sc = sgn(yC);
+        float zC[E];
+        if ((yC & 1) == 0) {
+            fft_row(z2 + (size_t)(yC >> 1) * N, v);
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                zC[r] = sc * v[r].x;
+                zN[r] = -sc * v[r].y;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < E; ++r) zC[r] = zN[r];
+        }
+        fft_row(z1 + (size_t)yC * N, v);  // v = height + i Dx of row yC (unsigned)
+
+        const size_t row = (size_t)y * N;
+        if (normals) {
+            // v_s = h(y-1) + 2 h(y) + h(y+1), d_s = h(y-1) - h(y+1); nx = v_s(x-1) - v_s(x+1),
+            // nz = d_s(x-1) + 2 d_s(x) + d_s(x+1)  (normal_map_cs.glsl:49-51)
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const float hC = sc * v[r].x;
+                buf[lds_pad(j + r * T)] = make_float2(hA[r] + 2.0f * hB[r] + hC, hA[r] - hC);
+            }
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int    x  = j + r * T;
+                const float2 lf = buf[lds_pad((x - 1) & (N - 1))];
+                const float2 rt = buf[lds_pad((x + 1) & (N - 1))];
+                const float2 md = buf[lds_pad(x)];
+                const float  nx = lf.x - rt.x;
+                const float  nz = lf.y + 2.0f * md.y + rt.y;
+                const float  inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
+                normals[row + x] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+            }
+        }
+        if (jac) {
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = make_float2(xB[r], zB[r]);
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int    x    = j + r * T;
+                const float2 lf   = buf[lds_pad((x - 1) & (N - 1))];
+                const float2 rt   = buf[lds_pad((x + 1) & (N - 1))];
+                const float  dxdx = (rt.x - lf.x) * inv2d;
+                const float  dzdx = (rt.y - lf.y) * inv2d;
+                const float  dxdz = (sc * v[r].y - xA[r]) * inv2d;
+                const float  dzdz = (zC[r] - zA[r]) * inv2d;
+                jac[row + x] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const int x = j + r * T;
+            height[row + x] = hB[r];
+            dx[row + x]     = xB[r];
+            dz[row + x]     = zB[r];
+            hA[r] = hB[r];
+            xA[r] = xB[r];
+            zA[r] = zB[r];
+            hB[r] = sc * v[r].x;
+            xB[r] = sc * v[r].y;
+            zB[r] = zC[r];
         }
     }
 }
 
 // ------------------------------------------------------------------ A7 normals (+ Jacobian)
 
-__device__ __forceinline__ float tap(const float* __restrict__ f, int N, int x, int y)
-{
-    return f[(size_t)((y + N) & (N - 1)) * N + ((x + N) & (N - 1))];
-}
-// texture() under LINEAR + REPEAT at a texel corner: 2x2 box over (x-1..x, y-1..y).
-__device__ __forceinline__ float tap_bilinear(const float* __restrict__ f, int N, int x, int y)
-{
-    return 0.25f * (tap(f, N, x - 1, y - 1) + tap(f, N, x, y - 1) + tap(f, N, x - 1, y) + tap(f, N, x, y));
-}
-
 // normal_map_cs.glsl:24-54 (Sobel on the height, ny = 1, normalize, w = 1),
 // and optionally the Jacobian J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
 // of the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), by periodic
 // central differences with spacing L/N.
-// 1-D grid of (N/64) x (N/4) x frames blocks of 64 x 4 texels; frame f reads
-// and writes at offset f*stride.  After the XCD remap, blocks of neighbouring
-// rows run on the same XCD, so the stencil's halo rows come from its L2.
+//
+// Tiled: a block owns a 64 x 16 texel tile and first stages height/dx/dz for
+// the tile plus a 2-texel periodic halo in LDS with coalesced row loads (the
+// bilinear taps reach two texels down/left, the others one), then every tap
+// comes from LDS.  1-D grid of (N/64) x (N/16) x frames blocks; frame f reads
+// and writes at offset f*stride.  After the XCD remap, tiles of neighbouring
+// rows run on the same XCD, so the halo rows are L2 hits.
+constexpr int kNrmW = 64, kNrmH = 16, kNrmHalo = 2;
+constexpr int kNrmLW = kNrmW + 2 * kNrmHalo, kNrmLH = kNrmH + 2 * kNrmHalo;
+
+// texture() of the staged tile under NEAREST + REPEAT (s[y][x], halo-offset
+// coordinates) or LINEAR + REPEAT at a texel corner: 2x2 box over (x-1..x, y-1..y).
+__device__ __forceinline__ float stap(const float (*s)[kNrmLW], int y, int x, int mode)
+{
+    return mode ? 0.25f * (s[y - 1][x - 1] + s[y - 1][x] + s[y][x - 1] + s[y][x]) : s[y][x];
+}
+
 __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ height, const float* __restrict__ dx,
                                                  const float* __restrict__ dz, float4* __restrict__ normals,
                                                  float* __restrict__ jac, int N, int mode, float lambda,
                                                  float inv2d, size_t stride)
 {
+    __shared__ float sh[kNrmLH][kNrmLW], sx[kNrmLH][kNrmLW], sz[kNrmLH][kNrmLW];
+    const int tiles_x = N / kNrmW, tiles_y = N / kNrmH;
     const int L  = xcd_logical(blockIdx.x, gridDim.x);
-    const int bx = L % (N / 64);
-    const int by = (L / (N / 64)) % (N / 4);
-    const int f  = L / ((N / 64) * (N / 4));
-    const int x = bx * 64 + (threadIdx.x & 63);
-    const int y = by * 4 + (threadIdx.x >> 6);
-    if (x >= N || y >= N) return;
+    const int bx = L % tiles_x;
+    const int by = (L / tiles_x) % tiles_y;
+    const int f  = L / (tiles_x * tiles_y);
     const size_t fo = (size_t)f * stride;
     height += fo;
     dx += fo;
     dz += fo;
     if (normals) normals += fo;
     if (jac) jac += fo;
-    const size_t i = (size_t)y * N + x;
-    if (normals) {
-        float z[8];
-        const int ox[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-        const int oy[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-            z[k] = mode ? tap_bilinear(height, N, x + ox[k], y + oy[k]) : tap(height, N, x + ox[k], y + oy[k]);
-        const float nz = z[0] + 2.0f * z[1] + z[2] - z[5] - 2.0f * z[6] - z[7];
-        const float nx = z[0] + 2.0f * z[3] + z[5] - z[2] - 2.0f * z[4] - z[7];
-        const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
-        normals[i] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+    const int x0 = bx * kNrmW, y0 = by * kNrmH;
+    for (int k = threadIdx.x; k < kNrmLH * kNrmLW; k += 256) {
+        const int ly = k / kNrmLW, lx = k % kNrmLW;
+        const size_t gi = (size_t)((y0 + ly - kNrmHalo) & (N - 1)) * N + ((x0 + lx - kNrmHalo) & (N - 1));
+        if (normals) sh[ly][lx] = height[gi];
+        if (jac) {
+            sx[ly][lx] = dx[gi];
+            sz[ly][lx] = dz[gi];
+        }
     }
-    if (jac) {
-        const float dxdx = (tap(dx, N, x + 1, y) - tap(dx, N, x - 1, y)) * inv2d;
-        const float dxdz = (tap(dx, N, x, y + 1) - tap(dx, N, x, y - 1)) * inv2d;
-        const float dzdx = (tap(dz, N, x + 1, y) - tap(dz, N, x - 1, y)) * inv2d;
-        const float dzdz = (tap(dz, N, x, y + 1) - tap(dz, N, x, y - 1)) * inv2d;
-        jac[i] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+    __syncthreads();
+    const int cx = (threadIdx.x & 63) + kNrmHalo;
+#pragma unroll
+    for (int k = 0; k < kNrmH / 4; ++k) {
+        const int ly = (threadIdx.x >> 6) + 4 * k;
+        const int cy = ly + kNrmHalo;
+        const size_t i = (size_t)(y0 + ly) * N + x0 + (threadIdx.x & 63);
+        if (normals) {
+            // z0..z7 at (x-1,y-1) (x,y-1) (x+1,y-1) (x-1,y) (x+1,y) (x-1,y+1) (x,y+1) (x+1,y+1)
+            const float z0 = stap(sh, cy - 1, cx - 1, mode), z1 = stap(sh, cy - 1, cx, mode);
+            const float z2 = stap(sh, cy - 1, cx + 1, mode), z3 = stap(sh, cy, cx - 1, mode);
+            const float z4 = stap(sh, cy, cx + 1, mode), z5 = stap(sh, cy + 1, cx - 1, mode);
+            const float z6 = stap(sh, cy + 1, cx, mode), z7 = stap(sh, cy + 1, cx + 1, mode);
+            const float nz  = z0 + 2.0f * z1 + z2 - z5 - 2.0f * z6 - z7;
+            const float nx  = z0 + 2.0f * z3 + z5 - z2 - 2.0f * z4 - z7;
+            const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
+            normals[i] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+        }
+        if (jac) {
+            const float dxdx = (sx[cy][cx + 1] - sx[cy][cx - 1]) * inv2d;
+            const float dxdz = (sx[cy + 1][cx] - sx[cy - 1][cx]) * inv2d;
+            const float dzdx = (sz[cy][cx + 1] - sz[cy][cx - 1]) * inv2d;
+            const float dzdz = (sz[cy + 1][cx] - sz[cy - 1][cx]) * inv2d;
+            jac[i] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+        }
     }
 }
 
@@ -416,21 +699,36 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 
 // Column-pass schedules (radix E, columns per block C); the autotuner picks one
 // per N.  Row pass: radix E, three transforms per block.
+// kind 0: k_cols (register-resident, block-synchronised transforms);
+// kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only).
 struct ColsVariant {
-    int E, C;
+    int E, C, kind;
 };
-constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1}, {16, 2}, {16, 4}, {16, 8}, {8, 2}, {8, 4}, {8, 8}};
-constexpr int         kRowsRadix[kNumRowsVariants]    = {16, 8};
+constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
+                                                         {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1}};
+// Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
+// band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused.
+struct RowsVariant {
+    int E, band;
+};
+constexpr RowsVariant kRowsVariants[kNumRowsVariants] = {{16, 0}, {8, 0}, {8, 4}, {8, 8}, {8, 16}, {8, 32}};
 
-template <int N, int E, int C>
+template <int N, int E, int C, int K>
 constexpr bool cols_fits()
 {
-    return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+    if constexpr (K == 0)
+        return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+    else
+        return E == 16 && N / 16 <= 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024 &&
+               C <= N / 2;
 }
-template <int N, int E>
+template <int N, int E, int B>
 constexpr bool rows_fits()
 {
-    return N >= 2 * E && 3 * (N / E) <= 1024;
+    if constexpr (B == 0)
+        return N >= 2 * E && 3 * (N / E) <= 1024;
+    else
+        return N >= 2 * E && N % B == 0 && N / E <= 1024;
 }
 
 constexpr int radix_slot(int E) { return E == 16 ? 2 : E == 8 ? 1 : 0; }
@@ -438,9 +736,16 @@ constexpr int radix_slot(int E) { return E == 16 ? 2 : E == 8 ? 1 : 0; }
 template <int N, int V>
 static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
 {
-    constexpr int E = kColsVariants[V].E, C = kColsVariants[V].C;
-    if constexpr (!cols_fits<N, E, C>()) {
+    constexpr int E = kColsVariants[V].E, C = kColsVariants[V].C, K = kColsVariants[V].kind;
+    if constexpr (!cols_fits<N, E, C, K>()) {
         return hipErrorInvalidValue;
+    } else if constexpr (K == 1) {
+        constexpr int G      = strip_group<C>();
+        constexpr int strips = (N / 2) / C + 1;
+        constexpr int groups = (strips + G - 1) / G;
+        hipLaunchKernelGGL((k_cols_lds<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
+                           a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        return hipGetLastError();
     } else {
         constexpr int G       = strip_group<C>();
         constexpr int strips  = (N / 2) / C + 1;  // half plane tx in [0, N/2]
@@ -454,12 +759,19 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
 template <int N, int V>
 static hipError_t launch_rows_v(const FrameArgs& a, hipStream_t s)
 {
-    constexpr int E = kRowsRadix[V];
-    if constexpr (!rows_fits<N, E>()) {
+    constexpr int E = kRowsVariants[V].E, B = kRowsVariants[V].band;
+    if constexpr (!rows_fits<N, E, B>()) {
         return hipErrorInvalidValue;
-    } else {
+    } else if constexpr (B == 0) {
         hipLaunchKernelGGL((k_rows<N, E>), dim3(N / 2, a.nframes), dim3(3 * (N / E)), 0, s, a.z1, a.z2, a.height,
                            a.dx, a.dz, a.tw[radix_slot(E)], a.fb);
+        return hipGetLastError();
+    } else {
+        constexpr int T = N / E, GPB = T >= 64 ? 1 : 64 / T;
+        const int groups = (N / B) * a.nframes;
+        hipLaunchKernelGGL((k_rows_nrm<N, E, B>), dim3((groups + GPB - 1) / GPB), dim3(GPB * T), 0, s, a.z1, a.z2,
+                           a.height, a.dx, a.dz, a.normals, a.jacobian, a.tw[radix_slot(E)], a.fb, a.lambda,
+                           a.inv2d);
         return hipGetLastError();
     }
 }
@@ -477,20 +789,32 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 4: e = launch_cols_v<N, 4>(a, s); break;
         case 5: e = launch_cols_v<N, 5>(a, s); break;
         case 6: e = launch_cols_v<N, 6>(a, s); break;
+        case 7: e = launch_cols_v<N, 7>(a, s); break;
+        case 8: e = launch_cols_v<N, 8>(a, s); break;
+        case 9: e = launch_cols_v<N, 9>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
     }
+    // The fused row variants cover the normal map and Jacobian (nearest taps
+    // only); with bilinear normals the unfused radix-16 row pass is used.
+    int rv = a.rows_variant;
+    if (rv >= 0 && rv < kNumRowsVariants && kRowsVariants[rv].band > 0 && a.normal_mode != 0) rv = 0;
+    const bool fused = rv >= 0 && rv < kNumRowsVariants && kRowsVariants[rv].band > 0;
     if (stages & 2) {
-        switch (a.rows_variant) {
+        switch (rv) {
         case 0: e = launch_rows_v<N, 0>(a, s); break;
         case 1: e = launch_rows_v<N, 1>(a, s); break;
+        case 2: e = launch_rows_v<N, 2>(a, s); break;
+        case 3: e = launch_rows_v<N, 3>(a, s); break;
+        case 4: e = launch_rows_v<N, 4>(a, s); break;
+        case 5: e = launch_rows_v<N, 5>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
     }
-    if ((stages & 4) && (a.normals || a.jacobian)) {
-        hipLaunchKernelGGL(k_normals, dim3((N / 64) * (N / 4) * a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
+    if ((stages & 4) && !fused && (a.normals || a.jacobian)) {
+        hipLaunchKernelGGL(k_normals, dim3((N / kNrmW) * (N / kNrmH) * a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
                            a.normals, a.jacobian, N, a.normal_mode, a.lambda, a.inv2d, a.fb.out_stride);
         e = hipGetLastError();
     }
@@ -501,11 +825,14 @@ template <int N>
 static bool variant_ok_n(int cols_v, int rows_v)
 {
     constexpr bool c[kNumColsVariants] = {
-        cols_fits<N, kColsVariants[0].E, kColsVariants[0].C>(), cols_fits<N, kColsVariants[1].E, kColsVariants[1].C>(),
-        cols_fits<N, kColsVariants[2].E, kColsVariants[2].C>(), cols_fits<N, kColsVariants[3].E, kColsVariants[3].C>(),
-        cols_fits<N, kColsVariants[4].E, kColsVariants[4].C>(), cols_fits<N, kColsVariants[5].E, kColsVariants[5].C>(),
-        cols_fits<N, kColsVariants[6].E, kColsVariants[6].C>()};
-    constexpr bool r[kNumRowsVariants] = {rows_fits<N, kRowsRadix[0]>(), rows_fits<N, kRowsRadix[1]>()};
+#define OCN_FITS(v) cols_fits<N, kColsVariants[v].E, kColsVariants[v].C, kColsVariants[v].kind>()
+        OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
+        OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9)};
+#undef OCN_FITS
+#define OCN_RFITS(v) rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band>()
+    constexpr bool r[kNumRowsVariants] = {OCN_RFITS(0), OCN_RFITS(1), OCN_RFITS(2),
+                                          OCN_RFITS(3), OCN_RFITS(4), OCN_RFITS(5)};
+#undef OCN_RFITS
     return cols_v >= 0 && cols_v < kNumColsVariants && rows_v >= 0 && rows_v < kNumRowsVariants && c[cols_v] &&
            r[rows_v];
 }
@@ -552,7 +879,9 @@ void cols_variant_shape(int v, int* E, int* C)
     *C = kColsVariants[v].C;
 }
 
-int rows_variant_radix(int v) { return kRowsRadix[v]; }
+int rows_variant_radix(int v) { return kRowsVariants[v].E; }
+
+int rows_variant_band(int v) { return kRowsVariants[v].band; }
 
 hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* const* tw, hipStream_t s)
 {
@@ -579,7 +908,7 @@ hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, fl
 hipError_t launch_normals(int N, const float* height, const float* dx, const float* dz, float4* normals,
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_normals, dim3((N / 64) * (N / 4)), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
+    hipLaunchKernelGGL(k_normals, dim3((N / kNrmW) * (N / kNrmH)), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
                        lambda, inv2d, (size_t)0);
     return hipGetLastError();
 }

@@ -100,7 +100,7 @@ def lib():
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_autotune": [vp, i32, i32, vp],
             "ocn_set_schedule": [vp, i32, i32],
-            "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 5,
+            "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 6,
             "ocn_get_fields": [vp, F],
             "ocn_get_spectrum": [vp] + [ctypes.POINTER(vp)] * 4,
             "ocn_copy_fields_to_host": [vp] * 7,
@@ -334,9 +334,9 @@ class Ocean:
         check(lib().ocn_set_schedule(self.h, int(cols_variant), int(rows_variant)))
 
     def schedule(self, nframes=1):
-        v = [ctypes.c_int() for _ in range(5)]
+        v = [ctypes.c_int() for _ in range(6)]
         check(lib().ocn_get_schedule(self.h, int(nframes), *[ctypes.byref(x) for x in v]))
-        keys = ("cols_variant", "rows_variant", "cols_radix", "cols_per_block", "rows_radix")
+        keys = ("cols_variant", "rows_variant", "cols_radix", "cols_per_block", "rows_radix", "rows_band")
         return dict(zip(keys, [x.value for x in v]))
 
     # buffers

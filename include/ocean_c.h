@@ -197,11 +197,13 @@ int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const oc
  * it synchronises) and records the fastest for (device, N, frames-per-launch
  * class); later launches of any handle with that N use it.  ocn_set_schedule
  * pins a variant pair on this handle; ocn_get_schedule reports the pair used
- * for launches of nframes frames and its shape (any out-pointer may be NULL). */
+ * for launches of nframes frames and its shape (any out-pointer may be NULL);
+ * rows_band > 0 means the normal map and Jacobian are fused into the row pass,
+ * which walks bands of that many rows. */
 int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream);
 int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant);
 int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* rows_variant, int* cols_radix,
-                     int* cols_per_block, int* rows_radix);
+                     int* cols_per_block, int* rows_radix, int* rows_band);
 
 /* ---------------------------------------------------------------- buffers */
 
