@@ -41,6 +41,17 @@ CANON_FRAME = 96
 CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20, "k_rows_nrm": 56}
 LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
 
+# HBM bytes per texel-frame measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (each scaled by what the counter reads for a device-to-device copy of known
+# size: fetch x 2.0, write x 1.0) for this workload, keyed by kernel and the
+# (column, row) schedule it ran under; see
+# profiles/k_rows_nrm_k_cols_lds_n1024_batched_pmc_traffic.md.  Reported only
+# when the schedule in use matches, otherwise null.
+MEASURED_TRAFFIC = {
+    ("k_rows_nrm", 9, 5): 13.0 + 32.0,
+    ("k_cols_lds", 9, 5): 2.1 + 19.7,
+}
+
 
 def workload_params():
     import pyocean
@@ -169,6 +180,8 @@ def main():
     dom_bytes = CANON_STAGE[dominant.replace("k_cols_lds", "k_cols")] * nn * lf
     achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
+    tb = MEASURED_TRAFFIC.get((dominant, sched["cols_variant"], sched["rows_variant"]))
+    traffic = int(round(tb * nn * lf)) if tb is not None else None
 
     oc.close()
     for v in bufs.values():
@@ -210,7 +223,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": dom_bytes,
         },
         "cpu_baseline": cpu,
