@@ -10,9 +10,13 @@ normals and Jacobian, all on the GPU with inputs resident in HBM.
 Prints one JSON line (metric, value in frames/s, roofline of the dominant
 kernel measured with HIP events, CPU baseline of the FP64 oracle).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+Beside it, the line carries the batched-cascade comparison (64 oceans x 3
+cascades in two launches per frame against per-ocean launches).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--cascade-frames F]
   python bench.py --mode copy      (device-to-device copy used to calibrate
                                     rocprofv3 FETCH_SIZE / WRITE_SIZE)
+  python bench.py --mode cascades  (the batched-cascade comparison alone)
 """
 import argparse
 import json
@@ -82,6 +86,74 @@ def cpu_baseline(seconds):
                       "Jacobian) of the FP64 oracle in %.1f s, 1 thread" % (n, el)}
 
 
+def cascades_bench(hip, frames, warmup):
+    """The batched-cascade workload (64 oceans x 3 cascades, L/N = 2000/256,
+    250/512, 30/1024, wind U(5,30) m/s, direction U(0,2pi), lambda U(0.8,1.5))
+    computed three ways on one stream, each timed with HIP events: the batch
+    (two launches per frame for all 192 cascades) replayed as a HIP graph and
+    launched eagerly, and per-ocean launches (one fused single-ocean update
+    per cascade, the schedule it picks alone) from a host loop and replayed
+    from a captured graph of all 192 updates."""
+    import pyocean
+    members = pyocean.cascade_params()
+    texels = sum(m.N * m.N for m in members)
+    stream = hip.stream()
+    e0, e1 = hip.event(), hip.event()
+
+    def timed(fn, n):
+        hip.record(e0, stream)
+        for i in range(n):
+            fn(i)
+        hip.record(e1, stream)
+        return hip.elapsed_ms(e0, e1) / n
+
+    out = {"members": len(members), "texels_per_frame": texels}
+    with pyocean.Batch(members, stream=stream) as b:
+        b.tilde_h0_k()
+        for i in range(warmup):
+            b.update_graph(i / 60.0)
+        out["batch_graph_ms"] = timed(lambda i: b.update_graph(i / 60.0), frames)
+        for i in range(warmup):
+            b.update(i / 60.0)
+        out["batch_eager_ms"] = timed(lambda i: b.update(i / 60.0), frames)
+    hip.stream_sync(stream)
+    ocs = [pyocean.Ocean(m, stream=stream) for m in members]
+    try:
+        for oc in ocs:
+            oc.tilde_h0_k()
+            oc.update(0.0)
+        hip.stream_sync(stream)
+        po = max(3, frames // 10)
+
+        def per_ocean(i):
+            for oc in ocs:
+                oc.update(i / 60.0)
+
+        per_ocean(0)
+        out["per_ocean_loop_ms"] = timed(per_ocean, po)
+        g, ex = hip.capture(stream, lambda: per_ocean(1))
+        try:
+            for _ in range(3):
+                hip.graph_launch(ex, stream)
+            out["per_ocean_graph_ms"] = timed(lambda i: hip.graph_launch(ex, stream), po)
+        finally:
+            hip.stream_sync(stream)
+            hip.graph_destroy(g, ex)
+    finally:
+        for oc in ocs:
+            oc.close()
+    hip.stream_sync(stream)
+    hip.stream_destroy(stream)
+    for k in ("batch_graph_ms", "batch_eager_ms", "per_ocean_loop_ms", "per_ocean_graph_ms"):
+        out[k] = round(out[k], 4)
+    best = min(out["batch_graph_ms"], out["batch_eager_ms"])
+    out["batch_frames_per_s"] = round(1e3 / best, 2)
+    out["batch_canonical_GBps"] = round(CANON_FRAME * texels / (best * 1e-3) / 1e9, 1)
+    out["speedup_vs_per_ocean_loop"] = round(out["per_ocean_loop_ms"] / best, 3)
+    out["speedup_vs_per_ocean_graph"] = round(out["per_ocean_graph_ms"] / best, 3)
+    return out
+
+
 def run_copy(hip):
     """Device-to-device copies of 256 MiB (rocprofv3 counter calibration)."""
     nbytes = 256 << 20
@@ -101,7 +173,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["frames", "copy"], default="frames")
+    ap.add_argument("--mode", choices=["frames", "copy", "cascades"], default="frames")
+    ap.add_argument("--cascade-frames", type=int, default=60,
+                    help="frames of the batched-cascade comparison (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the ocean path runs on a single MI355X); "
@@ -115,6 +189,9 @@ def main():
         sys.exit(1)
     if args.mode == "copy":
         run_copy(hip)
+        return
+    if args.mode == "cascades":
+        print(json.dumps({"mode": "cascades", **cascades_bench(hip, max(1, args.cascade_frames), 10)}), flush=True)
         return
 
     nn = N * N
@@ -188,6 +265,7 @@ def main():
         hip.free(v)
     hip.stream_destroy(stream)
 
+    casc = cascades_bench(hip, args.cascade_frames, 10) if args.cascade_frames > 0 else None
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
     out = {
         "metric": "ocean-frames/s + HBM GB/s (% peak) at N=1024; batched & fwd+bwd vs plain on GPU",
@@ -227,6 +305,7 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
         },
         "cpu_baseline": cpu,
+        "batched_cascades": casc,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,0 +1,56 @@
+// ocean_batch.h -- internal interface of the batched kernels (ocean_batch.hip)
+// for the C ABI (ocean_capi.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ocean_kernels.h"
+
+namespace ocn {
+
+// Block shapes shared by every N in a batch.
+constexpr int kBatchColsThreads = 384;   // 3 * C * (N/16) with C = kBatchColsWidth / N
+constexpr int kBatchColsWidth   = 2048;  // C * N: columns per column block times N
+constexpr int kBatchRowsThreads = 256;   // (256 / T) groups of T = N/8 threads
+constexpr int kBatchMinN        = 64;
+constexpr int kBatchMaxN        = 2048;
+
+// Row-pass radix and band height per N: radix 8 keeps the fused row kernel
+// at two waves per SIMD; short bands at small N keep a row block busy.
+__host__ __device__ constexpr int batch_rows_radix(int) { return 8; }
+__host__ __device__ constexpr int batch_rows_band(int N) { return N <= 64 ? 2 : N <= 256 ? 8 : N <= 512 ? 16 : 32; }
+
+// Device-side descriptor of one batch member (one ocean cascade, one frame).
+struct BatchMember {
+    const float4* h0;
+    float2*       z1;
+    float2*       z2;
+    const float2* tw8;
+    const float2* tw16;
+    float*        height;
+    float*        dx;
+    float*        dz;
+    float4*       normals;
+    float*        jac;
+    SpecConst     sc;
+    float         lambda;
+    float         inv2d;
+};
+
+// Work items: int2 (member, strip) for the column pass, (member, first band)
+// for the row pass (each row block covers 256 / T consecutive bands).
+inline int batch_cols_items_per_member(int N) { return (N / 2) / (kBatchColsWidth / N) + 1; }
+inline int batch_rows_items_per_member(int N)
+{
+    const int T = N / batch_rows_radix(N), gpb = kBatchRowsThreads / T, bands = N / batch_rows_band(N);
+    return (bands + gpb - 1) / gpb;
+}
+inline int batch_rows_groups_per_block(int N) { return kBatchRowsThreads / (N / batch_rows_radix(N)); }
+
+hipError_t  launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
+                               int n_rows, double t_rev, hipStream_t s);
+const void* batch_cols_kernel();
+const void* batch_rows_kernel();
+
+}  // namespace ocn

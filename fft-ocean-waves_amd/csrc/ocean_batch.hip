@@ -1,0 +1,98 @@
+// ocean_batch.hip -- batched frames of many oceans / cascades of uneven N in one
+// launch per pass.  No reference counterpart: the reference simulates a single
+// N=256 ocean (main.cpp:17); this is the multi-cascade extension.
+//
+// A batch member is one ocean cascade (its own N, parameters, h0, intermediate
+// and outputs).  A frame is two launches over device tables of work items:
+//   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, 2048/N>, 384 threads
+//   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8, R>,     256 threads
+// Both block shapes are the same for every N (C = 2048/N columns per column
+// block, 256/T band groups per row block), so members of different N share a
+// launch; the kernels dispatch on the member's N.  Blocks are independent (no
+// inter-block waits).  Items are ordered member-major, and the XCD remap keeps
+// neighbouring items, which share Z lines and halo rows, on one XCD.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "ocean_batch.h"
+#include "ocean_device.hpp"
+
+namespace ocn {
+
+namespace {
+
+constexpr int kColsLds = 3 * 2176;  // 3 * C * lds_len(N) with C*N = 2048
+constexpr int kRowsLds = 2176;      // (256 / T) * lds_len(N) with T = N/8
+
+template <int N>
+__device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip, double t_rev, float2* lds)
+{
+    constexpr int C = kBatchColsWidth / N;
+    using Sync = typename std::conditional<(N / 16 <= 64), WaveSync, BlockSync>::type;
+    cols_lds_strip<N, C, Sync>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
+}
+
+template <int N>
+__device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_band, float2* lds)
+{
+    constexpr int E = batch_rows_radix(N), T = N / E, R = batch_rows_band(N);
+    static_assert(T <= kBatchRowsThreads && kBatchRowsThreads % T == 0, "whole groups per row block");
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    const int g = threadIdx.x / T, j = threadIdx.x % T;
+    const int band = first_band + g;
+    if (band >= N / R) return;  // whole group (wave-aligned when GPB > 1)
+    rows_nrm_band<N, E, R, Sync>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac, E == 8 ? m.tw8 : m.tw16,
+                                 lds + g * lds_len(N), j, band, m.lambda, m.inv2d);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(kBatchColsThreads) void k_batch_cols(const BatchMember* __restrict__ members,
+                                                                  const int2* __restrict__ items, double t_rev)
+{
+    __shared__ float2 lds[kColsLds];
+    const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchMember m  = members[it.x];
+    switch (m.sc.N) {
+    case 64: batch_cols_item<64>(m, it.y, t_rev, lds); break;
+    case 128: batch_cols_item<128>(m, it.y, t_rev, lds); break;
+    case 256: batch_cols_item<256>(m, it.y, t_rev, lds); break;
+    case 512: batch_cols_item<512>(m, it.y, t_rev, lds); break;
+    case 1024: batch_cols_item<1024>(m, it.y, t_rev, lds); break;
+    case 2048: batch_cols_item<2048>(m, it.y, t_rev, lds); break;
+    default: break;
+    }
+}
+
+__global__ __launch_bounds__(kBatchRowsThreads) void k_batch_rows(const BatchMember* __restrict__ members,
+                                                                  const int2* __restrict__ items)
+{
+    __shared__ float2 lds[kRowsLds];
+    const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchMember m  = members[it.x];
+    switch (m.sc.N) {
+    case 64: batch_rows_item<64>(m, it.y, lds); break;
+    case 128: batch_rows_item<128>(m, it.y, lds); break;
+    case 256: batch_rows_item<256>(m, it.y, lds); break;
+    case 512: batch_rows_item<512>(m, it.y, lds); break;
+    case 1024: batch_rows_item<1024>(m, it.y, lds); break;
+    case 2048: batch_rows_item<2048>(m, it.y, lds); break;
+    default: break;
+    }
+}
+
+hipError_t launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
+                              int n_rows, double t_rev, hipStream_t s)
+{
+    if (n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_cols, dim3(n_cols), dim3(kBatchColsThreads), 0, s, members, cols_items, t_rev);
+    hipLaunchKernelGGL(k_batch_rows, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+    return hipGetLastError();
+}
+
+const void* batch_cols_kernel() { return reinterpret_cast<const void*>(&k_batch_cols); }
+const void* batch_rows_kernel() { return reinterpret_cast<const void*>(&k_batch_rows); }
+
+}  // namespace ocn

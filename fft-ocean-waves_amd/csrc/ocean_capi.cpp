@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/ocean_c.h"
+#include "ocean_batch.h"
 #include "ocean_kernels.h"
 
 namespace {
@@ -318,6 +320,75 @@ int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* ou
 }  // namespace
 
 // ====================================================================== C ABI
+
+// ====================================================================== batches
+
+struct ocn_batch {
+    int                                    device = 0;
+    std::vector<ocn_params>                params;
+    std::vector<ocn::BatchMember>          members;  // host copy of the device descriptors
+    ocn::BatchMember*                      d_members = nullptr;
+    int2*                                  d_cols    = nullptr;
+    int2*                                  d_rows    = nullptr;
+    int                                    n_cols    = 0;
+    int                                    n_rows    = 0;
+    char*                                  pool      = nullptr;  // every member's buffers
+    std::map<int, std::array<float2*, 2>>  tw;  // per N: radix-8 and radix-16 Stockham twiddles
+    bool                                   h0_ready  = false;
+    hipGraph_t                             graph     = nullptr;
+    hipGraphExec_t                         exec      = nullptr;
+    hipGraphNode_t                         node_cols = nullptr;
+    hipGraphNode_t                         node_rows = nullptr;
+};
+
+namespace {
+
+void batch_free(ocn_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* ptrs[] = {b->d_members, b->d_cols, b->d_rows, b->pool};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (auto& kv : b->tw)
+        for (float2* q : kv.second)
+            if (q) (void)hipFree(q);
+}
+
+int upload_table(float2** dst, const std::vector<float>& t)
+{
+    int rc = dalloc(dst, t.size() / 2 + 1, "twiddles");
+    if (rc || t.empty()) return rc;
+    hipError_t e = hipMemcpy(*dst, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpy(twiddles)");
+}
+
+template <class T>
+int upload_vector(T** dst, const std::vector<T>& v, const char* what)
+{
+    int rc = dalloc(dst, v.size(), what);
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, what);
+}
+
+int check_batch(const ocn_batch* b)
+{
+    if (!b) return fail(OCN_EINVAL, "batch is NULL");
+    return OCN_OK;
+}
+
+int batch_ready(const ocn_batch* b, double t)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!std::isfinite(t)) return fail(OCN_EINVAL, "t must be finite");
+    if (!b->h0_ready) return fail(OCN_ESTATE, "the spectra do not exist yet: call ocn_batch_tilde_h0_k first");
+    hipError_t e = hipSetDevice(b->device);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipSetDevice");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -711,6 +782,199 @@ int ocn_twiddle_factors(int N, float* out)
             o[3] = (float)ib;
         }
     }
+    return OCN_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
+{
+    if (!out) return fail(OCN_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!params || count < 1 || count > OCN_BATCH_MAX_MEMBERS)
+        return fail(OCN_EINVAL, "params must be non-NULL and count in [1, %d]", OCN_BATCH_MAX_MEMBERS);
+    for (int i = 0; i < count; ++i) {
+        if (validate(&params[i])) return fail(OCN_EINVAL, "member %d: %s", i, std::string(g_err).c_str());
+        if (params[i].N < ocn::kBatchMinN || params[i].N > ocn::kBatchMaxN)
+            return fail(OCN_EINVAL, "member %d: batched N must be in [%d, %d]", i, ocn::kBatchMinN, ocn::kBatchMaxN);
+        if (params[i].normal_mode != OCN_NORMALS_NEAREST)
+            return fail(OCN_EINVAL, "member %d: the batched frame fuses the nearest-tap normal map only", i);
+    }
+    ocn_batch* b = new (std::nothrow) ocn_batch();
+    if (!b) return fail(OCN_ENOMEM, "host allocation of the batch failed");
+    b->params.assign(params, params + count);
+    int        rc = OCN_OK;
+    hipError_t e  = hipGetDevice(&b->device);
+    if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
+    // Per member, 256-byte aligned: h0 16, z1 8, z2 4, height/dx/dz/jacobian 4 each, normals 16 bytes per texel.
+    auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
+    size_t total = 0;
+    for (const ocn_params& p : b->params) {
+        const size_t nn = (size_t)p.N * p.N;
+        total += al(nn * 16) + al(nn * 8) + al(nn * 4) + 4 * al(nn * 4) + al(nn * 16);
+    }
+    if (!rc) rc = dalloc(&b->pool, total, "batch pool");
+    for (const ocn_params& p : b->params) {
+        if (rc || b->tw.count(p.N)) continue;
+        std::array<float2*, 2>& t = b->tw[p.N];
+        t = {nullptr, nullptr};
+        if (!(rc = upload_table(&t[0], stockham_twiddles(p.N, 8)))) rc = upload_table(&t[1], stockham_twiddles(p.N, 16));
+    }
+    if (!rc) {
+        char*             q = b->pool;
+        std::vector<int2> cols, rows;
+        for (int i = 0; i < count; ++i) {
+            const ocn_params& p  = b->params[i];
+            const size_t      nn = (size_t)p.N * p.N;
+            ocn::BatchMember  m{};
+            m.h0      = reinterpret_cast<const float4*>(q), q += al(nn * 16);
+            m.z1      = reinterpret_cast<float2*>(q), q += al(nn * 8);
+            m.z2      = reinterpret_cast<float2*>(q), q += al(nn * 4);
+            m.height  = reinterpret_cast<float*>(q), q += al(nn * 4);
+            m.dx      = reinterpret_cast<float*>(q), q += al(nn * 4);
+            m.dz      = reinterpret_cast<float*>(q), q += al(nn * 4);
+            m.jac     = reinterpret_cast<float*>(q), q += al(nn * 4);
+            m.normals = reinterpret_cast<float4*>(q), q += al(nn * 16);
+            m.tw8     = b->tw[p.N][0];
+            m.tw16    = b->tw[p.N][1];
+            m.sc      = spec_const(p);
+            m.lambda  = (float)p.choppiness;
+            m.inv2d   = (float)((double)p.N / (2.0 * p.L));
+            b->members.push_back(m);
+            for (int k = 0; k < ocn::batch_cols_items_per_member(p.N); ++k) cols.push_back(make_int2(i, k));
+            const int gpb = ocn::batch_rows_groups_per_block(p.N);
+            for (int k = 0; k < ocn::batch_rows_items_per_member(p.N); ++k) rows.push_back(make_int2(i, k * gpb));
+        }
+        b->n_cols = (int)cols.size();
+        b->n_rows = (int)rows.size();
+        if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) &&
+            !(rc = upload_vector(&b->d_cols, cols, "batch column items")))
+            rc = upload_vector(&b->d_rows, rows, "batch row items");
+    }
+    if (rc) {
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return OCN_OK;
+}
+
+int ocn_batch_destroy(ocn_batch* b)
+{
+    if (!b) return OCN_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+    delete b;
+    return OCN_OK;
+}
+
+int ocn_batch_size(const ocn_batch* b, int* count)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!count) return fail(OCN_EINVAL, "count is NULL");
+    *count = (int)b->params.size();
+    return OCN_OK;
+}
+
+int ocn_batch_get_params(const ocn_batch* b, int member, ocn_params* p)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!p) return fail(OCN_EINVAL, "p is NULL");
+    if (member < 0 || member >= (int)b->params.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->params.size());
+    *p = b->params[member];
+    return OCN_OK;
+}
+
+int ocn_batch_tilde_h0_k(ocn_batch* b, void* stream)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        e = ocn::launch_h0(b->params[i].N, const_cast<float4*>(b->members[i].h0), nullptr, b->members[i].sc,
+                           (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "batch tilde_h0_k launch");
+    }
+    b->h0_ready = true;
+    return OCN_OK;
+}
+
+int ocn_batch_update(ocn_batch* b, double t, void* stream)
+{
+    int rc = batch_ready(b, t);
+    if (rc) return rc;
+    hipError_t e = ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows,
+                                           t / (2.0 * kPi), (hipStream_t)stream);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "batch update launch");
+}
+
+int ocn_batch_update_graph(ocn_batch* b, double t, void* stream)
+{
+    int rc = batch_ready(b, t);
+    if (rc) return rc;
+    // The graph holds two kernel nodes (columns -> rows) and nothing else; t
+    // reaches the column node through its parameters on every replay.
+    double              t_rev       = t / (2.0 * kPi);
+    void*               cols_args[] = {&b->d_members, &b->d_cols, &t_rev};
+    void*               rows_args[] = {&b->d_members, &b->d_rows};
+    hipKernelNodeParams kc{};
+    kc.func           = const_cast<void*>(ocn::batch_cols_kernel());
+    kc.gridDim        = dim3(b->n_cols);
+    kc.blockDim       = dim3(ocn::kBatchColsThreads);
+    kc.sharedMemBytes = 0;
+    kc.kernelParams   = cols_args;
+    kc.extra          = nullptr;
+    hipError_t e;
+    if (!b->exec) {
+        hipKernelNodeParams kr = kc;
+        kr.func                = const_cast<void*>(ocn::batch_rows_kernel());
+        kr.gridDim             = dim3(b->n_rows);
+        kr.blockDim            = dim3(ocn::kBatchRowsThreads);
+        kr.kernelParams        = rows_args;
+        if ((e = hipGraphCreate(&b->graph, 0)) != hipSuccess) {
+            b->graph = nullptr;
+            return hip_fail(e, "hipGraphCreate");
+        }
+        if ((e = hipGraphAddKernelNode(&b->node_cols, b->graph, nullptr, 0, &kc)) != hipSuccess ||
+            (e = hipGraphAddKernelNode(&b->node_rows, b->graph, &b->node_cols, 1, &kr)) != hipSuccess)
+            return hip_fail(e, "hipGraphAddKernelNode");
+        if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) {
+            b->exec = nullptr;
+            return hip_fail(e, "hipGraphInstantiate");
+        }
+    } else if ((e = hipGraphExecKernelNodeSetParams(b->exec, b->node_cols, &kc)) != hipSuccess) {
+        return hip_fail(e, "hipGraphExecKernelNodeSetParams");
+    }
+    e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!out) return fail(OCN_EINVAL, "out is NULL");
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    const ocn::BatchMember& m = b->members[member];
+    *out = ocn_fields{m.height, m.dx, m.dz, reinterpret_cast<float*>(m.normals), m.jac};
+    return OCN_OK;
+}
+
+int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!h0) return fail(OCN_EINVAL, "h0 is NULL");
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    *h0 = const_cast<float4*>(b->members[member].h0);
     return OCN_OK;
 }
 

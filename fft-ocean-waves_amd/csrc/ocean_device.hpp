@@ -1,0 +1,364 @@
+// ocean_device.hpp -- device functions shared by the single-ocean kernels
+// (ocean_kernels.hip) and the batched kernels (ocean_batch.hip): noise and the
+// Phillips spectrum, the dispersion phase, the Hermitian fold, the XCD-aware
+// block order, the LDS-staged column transform of one strip and the fused
+// row + normal-map + Jacobian walk of one band.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fft_core.hpp"
+#include "ocean_kernels.h"
+
+namespace ocn {
+
+// ------------------------------------------------------------------ noise
+
+// Philox4x32-10 (Salmon et al., SC'11).
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint2 k)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k.x += 0x9E3779B9u; k.y += 0xBB67AE85u; }
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+    }
+    return c;
+}
+
+// (m + 1/2) * 2^-23 from the top 23 bits: exact in FP32.
+__device__ __forceinline__ float u01(uint32_t x) { return ((float)(x >> 9) + 0.5f) * (1.0f / 8388608.0f); }
+
+// gauss_rnd (tilde_h0_k_cs.glsl:51-68): clamp to [0.001, 1], two Box-Muller pairs.
+__device__ __forceinline__ float4 box_muller(float n0, float n1, float n2, float n3)
+{
+    n0 = fminf(fmaxf(n0, 0.001f), 1.0f);
+    n1 = fminf(fmaxf(n1, 0.001f), 1.0f);
+    n2 = fminf(fmaxf(n2, 0.001f), 1.0f);
+    n3 = fminf(fmaxf(n3, 0.001f), 1.0f);
+    float s0, c0, s1, c1;
+    sincospif(2.0f * n0, &s0, &c0);
+    sincospif(2.0f * n2, &s1, &c1);
+    const float r0 = sqrtf(-2.0f * logf(n1));
+    const float r1 = sqrtf(-2.0f * logf(n3));
+    return make_float4(r0 * c0, r0 * s0, r1 * c1, r1 * s1);
+}
+
+// clamp(sqrt(P(k))/sqrt(2), -4000, 4000) (tilde_h0_k_cs.glsl:35-45, 76-88).
+// k_hat = k / |k|_clamped: equal to normalize(k) for k != 0, and 0 at DC where
+// normalize(0) is undefined.  pow(d, 2.0) is d*d.
+__device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int ty)
+{
+    const float kx = sc.kscale * (float)(tx - sc.N / 2);
+    const float ky = sc.kscale * (float)(ty - sc.N / 2);
+    float kmag = sqrtf(kx * kx + ky * ky);
+    if (kmag < 0.00001f) kmag = 0.00001f;
+    const float k2  = kmag * kmag;
+    const float sup = expf(-k2 * sc.l2);
+    const float d   = (kx * sc.wx + ky * sc.wy) / kmag;
+    const float P   = sc.A * expf(-1.0f / (k2 * sc.Lp2)) * (d * d) * sup / (k2 * k2);
+    return fminf(sqrtf(P) * 0.70710678118654752f, 4000.0f);
+}
+
+// ------------------------------------------------------------------ A4 time evolution
+
+// Dispersion phase w t / (2 pi) with w = sqrt(g |k|_clamped) (tilde_h0_t_cs.glsl:77-79),
+// as a fraction of a turn in [-1/2, 1/2].  w t reaches hundreds of radians,
+// where FP32 rounding of w alone would cost ~1e-5 rad, so w is refined in FP64:
+// an FP32 estimate w0 plus one Newton step on w^4 = g^2 |k|^2 (|k|^2 is exact
+// from the integer wave numbers), then the product with t / (2 pi) is reduced
+// to one turn before the FP32 sincos.  m = nx^2 + ny^2.
+__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
+{
+    kf = fmaxf(sc.kscale * __builtin_amdgcn_sqrtf((float)m), 0.00001f);
+    const float  w0 = __builtin_amdgcn_sqrtf(sc.g * kf);
+    const double Q  = fmax(sc.k2_d * (double)m, 1e-10) * sc.g2_d;  // g^2 |k|_clamped^2
+    const double w  = (double)w0;
+    const double w2 = w * w;
+    const double w1 = w - (w2 * w2 - Q) * (double)(0.25f * __builtin_amdgcn_rcpf(w0 * w0 * w0));
+    const double rev = w1 * t_rev;
+    return (float)(rev - rint(rev));
+}
+
+// sin and cos of 2 pi fr for fr in [-1/2, 1/2]: reduction by quarter turns
+// q = rint(4 fr) (the remainder r = fr - q/4 is exact), Taylor polynomials of
+// sin and cos on |2 pi r| <= pi/4 to degree 9 and 10 (truncation < 2e-9),
+// then rotation by q quarter turns.
+__device__ __forceinline__ void sincos_turns(float fr, float& s, float& c)
+{
+    const float q  = rintf(4.0f * fr);
+    const float th = (fr - 0.25f * q) * 6.28318530717958647692f;
+    const float t2 = th * th;
+    const float sp = th * fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 2.75573192239858907e-6f, -1.98412698412698413e-4f),
+                                                 8.33333333333333333e-3f), -1.66666666666666667e-1f), 1.0f);
+    const float cp = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192239858907e-7f, 2.48015873015873016e-5f),
+                                                       -1.38888888888888889e-3f), 4.16666666666666667e-2f), -0.5f),
+                          1.0f);
+    const int qi = (int)q & 3;
+    s = qi == 0 ? sp : qi == 1 ? cp : qi == 2 ? -sp : -cp;
+    c = qi == 0 ? cp : qi == 1 ? -sp : qi == 2 ? -cp : sp;
+}
+
+// a e^{i th} + C(b) e^{-i th} for h = (a, b) = (h0k, h0mk) (tilde_h0_t_cs.glsl:94-110).
+__device__ __forceinline__ float2 evolve_amp(const SpecConst& sc, float4 h, float s, float c)
+{
+    const float bi = sc.conj_mode ? -h.w : h.w;
+    return make_float2(fmaf(h.x, c, -h.y * s) + fmaf(h.z, c, bi * s), fmaf(h.x, s, h.y * c) + fmaf(bi, c, -h.z * s));
+}
+
+// -i m * H  (tilde_h0_t_cs.glsl:113-126)
+__device__ __forceinline__ float2 choppy(float m, float2 h) { return make_float2(m * h.y, -m * h.x); }
+
+// Hermitian parts (F(t) + conj F(t'))/2 of the three spectra at texel t and its
+// mirror t' = -t mod N.  Both share |k| (n' = -n except on the Nyquist row or
+// column, where n' = n = -N/2), hence one phase and one sincos for the pair.
+// p = h0 at t, q = h0 at t'; (nx, ny) and (qx, qy) are their centred indices.
+__device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
+                                          double t_rev, float2& ay, float2& ax, float2& az)
+{
+    float kf, s, c;
+    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kf), s, c);
+    const float  kinv = sc.kscale * __builtin_amdgcn_rcpf(kf);
+    const float2 H = evolve_amp(sc, p, s, c);
+    const float2 G = evolve_amp(sc, q, s, c);
+    const float mx = (float)nx * kinv, mz = (float)ny * kinv;
+    const float gx = (float)qx * kinv, gz = (float)qy * kinv;
+    ay = make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
+    // -i m H = (m H.y, -m H.x);  conj(-i g G) = (g G.y, g G.x)
+    ax = make_float2(0.5f * (mx * H.y + gx * G.y), 0.5f * (gx * G.x - mx * H.x));
+    az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
+}
+
+// ------------------------------------------------------------------ XCD-aware block order
+
+// MI355X spreads the workgroups of a launch round-robin over its 8 XCDs
+// (workgroup b runs on XCD b % 8), and each XCD has its own L2.  This maps the
+// hardware block index to a logical index such that XCD x processes the
+// contiguous logical range [x*q + min(x, r), ...) in order (q = n / 8,
+// r = n % 8): consecutive logical blocks share an L2.  It is a bijection on
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_logical(int b, int n)
+{
+    constexpr int X = 8;
+    const int q = n / X, r = n % X, x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+}
+
+// Column strips per block group: G strips of C columns fill 128 B of a Z row.
+template <int C>
+__host__ __device__ constexpr int strip_group() { return C >= 16 ? 1 : 16 / C; }
+
+
+// ------------------------------------------------------------------ column pass: one strip
+
+// LDS-staged column transforms of one strip: C half-plane columns starting at
+// tx = strip*C (tx <= N/2), block of 3*C*T threads (T = N/16), frame time
+// t_rev; z1/z2 point at the frame's intermediate, lds holds 3*C*lds_len(N).
+// Phase A: every thread evaluates Hermitian parts (consecutive threads take
+// consecutive columns, then rows) into per-(field, column) LDS buffers.
+// Phase B: each of the 3C transforms runs inside one T-thread group (Sync:
+// WaveSync when a group lies in one wave, else BlockSync) and leaves its
+// result in natural order in the same buffer.  Phase C: every thread stores
+// Z1 = G_h + i G_dx (columns tx and N-tx) and Z2 = G_dz(2p) + i G_dz(2p+1).
+template <int N, int C, class Sync>
+__device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
+                                               float2* __restrict__ z2, const float2* __restrict__ tw,
+                                               const SpecConst& sc, double t_rev, int strip, float2* lds)
+{
+    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
+    constexpr int ITERS = (C * N + NT - 1) / NT;
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        if (k < C * N) {
+            const int c   = k % C, ty = k / C;
+            const int txc = min(strip * C + c, N / 2);
+            const int mxc = (N - txc) & (N - 1);
+            const int my  = (N - ty) & (N - 1);
+            float2 ay, ax, az;
+            pair_fold(sc, h0[(size_t)ty * N + txc], h0[(size_t)my * N + mxc], txc - N / 2, ty - N / 2, mxc - N / 2,
+                      my - N / 2, t_rev, ay, ax, az);
+            lds[(0 * C + c) * LEN + lds_pad(ty)] = ay;
+            lds[(1 * C + c) * LEN + lds_pad(ty)] = ax;
+            lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
+        }
+    }
+    __syncthreads();
+    {
+        const int g = threadIdx.x / T, j = threadIdx.x % T;
+        float2* buf = lds + g * LEN;
+        float2  v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+        Sync{}();
+#pragma unroll
+        for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        if (k < C * N) {
+            const int c = k % C, y = k / C;
+            const int tx = strip * C + c;
+            if (tx <= N / 2) {
+                const int    mxc = (N - tx) & (N - 1);
+                const float2 gy  = lds[(0 * C + c) * LEN + lds_pad(y)];
+                const float2 gx  = lds[(1 * C + c) * LEN + lds_pad(y)];
+                const size_t row = (size_t)y * N;
+                z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
+                if (mxc != tx) z1[row + mxc] = make_float2(gy.x + gx.y, gx.x - gy.y);
+                if (y < N / 2) {  // Z2 row y pairs G_dz rows 2y and 2y+1
+                    const float2 lo = lds[(2 * C + c) * LEN + lds_pad(2 * y)];
+                    const float2 hi = lds[(2 * C + c) * LEN + lds_pad(2 * y + 1)];
+                    z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
+                    if (mxc != tx) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ row pass + normals + Jacobian: one band
+
+// Row transforms fused with the normal map (nearest taps) and the Jacobian
+// for one band of R rows [y0, y0 + R), y0 = band*R even.  A group of T = N/E
+// threads (thread j holds x = j + r*T) walks the band with a three-row window
+// in registers: for each new row it transforms Z1 (height + i Dx) and, on
+// even rows, the Z2 row holding Dz of that row and the next, then applies
+// (-1)^(x+y) / N^2 (inversion_cs.glsl:35-41).  The x-neighbours of the stencil
+// pass through the group's LDS buffer buf (lds_len(N) entries); halo rows
+// y0-1 and y0+R are transformed too (periodic).  Sobel taps as
+// normal_map_cs.glsl:37-53; J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
+// for the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), central
+// differences with spacing L/N.  All pointers are at the frame's data;
+// normals or jac may be null.
+template <int N, int E, int R, class Sync>
+__device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, const float2* __restrict__ z2,
+                                              float* __restrict__ height, float* __restrict__ dx,
+                                              float* __restrict__ dz, float4* __restrict__ normals,
+                                              float* __restrict__ jac, const float2* __restrict__ tw, float2* buf,
+                                              int j, int band, float lambda, float inv2d)
+{
+    constexpr int T = N / E;
+    static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
+    constexpr float scale = 1.0f / ((float)N * (float)N);
+    const int y0 = band * R;
+
+    // inverse transform of one Z row into v (natural order, thread j holds x = j + r*T)
+    auto fft_row = [&](const float2* src, float2 (&v)[E]) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+    };
+    // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y
+    auto sgn = [&](int y) { return ((j + y) & 1) ? -scale : scale; };
+
+    float hA[E], xA[E], zA[E], hB[E], xB[E], zB[E], zN[E];
+    float2 v[E];
+    {
+        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the imaginary part of pair yA >> 1
+        fft_row(z2 + (size_t)(yA >> 1) * N, v);
+        const float sa = sgn(yA);
+#pragma unroll
+        for (int r = 0; r < E; ++r) zA[r] = sa * v[r].y;
+        fft_row(z1 + (size_t)yA * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            hA[r] = sa * v[r].x;
+            xA[r] = sa * v[r].y;
+        }
+        const float sb = sgn(y0);
+        fft_row(z2 + (size_t)(y0 >> 1) * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            zB[r] = sb * v[r].x;
+            zN[r] = -sb * v[r].y;  // Dz of row y0 + 1 (opposite sign)
+        }
+        fft_row(z1 + (size_t)y0 * N, v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            hB[r] = sb * v[r].x;
+            xB[r] = sb * v[r].y;
+        }
+    }
+
+#pragma unroll 1
+    for (int k = 0; k < R; ++k) {
+        const int   y  = y0 + k;
+        const int   yC = (y + 1) & (N - 1);
+        const float sc = sgn(yC);
+        float zC[E];
+        if ((yC & 1) == 0) {
+            fft_row(z2 + (size_t)(yC >> 1) * N, v);
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                zC[r] = sc * v[r].x;
+                zN[r] = -sc * v[r].y;
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < E; ++r) zC[r] = zN[r];
+        }
+        fft_row(z1 + (size_t)yC * N, v);  // v = height + i Dx of row yC (unsigned)
+
+        const size_t row = (size_t)y * N;
+        if (normals) {
+            // v_s = h(y-1) + 2 h(y) + h(y+1), d_s = h(y-1) - h(y+1); nx = v_s(x-1) - v_s(x+1),
+            // nz = d_s(x-1) + 2 d_s(x) + d_s(x+1)  (normal_map_cs.glsl:49-51)
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const float hC = sc * v[r].x;
+                buf[lds_pad(j + r * T)] = make_float2(hA[r] + 2.0f * hB[r] + hC, hA[r] - hC);
+            }
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int    x  = j + r * T;
+                const float2 lf = buf[lds_pad((x - 1) & (N - 1))];
+                const float2 rt = buf[lds_pad((x + 1) & (N - 1))];
+                const float2 md = buf[lds_pad(x)];
+                const float  nx = lf.x - rt.x;
+                const float  nz = lf.y + 2.0f * md.y + rt.y;
+                const float  inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
+                normals[row + x] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+            }
+        }
+        if (jac) {
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = make_float2(xB[r], zB[r]);
+            Sync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int    x    = j + r * T;
+                const float2 lf   = buf[lds_pad((x - 1) & (N - 1))];
+                const float2 rt   = buf[lds_pad((x + 1) & (N - 1))];
+                const float  dxdx = (rt.x - lf.x) * inv2d;
+                const float  dzdx = (rt.y - lf.y) * inv2d;
+                const float  dxdz = (sc * v[r].y - xA[r]) * inv2d;
+                const float  dzdz = (zC[r] - zA[r]) * inv2d;
+                jac[row + x] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const int x = j + r * T;
+            height[row + x] = hB[r];
+            dx[row + x]     = xB[r];
+            dz[row + x]     = zB[r];
+            hA[r] = hB[r];
+            xA[r] = xB[r];
+            zA[r] = zB[r];
+            hB[r] = sc * v[r].x;
+            xB[r] = sc * v[r].y;
+            zB[r] = zC[r];
+        }
+    }
+}
+
+}  // namespace ocn

@@ -31,6 +31,9 @@ EXPORTS = [
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
     "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
+    "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
+    "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
+    "ocn_batch_get_fields", "ocn_batch_get_h0",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
 
@@ -106,6 +109,15 @@ def lib():
             "ocn_copy_fields_to_host": [vp] * 7,
             "ocn_bit_reversed_indices": [i32, ctypes.POINTER(ctypes.c_int32)],
             "ocn_twiddle_factors": [i32, ctypes.POINTER(ctypes.c_float)],
+            "ocn_batch_create": [P, i32, ctypes.POINTER(vp)],
+            "ocn_batch_destroy": [vp],
+            "ocn_batch_size": [vp, ctypes.POINTER(i32)],
+            "ocn_batch_get_params": [vp, i32, P],
+            "ocn_batch_tilde_h0_k": [vp, vp],
+            "ocn_batch_update": [vp, dbl, vp],
+            "ocn_batch_update_graph": [vp, dbl, vp],
+            "ocn_batch_get_fields": [vp, i32, F],
+            "ocn_batch_get_h0": [vp, i32, ctypes.POINTER(vp)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -181,6 +193,9 @@ class Hip:
             "hipEventSynchronize": [vp],
             "hipEventElapsedTime": [ctypes.POINTER(ctypes.c_float), vp, vp],
             "hipGetLastError": [],
+            "hipStreamBeginCapture": [vp, ctypes.c_int], "hipStreamEndCapture": [vp, ctypes.POINTER(vp)],
+            "hipGraphInstantiate": [ctypes.POINTER(vp), vp, vp, vp, sz], "hipGraphLaunch": [vp, vp],
+            "hipGraphExecDestroy": [vp], "hipGraphDestroy": [vp],
         }.items():
             fn = getattr(r, name)
             fn.argtypes = args
@@ -230,6 +245,26 @@ class Hip:
 
     def stream_destroy(self, s):
         self.ok(self.rt.hipStreamDestroy(s), "hipStreamDestroy")
+
+    def capture(self, stream, fn):
+        """Records what fn() launches on `stream` into a graph (thread-local
+        capture mode); returns (graph, exec)."""
+        self.ok(self.rt.hipStreamBeginCapture(stream, 1), "hipStreamBeginCapture")
+        g = ctypes.c_void_p()
+        try:
+            fn()
+        finally:
+            self.ok(self.rt.hipStreamEndCapture(stream, ctypes.byref(g)), "hipStreamEndCapture")
+        ex = ctypes.c_void_p()
+        self.ok(self.rt.hipGraphInstantiate(ctypes.byref(ex), g, None, None, 0), "hipGraphInstantiate")
+        return g.value, ex.value
+
+    def graph_launch(self, ex, stream):
+        self.ok(self.rt.hipGraphLaunch(ex, stream), "hipGraphLaunch")
+
+    def graph_destroy(self, g, ex):
+        self.ok(self.rt.hipGraphExecDestroy(ex), "hipGraphExecDestroy")
+        self.ok(self.rt.hipGraphDestroy(g), "hipGraphDestroy")
 
     def event(self):
         e = ctypes.c_void_p()
@@ -372,3 +407,102 @@ class Ocean:
         _, hy, hx, hz = self.spectrum_ptrs()
         N = self.N
         return [self.hip.download(q, (N, N), np.complex64) for q in (hy, hx, hz)]
+
+
+# ---------------------------------------------------------------- batches of oceans / cascades
+
+def cascade_params(n_oceans=64, cascades=((2000.0, 256), (250.0, 512), (30.0, 1024)), seed=42,
+                   wind_speed=(5.0, 30.0), choppiness=(0.8, 1.5), amplitude=4.0):
+    """Members of a batch of independent oceans, each with several cascades.
+
+    Ocean o draws its wind speed ~ U(wind_speed), direction angle ~ U(0, 2 pi)
+    and choppiness ~ U(choppiness) from numpy's Philox generator keyed by
+    seed + o; all its cascades share them.  Cascade c of ocean o has the
+    given (L, N), suppression L/2000, and noise seed `seed` with
+    ocean_id = o, cascade_id = c in the Philox counter, so every member's
+    noise is independent.  Member order: ocean-major."""
+    out = []
+    for o in range(n_oceans):
+        rng = np.random.Generator(np.random.Philox(key=seed + o))
+        v = float(rng.uniform(*wind_speed))
+        th = float(rng.uniform(0.0, 2.0 * np.pi))
+        lam = float(rng.uniform(*choppiness))
+        for c, (L, N) in enumerate(cascades):
+            out.append(make_params(N=N, L=L, wind_speed=v, wind_dir=(np.cos(th), np.sin(th)), amplitude=amplitude,
+                                   choppiness=lam, seed=seed, ocean_id=o, cascade_id=c))
+    return out
+
+
+class Batch:
+    """Many oceans / cascades (uneven N) computed by one pair of launches per frame."""
+
+    def __init__(self, members, stream=None):
+        arr = (OcnParams * len(members))(*members)
+        h = ctypes.c_void_p()
+        check(lib().ocn_batch_create(arr, len(members), ctypes.byref(h)))
+        self.h = h.value
+        self.members = list(members)
+        self.stream = stream
+        self.hip = Hip()
+
+    def close(self):
+        if self.h:
+            check(lib().ocn_batch_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        n = ctypes.c_int()
+        check(lib().ocn_batch_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def params(self, member):
+        p = OcnParams()
+        check(lib().ocn_batch_get_params(self.h, int(member), ctypes.byref(p)))
+        return p
+
+    def tilde_h0_k(self):
+        check(lib().ocn_batch_tilde_h0_k(self.h, self.stream))
+
+    def update(self, t):
+        check(lib().ocn_batch_update(self.h, float(t), self.stream))
+
+    def update_graph(self, t):
+        check(lib().ocn_batch_update_graph(self.h, float(t), self.stream))
+
+    def fields(self, member):
+        f = OcnFields()
+        check(lib().ocn_batch_get_fields(self.h, int(member), ctypes.byref(f)))
+        return f
+
+    def h0_ptr(self, member):
+        q = ctypes.c_void_p()
+        check(lib().ocn_batch_get_h0(self.h, int(member), ctypes.byref(q)))
+        return q.value
+
+    def download_fields(self, member):
+        self.hip.sync()
+        f, N = self.fields(member), self.members[member].N
+        return {
+            "height": self.hip.download(f.height, (N, N)),
+            "dx": self.hip.download(f.dx, (N, N)),
+            "dz": self.hip.download(f.dz, (N, N)),
+            "normals": self.hip.download(f.normals, (N, N, 4)),
+            "jacobian": self.hip.download(f.jacobian, (N, N)),
+        }
+
+    def download_h0(self, member):
+        self.hip.sync()
+        N = self.members[member].N
+        return self.hip.download(self.h0_ptr(member), (N, N, 4))

@@ -98,6 +98,10 @@ typedef struct ocn_fields {
 } ocn_fields;
 
 typedef struct ocn_handle ocn_handle;
+typedef struct ocn_batch  ocn_batch;
+
+/* Largest member count of one batch. */
+#define OCN_BATCH_MAX_MEMBERS 4096
 
 /* ---------------------------------------------------------------- lifecycle */
 
@@ -222,6 +226,41 @@ int ocn_get_spectrum(const ocn_handle* h, void** h0, void** hy, void** hx, void*
  * callers without a HIP runtime of their own. */
 int ocn_copy_fields_to_host(ocn_handle* h, float* height, float* dx, float* dz, float* normals, float* jacobian,
                             void* stream);
+
+/* ---------------------------------------------------------------- batches
+ *
+ * No reference counterpart: the reference simulates one N=256 ocean
+ * (main.cpp:17).  A batch holds many oceans / cascades, each with its own
+ * ocn_params (N a power of two in [64, 2048], different N may mix; nearest-tap
+ * normals only), and computes one frame of every member in two launches: a
+ * column pass over all members' strips, then a row pass that also writes the
+ * normal map and Jacobian.  Each member's outputs equal what ocn_update_into
+ * of a single ocean with the same parameters writes, to float rounding.  For
+ * callers that animate many oceans or several cascades per ocean. */
+
+/* Allocates one device pool for every member on the current HIP device and
+ * validates each member's parameters (errors name the member). */
+int ocn_batch_create(const ocn_params* members, int count, ocn_batch** out);
+int ocn_batch_destroy(ocn_batch* b);
+int ocn_batch_size(const ocn_batch* b, int* count);
+int ocn_batch_get_params(const ocn_batch* b, int member, ocn_params* p);
+
+/* tilde_h0_k of every member (Philox noise, as ocn_tilde_h0_k). */
+int ocn_batch_tilde_h0_k(ocn_batch* b, void* stream);
+
+/* One frame at time t of every member: two kernel launches on `stream`. */
+int ocn_batch_update(ocn_batch* b, double t, void* stream);
+
+/* The same frame through a HIP graph of the two kernels, built on the first
+ * call and replayed afterwards with t set through the graph's node
+ * parameters (the graph holds kernel nodes only).  Calls on one batch must
+ * not overlap in time with each other or with ocn_batch_update. */
+int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
+
+/* Device pointers of one member's outputs (valid until ocn_batch_destroy)
+ * and its h0 spectrum (float4[N*N]). */
+int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out);
+int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0);
 
 /* ---------------------------------------------------------------- host utilities */
 

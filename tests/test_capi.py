@@ -93,6 +93,63 @@ def test_null_handles_and_pointers_rejected():
     assert L.ocn_status_string(pyocean.OCN_ESTATE) == b"OCN_ESTATE"
 
 
+def _batch_create(members):
+    arr = (pyocean.OcnParams * max(len(members), 1))(*members)
+    h = ctypes.c_void_p()
+    rc = pyocean.lib().ocn_batch_create(arr, len(members), ctypes.byref(h))
+    return rc, h.value
+
+
+@pytest.mark.parametrize("case", ["empty", "N4096", "N32", "bilinear", "bad_member"])
+def test_batch_create_rejects_bad_members_before_hip(case):
+    good = pyocean.make_params(N=256)
+    members = [good, pyocean.make_params(N=512)]
+    if case == "empty":
+        members = []
+    elif case == "N4096":
+        members[1] = pyocean.make_params(N=4096)
+    elif case == "N32":
+        members[1] = pyocean.make_params(N=32)
+    elif case == "bilinear":
+        members[1] = pyocean.make_params(N=512, normal_mode=pyocean.OCN_NORMALS_BILINEAR)
+    else:
+        members[1] = pyocean.make_params(N=512, L=-1.0)
+    rc, h = _batch_create(members)
+    assert rc == pyocean.OCN_EINVAL and h is None
+    msg = pyocean.lib().ocn_last_error().decode()
+    if case != "empty":
+        assert "member 1" in msg, msg
+
+
+def test_batch_null_pointers_rejected():
+    L = pyocean.lib()
+    assert L.ocn_batch_create(None, 1, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_tilde_h0_k(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_update(None, 0.0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_update_graph(None, 0.0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_get_fields(None, 0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_size(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_destroy(None) == pyocean.OCN_OK
+
+
+def test_cascade_params_match_the_batched_workload():
+    # 64 oceans x 3 cascades (L, N) = (2000, 256), (250, 512), (30, 1024); per-ocean draws shared by its cascades
+    ms = pyocean.cascade_params()
+    assert len(ms) == 192
+    assert [(m.L, m.N) for m in ms[:3]] == [(2000.0, 256), (250.0, 512), (30.0, 1024)]
+    for o in range(64):
+        a, b, c = ms[3 * o: 3 * o + 3]
+        assert a.wind_speed == b.wind_speed == c.wind_speed and 5.0 <= a.wind_speed < 30.0
+        assert a.choppiness == b.choppiness == c.choppiness and 0.8 <= a.choppiness < 1.5
+        assert abs(a.wind_dir_x ** 2 + a.wind_dir_y ** 2 - 1.0) < 1e-12
+        assert (a.ocean_id, b.cascade_id, c.cascade_id, a.seed, a.amplitude) == (o, 1, 2, 42, 4.0)
+        assert a.suppression == a.L / 2000.0
+    ws = np.array([m.wind_speed for m in ms[::3]])
+    assert len(set(ws.tolist())) == 64                       # independent draws per ocean
+    again = pyocean.cascade_params()
+    assert [m.wind_speed for m in again] == [m.wind_speed for m in ms]
+
+
 @pytest.mark.parametrize("N", [2, 8, 256, 1024, 4096])
 def test_bit_reversed_indices_match_oracle(N):
     assert np.array_equal(pyocean.bit_reversed_indices(N), oracle.bit_reversed_indices(N))
