@@ -25,6 +25,12 @@
  *                                                  sinusoid gives the analytic Sobel response.
  *   orc_jacobian                                 : zero displacement -> 1; a linear-in-x
  *                                                  displacement field gives the closed form.
+ *   orc_fit_eval                                 : loss from an independent numpy inverse DFT;
+ *                                                  dL/dh0 vs FP64 central differences on
+ *                                                  sampled texels (both conj modes); the
+ *                                                  adjoint identity <F x, y> = <x, F^T y>.
+ *   orc_fit_param_grad                           : dL/d(V, theta, A) vs FP64 central
+ *                                                  differences of the loss (h = 1e-6).
  */
 #include "ocean_oracle.h"
 
@@ -412,6 +418,133 @@ void orc_jacobian(int N, double L, double lambda, const double* dx, const double
             out[(size_t)y * N + x] =
                 (1.0 - lambda * dxdx) * (1.0 - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
         }
+}
+
+/* ------------------------------------------------------------------ A10 adjoint */
+
+/* No reference counterpart: the differentiable fit of the path's height
+ * output.  Loss over K frames at times t_k against targets T_k:
+ *   L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2,
+ * with h_k = orc_inversion(orc_butterfly_fft2d(orc_spectrum_t(h0, t_k).hy)),
+ * i.e. h_k(x) = (-1)^(x+y)/N^2 Re sum_t H_k(t) e^{+2 pi i t.x/N}.
+ * The adjoint of that real-part inverse DFT maps the residual
+ * r_k = 2 (h_k - T_k)/(K N^2) to G_k(t) = sum_x (-1)^(x+y) r_k(x)/N^2
+ * e^{-2 pi i t.x/N}, the forward DFT of a real field, here conj(IDFT(q)).
+ * Gradients are written as d/dRe + i d/dIm.  With phi = w(k) t_k:
+ *   dL/dh0k  = sum_k e^{-i phi} G_k
+ *   dL/dh0mk = sum_k e^{+i phi} G_k          (conj_mode 0: H = .. + h0mk e^{-i phi})
+ *            = sum_k e^{-i phi} conj(G_k)    (conj_mode 1: H = .. + conj(h0mk) e^{-i phi}). */
+int orc_fit_eval(const orc_params* p, const double* h0, int K, const double* times, const double* targets,
+                 double* loss, double* grad_h0, int nthreads)
+{
+    if (check_params(p) || !h0 || K < 1 || !times || !targets || !loss) return -1;
+    const int N = p->N;
+    const size_t nn = (size_t)N * N;
+    double* tw = (double*)malloc(sizeof(double) * 4 * (size_t)orc_log2n(N) * N);
+    double* buf = (double*)malloc(sizeof(double) * 2 * nn);
+    double* scratch = (double*)malloc(sizeof(double) * 2 * nn);
+    double* h = (double*)malloc(sizeof(double) * nn);
+    int rc = -1;
+    if (!tw || !buf || !scratch || !h) goto done;
+    orc_twiddle_factors(N, tw);
+    if (grad_h0) memset(grad_h0, 0, sizeof(double) * 4 * nn);
+    double acc = 0.0;
+    const double rscale = 2.0 / ((double)K * (double)nn) / (double)nn;
+    for (int k = 0; k < K; ++k) {
+        orc_spectrum_t(p, h0, times[k], buf, NULL, NULL);
+        orc_butterfly_fft2d(N, tw, buf, scratch, nthreads);
+        orc_inversion(N, buf, h);
+        const double* T = targets + (size_t)k * nn;
+        for (size_t i = 0; i < nn; ++i) {
+            const double d = h[i] - T[i];
+            acc += d * d;
+        }
+        if (!grad_h0) continue;
+        for (int y = 0; y < N; ++y)
+            for (int x = 0; x < N; ++x) {
+                const size_t i = (size_t)y * N + x;
+                const double perm = ((x + y) & 1) ? -1.0 : 1.0;
+                buf[2 * i]     = perm * (h[i] - T[i]) * rscale;
+                buf[2 * i + 1] = 0.0;
+            }
+        orc_butterfly_fft2d(N, tw, buf, scratch, nthreads);
+        for (int ty = 0; ty < N; ++ty)
+            for (int tx = 0; tx < N; ++tx) {
+                const size_t i = (size_t)ty * N + tx;
+                const double Gr = buf[2 * i], Gi = -buf[2 * i + 1];
+                double kx, ky, kmag;
+                wave_vector(p, tx, ty, &kx, &ky, &kmag);
+                const double ph = sqrt(p->gravity * kmag) * times[k];
+                const double c = cos(ph), s = sin(ph);
+                double* g = grad_h0 + i * 4;
+                g[0] += c * Gr + s * Gi;
+                g[1] += c * Gi - s * Gr;
+                if (p->conj_mode == 1) {
+                    g[2] += c * Gr - s * Gi;
+                    g[3] += -s * Gr - c * Gi;
+                } else {
+                    g[2] += c * Gr - s * Gi;
+                    g[3] += s * Gr + c * Gi;
+                }
+            }
+    }
+    *loss = acc / ((double)K * (double)nn);
+    rc = 0;
+done:
+    free(tw); free(buf); free(scratch); free(h);
+    return rc;
+}
+
+/* The four unit-variance normals of texel (tx, ty) that orc_h0 scales by the
+ * Phillips amplitude. */
+static void texel_xi(const orc_params* p, int tx, int ty, double xi[4])
+{
+    const int N = p->N;
+    uint32_t key[2] = {(uint32_t)p->seed, (uint32_t)(p->seed >> 32)};
+    uint32_t ctr[4] = {(uint32_t)(ty * N + tx), p->ocean_id, p->cascade_id, 0u}, r[4];
+    orc_philox4x32_10(ctr, key, r);
+    double u[4] = {orc_u01(r[0]), orc_u01(r[1]), orc_u01(r[2]), orc_u01(r[3])};
+    box_muller(u, xi);
+}
+
+/* dL/d(wind_speed V, wind angle theta, amplitude A) from dL/dh0 through
+ * h0 = s(k; V, theta, A) xi (orc_h0), wind direction (cos theta, sin theta)
+ * with theta = atan2(wind_dir_y, wind_dir_x).  With s = sqrt(P/2) unclamped:
+ *   ds/dA = s/(2A),  ds/dV = s 2 g^2/(|k|^2 V^5),  ds/dtheta = s (k_hat.w_perp)/(k_hat.w),
+ * w_perp = (-sin theta, cos theta); zero where the clamp at 4000 binds or s = 0.
+ * dL/ds = Re(conj(dL/dh0k) s xi01 / s) + Re(conj(dL/dh0mk) xi23). */
+int orc_fit_param_grad(const orc_params* p, const double* grad_h0, double grad[3])
+{
+    if (check_params(p) || !grad_h0 || !grad) return -1;
+    const int N = p->N;
+    double wx, wy;
+    wind_unit(p, &wx, &wy);
+    const double g = p->gravity, V = p->wind_speed, A = p->amplitude;
+    const double Lp = V * V / g;
+    double dV = 0.0, dT = 0.0, dA = 0.0;
+    for (int ty = 0; ty < N; ++ty)
+        for (int tx = 0; tx < N; ++tx) {
+            double kx, ky, kmag;
+            wave_vector(p, tx, ty, &kx, &ky, &kmag);
+            const double k2 = kmag * kmag;
+            const double sup = exp(-k2 * p->suppression * p->suppression);
+            const double d = (kx * wx + ky * wy) / kmag;
+            const double P = A * exp(-1.0 / (k2 * Lp * Lp)) * (d * d) * sup / (k2 * k2);
+            const double s = sqrt(P) / sqrt(2.0);
+            if (!(s > 0.0) || s > 4000.0) continue;
+            double xi[4];
+            texel_xi(p, tx, ty, xi);
+            const double* gg = grad_h0 + ((size_t)ty * N + tx) * 4;
+            const double dLds = gg[0] * xi[0] + gg[1] * xi[1] + gg[2] * xi[2] + gg[3] * xi[3];
+            const double dperp = (-kx * wy + ky * wx) / kmag;
+            dV += dLds * s * 2.0 * g * g / (k2 * V * V * V * V * V);
+            dT += dLds * s * dperp / d;
+            dA += dLds * s / (2.0 * A);
+        }
+    grad[0] = dV;
+    grad[1] = dT;
+    grad[2] = dA;
+    return 0;
 }
 
 /* ------------------------------------------------------------------ frame */

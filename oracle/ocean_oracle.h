@@ -83,6 +83,15 @@ int orc_spectrum_t(const orc_params* p, const double* h0, double t,
  * by central differences on the grid spacing L/N, periodic. */
 void orc_jacobian(int N, double L, double lambda, const double* dx, const double* dz, double* out);
 
+/* A10 extension (no reference code): the differentiable fit of the height.
+ * L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2 over K frames at times[k],
+ * targets K*N*N.  grad_h0 (N*N*4, may be NULL) receives dL/dh0 as
+ * d/dRe + i d/dIm of (h0k, h0mk).  orc_fit_param_grad chains dL/dh0 to
+ * grad = dL/d(wind_speed, wind angle, amplitude). */
+int orc_fit_eval(const orc_params* p, const double* h0, int K, const double* times, const double* targets,
+                 double* loss, double* grad_h0, int nthreads);
+int orc_fit_param_grad(const orc_params* p, const double* grad_h0, double grad[3]);
+
 /* Whole frame (main.cpp:240-244): spectrum(t), three butterfly IFFTs + inversion,
  * normal map, Jacobian.  Any output pointer may be NULL to skip it. */
 int orc_frame(const orc_params* p, const double* h0, double t,

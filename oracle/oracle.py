@@ -66,6 +66,8 @@ def lib():
         L.orc_jacobian.restype = None
         L.orc_frame.argtypes = [ctypes.POINTER(OrcParams), dp, ctypes.c_double, dp, dp, dp, dp, dp,
                                 ctypes.c_int]
+        L.orc_fit_eval.argtypes = [ctypes.POINTER(OrcParams), dp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
+        L.orc_fit_param_grad.argtypes = [ctypes.POINTER(OrcParams), dp, dp]
         _lib = L
     return _lib
 
@@ -199,3 +201,26 @@ def frame(params, h0arr, t, nthreads=1, want=("height", "dx", "dz", "normals", "
     rc = lib().orc_frame(ctypes.byref(params), _dp(a), float(t), *ptrs, int(nthreads))
     assert rc == 0
     return res
+
+
+def fit_eval(params, h0arr, times, targets, want_grad=True, nthreads=1):
+    """Loss of K height frames against targets (K, N, N) and, if asked,
+    dL/dh0 (N, N, 4) by the FP64 adjoint."""
+    N = params.N
+    a = np.ascontiguousarray(h0arr, dtype=np.float64)
+    ts = np.ascontiguousarray(times, dtype=np.float64)
+    tg = np.ascontiguousarray(targets, dtype=np.float64)
+    assert tg.shape == (len(ts), N, N)
+    loss = ctypes.c_double()
+    g = np.zeros((N, N, 4), dtype=np.float64) if want_grad else None
+    rc = lib().orc_fit_eval(ctypes.byref(params), _dp(a), len(ts), _dp(ts), _dp(tg), ctypes.byref(loss),
+                            _dp(g) if want_grad else None, int(nthreads))
+    assert rc == 0
+    return (loss.value, g) if want_grad else loss.value
+
+
+def fit_param_grad(params, grad_h0):
+    g = np.ascontiguousarray(grad_h0, dtype=np.float64)
+    out = np.zeros(3, dtype=np.float64)
+    assert lib().orc_fit_param_grad(ctypes.byref(params), _dp(g), _dp(out)) == 0
+    return out
