@@ -20,6 +20,7 @@
 
 #include "../../include/ocean_c.h"
 #include "ocean_batch.h"
+#include "ocean_fit.h"
 #include "ocean_kernels.h"
 
 namespace {
@@ -386,6 +387,64 @@ int batch_ready(const ocn_batch* b, double t)
     if (!b->h0_ready) return fail(OCN_ESTATE, "the spectra do not exist yet: call ocn_batch_tilde_h0_k first");
     hipError_t e = hipSetDevice(b->device);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "hipSetDevice");
+}
+
+}  // namespace
+
+// ====================================================================== fits
+
+struct ocn_fit {
+    int                                   device = 0;
+    int                                   K      = 0;
+    std::vector<ocn_params>               params;
+    std::vector<ocn::FitMember>           members;  // host copy of the device descriptors
+    ocn::FitMember*                       d_members = nullptr;
+    ocn::FitVars*                         d_vars    = nullptr;
+    int2*                                 d_cols    = nullptr;
+    int2*                                 d_rows    = nullptr;
+    int                                   n_cols    = 0;
+    int                                   n_rows    = 0;
+    char*                                 pool      = nullptr;
+    std::map<int, std::array<float2*, 2>> tw;  // per N: radix-8 and radix-16 Stockham twiddles
+};
+
+namespace {
+
+void fit_free(ocn_fit* f)
+{
+    void* ptrs[] = {f->d_members, f->d_vars, f->d_cols, f->d_rows, f->pool};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    for (auto& kv : f->tw)
+        for (float2* q : kv.second)
+            if (q) (void)hipFree(q);
+}
+
+int check_fit(const ocn_fit* f, int member)
+{
+    if (!f) return fail(OCN_EINVAL, "fit is NULL");
+    if (member < 0 || member >= (int)f->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, f->members.size());
+    return OCN_OK;
+}
+
+ocn::FitVars initial_vars(const ocn_params& p)
+{
+    ocn::FitVars v{};
+    v.V     = p.wind_speed;
+    v.theta = std::atan2(p.wind_dir_y, p.wind_dir_x);
+    v.A     = p.amplitude;
+    return v;
+}
+
+int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, void* stream)
+{
+    if (!f) return fail(OCN_EINVAL, "fit is NULL");
+    hipError_t e = hipSetDevice(f->device);
+    if (e == hipSuccess)
+        e = ocn::launch_fit(f->d_members, (int)f->members.size(), f->d_cols, f->n_cols, f->d_rows, f->n_rows, mode,
+                            adam, (hipStream_t)stream);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "fit launch");
 }
 
 }  // namespace
@@ -975,6 +1034,197 @@ int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0)
     if (member < 0 || member >= (int)b->members.size())
         return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
     *h0 = const_cast<float4*>(b->members[member].h0);
+    return OCN_OK;
+}
+
+// ---------------------------------------------------------------- fits
+
+int ocn_fit_create(const ocn_params* params, int count, int K, const double* times, ocn_fit** out)
+{
+    if (!out) return fail(OCN_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!params || count < 1 || count > OCN_BATCH_MAX_MEMBERS)
+        return fail(OCN_EINVAL, "params must be non-NULL and count in [1, %d]", OCN_BATCH_MAX_MEMBERS);
+    if (!times || K < 1 || K > OCN_FIT_MAX_FRAMES)
+        return fail(OCN_EINVAL, "times must be non-NULL and K in [1, %d]", OCN_FIT_MAX_FRAMES);
+    for (int k = 0; k < K; ++k)
+        if (!std::isfinite(times[k])) return fail(OCN_EINVAL, "times[%d] is not finite", k);
+    for (int i = 0; i < count; ++i) {
+        if (validate(&params[i])) return fail(OCN_EINVAL, "member %d: %s", i, std::string(g_err).c_str());
+        if (params[i].N < ocn::kFitMinN || params[i].N > ocn::kFitMaxN)
+            return fail(OCN_EINVAL, "member %d: fitted N must be in [%d, %d]", i, ocn::kFitMinN, ocn::kFitMaxN);
+    }
+    ocn_fit* f = new (std::nothrow) ocn_fit();
+    if (!f) return fail(OCN_ENOMEM, "host allocation of the fit failed");
+    f->K = K;
+    f->params.assign(params, params + count);
+    const int  P  = (K + 1) / 2;
+    int        rc = OCN_OK;
+    hipError_t e  = hipGetDevice(&f->device);
+    if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
+    // Per member, 256-byte aligned: xi 16, Z and Y 8 P each, targets 4 K, dL/dh0 16 bytes per texel, partials.
+    auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
+    auto   bytes = [&](int N) {
+        const size_t nn = (size_t)N * N;
+        return al(nn * 16) + 2 * al(nn * 8 * P) + al(nn * 4 * K) + al(nn * 16) +
+               al(sizeof(float) * ocn::fit_rows_items_per_member(N, P)) +
+               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N));
+    };
+    size_t total = 0;
+    for (const ocn_params& p : f->params) total += bytes(p.N);
+    if (!rc) rc = dalloc(&f->pool, total, "fit pool");
+    if (!rc && (e = hipMemset(f->pool, 0, total)) != hipSuccess) rc = hip_fail(e, "hipMemset(fit pool)");
+    for (const ocn_params& p : f->params) {
+        if (rc || f->tw.count(p.N)) continue;
+        std::array<float2*, 2>& t = f->tw[p.N];
+        t = {nullptr, nullptr};
+        if (!(rc = upload_table(&t[0], stockham_twiddles(p.N, 8)))) rc = upload_table(&t[1], stockham_twiddles(p.N, 16));
+    }
+    if (!rc) rc = dalloc(&f->d_vars, (size_t)count, "fit variables");
+    if (!rc) {
+        std::vector<ocn::FitVars> vars;
+        char*                     q = f->pool;
+        std::vector<int2>         cols, rows;
+        for (int i = 0; i < count; ++i) {
+            const ocn_params& p  = f->params[i];
+            const size_t      nn = (size_t)p.N * p.N;
+            ocn::FitMember    m{};
+            m.N         = p.N;
+            m.K         = K;
+            m.P         = P;
+            m.conj_mode = p.conj_mode;
+            m.kscale    = (float)(2.0 * kPi / p.L);
+            m.l2        = (float)(p.suppression * p.suppression);
+            m.g         = (float)p.gravity;
+            m.inv_knn   = (float)(1.0 / ((double)K * (double)nn));
+            m.k2_d      = (2.0 * kPi / p.L) * (2.0 * kPi / p.L);
+            m.g2_d      = p.gravity * p.gravity;
+            m.g_d       = p.gravity;
+            for (int k = 0; k < K; ++k) m.t_rev[k] = times[k] / (2.0 * kPi);
+            m.xi        = reinterpret_cast<const float4*>(q), q += al(nn * 16);
+            m.z         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
+            m.y         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
+            m.targets   = reinterpret_cast<const float*>(q), q += al(nn * 4 * K);
+            m.grad_h0   = reinterpret_cast<float4*>(q), q += al(nn * 16);
+            m.n_loss_part = ocn::fit_rows_items_per_member(p.N, P);
+            m.n_grad_part = ocn::fit_cols_items_per_member(p.N);
+            m.loss_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * m.n_loss_part);
+            m.grad_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * 3 * m.n_grad_part);
+            m.tw8       = f->tw[p.N][0];
+            m.tw16      = f->tw[p.N][1];
+            m.vars      = f->d_vars + i;
+            m.key0      = (uint32_t)p.seed;
+            m.key1      = (uint32_t)(p.seed >> 32);
+            m.ocean_id  = p.ocean_id;
+            m.cascade_id = p.cascade_id;
+            f->members.push_back(m);
+            vars.push_back(initial_vars(p));
+            for (int k = 0; k < m.n_grad_part; ++k) cols.push_back(make_int2(i, k));
+            for (int k = 0; k < m.n_loss_part; ++k) rows.push_back(make_int2(i, k));
+        }
+        f->n_cols = (int)cols.size();
+        f->n_rows = (int)rows.size();
+        if (!(rc = upload_vector(&f->d_members, f->members, "fit members")) &&
+            !(rc = upload_vector(&f->d_cols, cols, "fit column items")) &&
+            !(rc = upload_vector(&f->d_rows, rows, "fit row items"))) {
+            e = hipMemcpy(f->d_vars, vars.data(), sizeof(ocn::FitVars) * vars.size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(fit variables)");
+        }
+        for (int i = 0; !rc && i < count; ++i)
+            if ((e = ocn::launch_fit_xi(f->members[i], nullptr)) != hipSuccess) rc = hip_fail(e, "fit noise launch");
+        if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
+    }
+    if (rc) {
+        fit_free(f);
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return OCN_OK;
+}
+
+int ocn_fit_destroy(ocn_fit* f)
+{
+    if (!f) return OCN_OK;
+    (void)hipSetDevice(f->device);
+    (void)hipDeviceSynchronize();
+    fit_free(f);
+    delete f;
+    return OCN_OK;
+}
+
+int ocn_fit_targets(const ocn_fit* f, int member, float** targets)
+{
+    int rc = check_fit(f, member);
+    if (rc) return rc;
+    if (!targets) return fail(OCN_EINVAL, "targets is NULL");
+    *targets = const_cast<float*>(f->members[member].targets);
+    return OCN_OK;
+}
+
+int ocn_fit_grad_h0(const ocn_fit* f, int member, void** grad_h0)
+{
+    int rc = check_fit(f, member);
+    if (rc) return rc;
+    if (!grad_h0) return fail(OCN_EINVAL, "grad_h0 is NULL");
+    *grad_h0 = f->members[member].grad_h0;
+    return OCN_OK;
+}
+
+int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3])
+{
+    int rc = check_fit(f, member);
+    if (rc) return rc;
+    if (!vars) return fail(OCN_EINVAL, "vars is NULL");
+    if (!(std::isfinite(vars[0]) && vars[0] > 0.0 && std::isfinite(vars[1]) && std::isfinite(vars[2]) &&
+          vars[2] > 0.0))
+        return fail(OCN_EINVAL, "wind speed and amplitude must be finite and > 0, the angle finite");
+    ocn::FitVars v{};
+    v.V     = vars[0];
+    v.theta = vars[1];
+    v.A     = vars[2];
+    hipError_t e = hipSetDevice(f->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(f->members[member].vars, &v, sizeof(v), hipMemcpyHostToDevice);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpy(fit variables)");
+}
+
+int ocn_fit_forward(ocn_fit* f, void* stream) { return fit_run(f, ocn::kFitLoss, ocn::AdamCfg{}, stream); }
+
+int ocn_fit_backward(ocn_fit* f, void* stream) { return fit_run(f, ocn::kFitGrad, ocn::AdamCfg{}, stream); }
+
+int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream)
+{
+    if (!adam) return fail(OCN_EINVAL, "adam is NULL");
+    for (int i = 0; i < 3; ++i)
+        if (!(std::isfinite(adam->lr[i]) && adam->lr[i] >= 0.0)) return fail(OCN_EINVAL, "lr[%d] must be >= 0", i);
+    if (!(adam->beta1 >= 0.0 && adam->beta1 < 1.0 && adam->beta2 >= 0.0 && adam->beta2 < 1.0 && adam->eps > 0.0))
+        return fail(OCN_EINVAL, "Adam needs beta1, beta2 in [0, 1) and eps > 0");
+    ocn::AdamCfg c{};
+    for (int i = 0; i < 3; ++i) c.lr[i] = adam->lr[i];
+    c.beta1 = adam->beta1;
+    c.beta2 = adam->beta2;
+    c.eps   = adam->eps;
+    return fit_run(f, ocn::kFitAdam, c, stream);
+}
+
+int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* loss, void* stream)
+{
+    int rc = check_fit(f, member);
+    if (rc) return rc;
+    ocn::FitVars v{};
+    hipError_t   e = hipSetDevice(f->device);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e == hipSuccess) e = hipMemcpy(&v, f->members[member].vars, sizeof(v), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(fit variables)");
+    if (vars) {
+        vars[0] = v.V;
+        vars[1] = v.theta;
+        vars[2] = v.A;
+    }
+    if (grad)
+        for (int i = 0; i < 3; ++i) grad[i] = v.grad[i];
+    if (loss) *loss = v.loss;
     return OCN_OK;
 }
 

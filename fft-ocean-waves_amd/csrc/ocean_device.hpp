@@ -46,10 +46,10 @@ __device__ __forceinline__ float4 box_muller(float n0, float n1, float n2, float
     return make_float4(r0 * c0, r0 * s0, r1 * c1, r1 * s1);
 }
 
-// clamp(sqrt(P(k))/sqrt(2), -4000, 4000) (tilde_h0_k_cs.glsl:35-45, 76-88).
+// sqrt(P(k))/sqrt(2) before the clamp (tilde_h0_k_cs.glsl:35-45, 76-88).
 // k_hat = k / |k|_clamped: equal to normalize(k) for k != 0, and 0 at DC where
 // normalize(0) is undefined.  pow(d, 2.0) is d*d.
-__device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int ty)
+__device__ __forceinline__ float phillips_amp_raw(const SpecConst& sc, int tx, int ty)
 {
     const float kx = sc.kscale * (float)(tx - sc.N / 2);
     const float ky = sc.kscale * (float)(ty - sc.N / 2);
@@ -59,7 +59,13 @@ __device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int t
     const float sup = expf(-k2 * sc.l2);
     const float d   = (kx * sc.wx + ky * sc.wy) / kmag;
     const float P   = sc.A * expf(-1.0f / (k2 * sc.Lp2)) * (d * d) * sup / (k2 * k2);
-    return fminf(sqrtf(P) * 0.70710678118654752f, 4000.0f);
+    return sqrtf(P) * 0.70710678118654752f;
+}
+
+// clamp(sqrt(P(k))/sqrt(2), -4000, 4000) (tilde_h0_k_cs.glsl:87-88); P >= 0.
+__device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int ty)
+{
+    return fminf(phillips_amp_raw(sc, tx, ty), 4000.0f);
 }
 
 // ------------------------------------------------------------------ A4 time evolution

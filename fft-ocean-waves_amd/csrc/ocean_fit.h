@@ -1,0 +1,73 @@
+// ocean_fit.h -- internal interface of the differentiable-fit kernels
+// (ocean_fit.hip) for the C ABI (ocean_capi.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocn {
+
+constexpr int kFitMaxK       = 16;    // frames (times) per fit
+constexpr int kFitMinN       = 64;
+constexpr int kFitMaxN       = 1024;
+constexpr int kFitColsWidth  = 1024;  // C * N: columns per column block times N
+constexpr int kFitColsThreads = kFitColsWidth / 8;  // 2 * C * (N/16): two transforms per column
+constexpr int kFitRowsThreads = 256;  // (256 / T) rows of T = N/8 threads
+constexpr int kFitReduceThreads = 256;
+
+// Fitted variables of one member, in device memory: written by the host
+// (set) and by the reduction kernel (loss, gradient, Adam state), read by the
+// column kernels of the next fit step.
+struct FitVars {
+    double V, theta, A;    // wind speed, wind angle (direction (cos, sin)), Phillips amplitude
+    double m[3], v[3];     // Adam moments of (V, theta, A)
+    double loss;           // last loss
+    double grad[3];        // last dL/d(V, theta, A)
+    int    step;           // Adam steps taken
+    int    pad_;
+};
+
+struct AdamCfg {
+    double lr[3];
+    double beta1, beta2, eps;
+};
+
+// Device-side descriptor of one fit member (one ocean, K frames).
+struct FitMember {
+    int     N, K, P;        // P = ceil(K/2) frame pairs
+    int     conj_mode;
+    float   kscale;         // 2 pi / L
+    float   l2;             // suppression^2
+    float   g;
+    float   inv_knn;        // 1 / (K N^2)
+    double  k2_d, g2_d;     // (2 pi / L)^2, g^2
+    double  g_d;            // g
+    double  t_rev[kFitMaxK];  // t_k / (2 pi)
+    const float4* xi;       // unit-amplitude noise, N*N
+    const float2* tw8;      // Stockham twiddles (N, 8) and (N, 16)
+    const float2* tw16;
+    float2* z;              // P * N*N: column-transformed spectra (two frames per complex field)
+    float2* y;              // P * N*N: row-transformed residuals
+    const float* targets;   // K * N*N
+    float4* grad_h0;        // N*N: dL/dh0 as (dRe h0k, dIm h0k, dRe h0mk, dIm h0mk)
+    float*  loss_part;      // one partial sum per row item
+    float*  grad_part;      // three partial sums per backward column item
+    int     n_loss_part, n_grad_part;
+    FitVars* vars;
+    uint32_t key0, key1, ocean_id, cascade_id;  // Philox noise of xi
+};
+
+__host__ __device__ constexpr int fit_cols_per_block(int N) { return kFitColsWidth / N; }
+inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
+inline int fit_rows_groups_per_block(int N) { return kFitRowsThreads / (N / 8); }
+inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_per_block(N) * P; }
+
+// Modes of the fit launch sequence.
+enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2 };
+
+hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
+// cols_items: int2 (member, strip); rows_items: int2 (member, local row item)
+hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
+                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, hipStream_t s);
+
+}  // namespace ocn

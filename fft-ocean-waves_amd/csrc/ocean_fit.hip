@@ -1,0 +1,431 @@
+// ocean_fit.hip -- differentiable fit of the ocean height: loss over K frames
+// against target heights, its adjoint down to dL/dh0 and dL/d(wind speed,
+// wind angle, amplitude), and an Adam step, for one or many oceans per
+// launch.  No reference counterpart: the reference only renders the forward
+// path (tilde_h0_k_cs.glsl, tilde_h0_t_cs.glsl, butterfly_cs.glsl,
+// inversion_cs.glsl), whose maths the forward part here repeats.
+//
+// Loss L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2.  h_k is real, so two
+// frames share one complex 2-D transform (Hermitian fold: Re IDFT(F) =
+// IDFT((F + conj F(-k))/2)), and the adjoint of h -> (-1)^(x+y) Re IDFT / N^2
+// is the forward DFT of the signed residual, again two frames per complex
+// transform.  Nothing per frame is stored between steps: h0 is recomputed
+// from the unit noise xi and the current variables in both column passes.
+//
+// One fit step, P = ceil(K/2) frame pairs:
+//   k_fit_cols_fwd  item (member, strip): h0 = s(V, theta, A) xi for the strip,
+//                   per pair the folded spectra of both frames, column IFFTs,
+//                   Z_p = G_k1 + i G_k2 (columns tx and N - tx)
+//   k_fit_rows      item (member, rows x pair): row IFFT -> h_k1, h_k2,
+//                   residual against the targets, loss partial; in gradient
+//                   mode the forward row DFT of the signed residual pair -> Y_p
+//   k_fit_cols_bwd  item (member, strip): per pair the forward column DFT of
+//                   Y_p for columns tx and N - tx, split into the two frames'
+//                   spectra G_k(t), accumulation of dL/dh0 over frames; then
+//                   dL/ds per texel and the three parameter partials
+//   k_fit_reduce    one block per member: loss, gradient, optional Adam step
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "ocean_device.hpp"
+#include "ocean_fit.h"
+
+namespace ocn {
+
+namespace {
+
+constexpr int kColsLds = 2 * (kFitColsWidth + kFitColsWidth / 16);  // 2 * C * lds_len(N)
+constexpr int kRowsLds = 2176;                                       // (256 / T) * lds_len(N), T = N/8
+
+// Spectrum constants of the member at its current variables.
+__device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars& v)
+{
+    SpecConst sc{};
+    sc.N = m.N;
+    sc.kscale = m.kscale;
+    const double Lp = v.V * v.V / m.g_d;
+    sc.Lp2 = (float)(Lp * Lp);
+    sc.l2 = m.l2;
+    sc.A = (float)v.A;
+    sc.wx = (float)cos(v.theta);
+    sc.wy = (float)sin(v.theta);
+    sc.g = m.g;
+    sc.conj_mode = m.conj_mode;
+    sc.k2_d = m.k2_d;
+    sc.g2_d = m.g2_d;
+    return sc;
+}
+
+// Hermitian part (H(t) + conj H(t'))/2 of the height spectrum at one time;
+// p = h0 at t, q = h0 at its mirror t', m2 = |n|^2 (shared by t and t').
+__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, int m2, double t_rev)
+{
+    float kf, s, c;
+    sincos_turns(phase_turns(sc, m2, t_rev, kf), s, c);
+    const float2 H = evolve_amp(sc, p, s, c);
+    const float2 G = evolve_amp(sc, q, s, c);
+    return make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
+}
+
+// dL/dh0 += the pull-back of the frame spectrum gradient G through
+// H = h0k e^{i phi} + C(h0mk) e^{-i phi}: ga += e^{-i phi} G, gb += e^{+i phi} G
+// (identity C) or e^{-i phi} conj(G) (conjugation).
+__device__ __forceinline__ void pull_back(const SpecConst& sc, float2 G, int m2, double t_rev, float2& ga, float2& gb)
+{
+    float kf, s, c;
+    sincos_turns(phase_turns(sc, m2, t_rev, kf), s, c);
+    ga.x += c * G.x + s * G.y;
+    ga.y += c * G.y - s * G.x;
+    gb.x += c * G.x - s * G.y;
+    gb.y += sc.conj_mode ? -(s * G.x + c * G.y) : (s * G.x + c * G.y);
+}
+
+template <class T>
+__device__ __forceinline__ T block_sum(T x, T* red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    const int w = threadIdx.x / 64;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = x;
+    __syncthreads();
+    T t = 0;
+    for (int i = 0; i < (int)(blockDim.x / 64); ++i) t += red[i];
+    return t;
+}
+
+// ---------------------------------------------------------------- forward columns
+
+template <int N>
+__device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)
+{
+    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = 2 * C * T, LEN = lds_len(N);
+    constexpr int IT = C * N / NT;
+    static_assert(NT == kFitColsThreads && (C * N) % NT == 0 && T <= 64, "column block shape");
+    const SpecConst sc = fit_spec(m, *m.vars);
+    const size_t    nn = (size_t)N * N;
+
+    float4 hp[IT], hq[IT];
+    int    m2[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, ty = k / C;
+        const int txc = min(strip * C + c, N / 2);
+        const int mxc = (N - txc) & (N - 1), my = (N - ty) & (N - 1);
+        const float  s = phillips_amp(sc, txc, ty);  // P(-k) = P(k)
+        const float4 a = m.xi[(size_t)ty * N + txc], b = m.xi[(size_t)my * N + mxc];
+        hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
+        hq[it] = make_float4(s * b.x, s * b.y, s * b.z, s * b.w);
+        const int nx = txc - N / 2, ny = ty - N / 2;
+        m2[it] = nx * nx + ny * ny;
+    }
+    for (int p = 0; p < m.P; ++p) {
+        const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[min(2 * p + 1, m.K - 1)];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int k = threadIdx.x + it * NT;
+            const int c = k % C, ty = k / C;
+            lds[c * LEN + lds_pad(ty)]       = fold_height(sc, hp[it], hq[it], m2[it], tr1);
+            lds[(C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], m2[it], tr2);
+        }
+        __syncthreads();
+        {
+            const int g = threadIdx.x / T, j = threadIdx.x % T;
+            float2*   buf = lds + g * LEN;
+            float2    v[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+            fft_inverse<N, E>(v, j, buf, m.tw16, WaveSync{});
+            WaveSync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+        }
+        __syncthreads();
+        float2* zp = m.z + (size_t)p * nn;
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int k = threadIdx.x + it * NT;
+            const int c = k % C, y = k / C;
+            const int tx = strip * C + c;
+            if (tx <= N / 2) {
+                const int    mxc = (N - tx) & (N - 1);
+                const float2 g1  = lds[c * LEN + lds_pad(y)];
+                const float2 g2  = lds[(C + c) * LEN + lds_pad(y)];
+                zp[(size_t)y * N + tx] = make_float2(g1.x - g2.y, g1.y + g2.x);
+                if (mxc != tx) zp[(size_t)y * N + mxc] = make_float2(g1.x + g2.y, g2.x - g1.y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- rows: heights, residual, adjoint rows
+
+template <int N>
+__device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adjoint, float2* lds, float* red)
+{
+    constexpr int E = 8, T = N / E, GPB = kFitRowsThreads / T, LEN = lds_len(N);
+    static_assert(T % 2 == 0 && GPB >= 1, "row block shape");
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    const int    p = local % m.P, rb = local / m.P;
+    const int    g = threadIdx.x / T, j = threadIdx.x % T;
+    const int    y = rb * GPB + g;
+    const size_t nn = (size_t)N * N;
+    float2*      buf = lds + g * LEN;
+
+    float2 v[E];
+    const float2* src = m.z + (size_t)p * nn + (size_t)y * N;
+#pragma unroll
+    for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+    fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
+
+    const int   k1 = 2 * p, k2 = 2 * p + 1;
+    const bool  two = k2 < m.K;
+    // (-1)^(x+y) / N^2; x = j + r*T with T even
+    const float sg = ((j + y) & 1) ? -1.0f / ((float)N * (float)N) : 1.0f / ((float)N * (float)N);
+    // adjoint input (-1)^(x+y) r / N^2 with r = 2 (h - target) / (K N^2)
+    const float  qs = sg * 2.0f * m.inv_knn;
+    const float* t1 = m.targets + (size_t)k1 * nn + (size_t)y * N;
+    const float* t2 = m.targets + (size_t)(two ? k2 : k1) * nn + (size_t)y * N;
+    float lsum = 0.0f;
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int   x  = j + r * T;
+        const float d1 = sg * v[r].x - t1[x];
+        const float d2 = two ? sg * v[r].y - t2[x] : 0.0f;
+        lsum = fmaf(d1, d1, fmaf(d2, d2, lsum));
+        v[r] = make_float2(qs * d1, -(qs * d2));  // conj of q1 + i q2: the forward DFT as conj(IDFT(conj))
+    }
+    if (adjoint) {
+        fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
+        float2* dst = m.y + (size_t)p * nn + (size_t)y * N;
+#pragma unroll
+        for (int r = 0; r < E; ++r) dst[j + r * T] = make_float2(v[r].x, -v[r].y);
+    }
+    lsum = block_sum(lsum, red);
+    if (threadIdx.x == 0) m.loss_part[local] = lsum;
+}
+
+// ---------------------------------------------------------------- adjoint columns + parameter gradient
+
+template <int N>
+__device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, float2* lds, float* red)
+{
+    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = 2 * C * T, LEN = lds_len(N);
+    constexpr int IT = 2 * C * N / NT;  // texels per thread (= N / T = 16)
+    static_assert(NT == kFitColsThreads && T <= 64, "column block shape");
+    const FitVars   vars = *m.vars;
+    const SpecConst sc = fit_spec(m, vars);
+    const size_t    nn = (size_t)N * N;
+
+    // texel it of this thread: slot kk % 2C (C strip columns, then their mirrors), row kk / 2C
+    auto column = [&](int slot) {
+        const int txc = strip * C + slot % C;
+        return slot < C ? txc : (N - txc) & (N - 1);
+    };
+    float2 ga[IT], gb[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) ga[it] = gb[it] = make_float2(0.0f, 0.0f);
+
+    for (int p = 0; p < m.P; ++p) {
+        const float2* yp = m.y + (size_t)p * nn;
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int kk = threadIdx.x + it * NT;
+            const int slot = kk % (2 * C), ty = kk / (2 * C);
+            const int txc = strip * C + slot % C;
+            const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + column(slot)] : make_float2(0.0f, 0.0f);
+            lds[slot * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
+        }
+        __syncthreads();
+        {
+            const int g = threadIdx.x / T, j = threadIdx.x % T;
+            float2*   buf = lds + g * LEN;
+            float2    v[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+            fft_inverse<N, E>(v, j, buf, m.tw16, WaveSync{});
+            WaveSync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+        }
+        __syncthreads();
+        const int    k1 = 2 * p, k2 = 2 * p + 1;
+        const bool   two = k2 < m.K;
+        const double tr1 = m.t_rev[k1], tr2 = m.t_rev[two ? k2 : k1];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int kk = threadIdx.x + it * NT;
+            const int slot = kk % (2 * C), ty = kk / (2 * C);
+            const int mslot = slot < C ? slot + C : slot - C, my = (N - ty) & (N - 1);
+            // F = conj of the stored transform; G1 = (F(u) + conj F(-u))/2, G2 = (F(u) - conj F(-u))/(2i)
+            const float2 fu = lds[slot * LEN + lds_pad(ty)];
+            const float2 fm = lds[mslot * LEN + lds_pad(my)];
+            const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
+            const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
+            const int nx = column(slot) - N / 2, ny = ty - N / 2;
+            const int m2 = nx * nx + ny * ny;
+            pull_back(sc, G1, m2, tr1, ga[it], gb[it]);
+            if (two) pull_back(sc, G2, m2, tr2, ga[it], gb[it]);
+        }
+        __syncthreads();
+    }
+
+    const double V = vars.V, A = vars.A;
+    const float  dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
+    const float  da_scale = (float)(0.5 / A);
+    float dV = 0.0f, dT = 0.0f, dA = 0.0f;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int kk = threadIdx.x + it * NT;
+        const int slot = kk % (2 * C), ty = kk / (2 * C);
+        const int txc = strip * C + slot % C;
+        if (txc > N / 2 || (slot >= C && (txc == 0 || txc == N / 2))) continue;  // not this block's texel
+        const int    tx = column(slot);
+        const size_t u  = (size_t)ty * N + tx;
+        m.grad_h0[u] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
+        const float s = phillips_amp_raw(sc, tx, ty);
+        if (!(s > 0.0f) || s > 4000.0f) continue;  // no amplitude, or the clamp binds
+        const float4 xi   = m.xi[u];
+        const float  dLds = ga[it].x * xi.x + ga[it].y * xi.y + gb[it].x * xi.z + gb[it].y * xi.w;
+        const float  kx = sc.kscale * (float)(tx - N / 2), ky = sc.kscale * (float)(ty - N / 2);
+        const float  kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
+        const float  d    = (kx * sc.wx + ky * sc.wy) / kmag;
+        const float  dp   = (ky * sc.wx - kx * sc.wy) / kmag;
+        const float  w    = dLds * s;
+        dV = fmaf(w, dv_scale / (kmag * kmag), dV);
+        dT = fmaf(w, dp / d, dT);
+        dA = fmaf(w, da_scale, dA);
+    }
+    dV = block_sum(dV, red);
+    dT = block_sum(dT, red);
+    dA = block_sum(dA, red);
+    if (threadIdx.x == 0) {
+        m.grad_part[3 * strip + 0] = dV;
+        m.grad_part[3 * strip + 1] = dT;
+        m.grad_part[3 * strip + 2] = dA;
+    }
+}
+
+}  // namespace
+
+#define OCN_FIT_SWITCH(N_, CALL)                 \
+    switch (N_) {                                \
+    case 64: CALL(64); break;                    \
+    case 128: CALL(128); break;                  \
+    case 256: CALL(256); break;                  \
+    case 512: CALL(512); break;                  \
+    case 1024: CALL(1024); break;                \
+    default: break;                              \
+    }
+
+__global__ __launch_bounds__(256) void k_fit_xi(float4* __restrict__ xi, int N, uint32_t key0, uint32_t key1,
+                                                uint32_t ocean_id, uint32_t cascade_id)
+{
+    const int tx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ty = blockIdx.y;
+    if (tx >= N) return;
+    const uint32_t idx = (uint32_t)ty * (uint32_t)N + (uint32_t)tx;
+    const uint4    r   = philox4x32_10(make_uint4(idx, ocean_id, cascade_id, 0u), make_uint2(key0, key1));
+    xi[idx] = box_muller(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
+}
+
+__global__ __launch_bounds__(kFitColsThreads) void k_fit_cols_fwd(const FitMember* __restrict__ members,
+                                                                  const int2* __restrict__ items)
+{
+    __shared__ float2 lds[kColsLds];
+    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const FitMember& m  = members[it.x];
+#define OCN_CALL(n) fit_cols_fwd<n>(m, it.y, lds)
+    OCN_FIT_SWITCH(m.N, OCN_CALL)
+#undef OCN_CALL
+}
+
+__global__ __launch_bounds__(kFitRowsThreads) void k_fit_rows(const FitMember* __restrict__ members,
+                                                              const int2* __restrict__ items, int adjoint)
+{
+    __shared__ float2 lds[kRowsLds];
+    __shared__ float  red[kFitRowsThreads / 64];
+    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const FitMember& m  = members[it.x];
+#define OCN_CALL(n) fit_rows<n>(m, it.y, adjoint != 0, lds, red)
+    OCN_FIT_SWITCH(m.N, OCN_CALL)
+#undef OCN_CALL
+}
+
+__global__ __launch_bounds__(kFitColsThreads) void k_fit_cols_bwd(const FitMember* __restrict__ members,
+                                                                  const int2* __restrict__ items)
+{
+    __shared__ float2 lds[kColsLds];
+    __shared__ float  red[kFitColsThreads / 64];
+    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const FitMember& m  = members[it.x];
+#define OCN_CALL(n) fit_cols_bwd<n>(m, it.y, lds, red)
+    OCN_FIT_SWITCH(m.N, OCN_CALL)
+#undef OCN_CALL
+}
+
+// One block per member: loss and gradient from the partial sums (FP64), then
+// optionally one Adam step on (V, theta, A) (Kingma & Ba 2015, bias-corrected).
+// V and A stay positive (floors 1e-3 and 1e-6).
+__global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMember* __restrict__ members, int mode,
+                                                                  AdamCfg cfg)
+{
+    __shared__ double red[kFitReduceThreads / 64];
+    const FitMember&  m = members[blockIdx.x];
+    double l = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int i = threadIdx.x; i < m.n_loss_part; i += blockDim.x) l += (double)m.loss_part[i];
+    if (mode != kFitLoss)
+        for (int i = threadIdx.x; i < m.n_grad_part; i += blockDim.x) {
+            g0 += (double)m.grad_part[3 * i + 0];
+            g1 += (double)m.grad_part[3 * i + 1];
+            g2 += (double)m.grad_part[3 * i + 2];
+        }
+    l = block_sum(l, red);
+    g0 = block_sum(g0, red);
+    g1 = block_sum(g1, red);
+    g2 = block_sum(g2, red);
+    if (threadIdx.x != 0) return;
+    FitVars& v = *m.vars;
+    v.loss = l / ((double)m.K * (double)m.N * (double)m.N);
+    if (mode == kFitLoss) return;
+    const double g[3] = {g0, g1, g2};
+    for (int i = 0; i < 3; ++i) v.grad[i] = g[i];
+    if (mode != kFitAdam) return;
+    v.step += 1;
+    const double b1t = 1.0 - pow(cfg.beta1, (double)v.step), b2t = 1.0 - pow(cfg.beta2, (double)v.step);
+    double* x[3] = {&v.V, &v.theta, &v.A};
+    for (int i = 0; i < 3; ++i) {
+        v.m[i] = cfg.beta1 * v.m[i] + (1.0 - cfg.beta1) * g[i];
+        v.v[i] = cfg.beta2 * v.v[i] + (1.0 - cfg.beta2) * g[i] * g[i];
+        *x[i] -= cfg.lr[i] * (v.m[i] / b1t) / (sqrt(v.v[i] / b2t) + cfg.eps);
+    }
+    v.V = fmax(v.V, 1e-3);
+    v.A = fmax(v.A, 1e-6);
+}
+
+hipError_t launch_fit_xi(const FitMember& m, hipStream_t s)
+{
+    dim3 grid((m.N + 255) / 256, m.N);
+    hipLaunchKernelGGL(k_fit_xi, grid, dim3(256), 0, s, const_cast<float4*>(m.xi), m.N, m.key0, m.key1, m.ocean_id,
+                       m.cascade_id);
+    return hipGetLastError();
+}
+
+hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
+                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, hipStream_t s)
+{
+    if (n_members < 1 || n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
+    const int adjoint = mode != kFitLoss;
+    hipLaunchKernelGGL(k_fit_cols_fwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
+    hipLaunchKernelGGL(k_fit_rows, dim3(n_rows), dim3(kFitRowsThreads), 0, s, members, rows_items, adjoint);
+    if (adjoint)
+        hipLaunchKernelGGL(k_fit_cols_bwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
+    hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam);
+    return hipGetLastError();
+}
+
+}  // namespace ocn

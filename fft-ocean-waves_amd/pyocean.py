@@ -34,6 +34,8 @@ EXPORTS = [
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
     "ocn_batch_get_fields", "ocn_batch_get_h0",
+    "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
+    "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
 
@@ -61,6 +63,11 @@ class OcnParams(ctypes.Structure):
 class OcnFields(ctypes.Structure):
     _fields_ = [("height", ctypes.c_void_p), ("dx", ctypes.c_void_p), ("dz", ctypes.c_void_p),
                 ("normals", ctypes.c_void_p), ("jacobian", ctypes.c_void_p)]
+
+
+class OcnAdam(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_double * 3), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
+                ("eps", ctypes.c_double)]
 
 
 class OceanError(RuntimeError):
@@ -118,6 +125,15 @@ def lib():
             "ocn_batch_update_graph": [vp, dbl, vp],
             "ocn_batch_get_fields": [vp, i32, F],
             "ocn_batch_get_h0": [vp, i32, ctypes.POINTER(vp)],
+            "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
+            "ocn_fit_destroy": [vp],
+            "ocn_fit_targets": [vp, i32, ctypes.POINTER(vp)],
+            "ocn_fit_set_vars": [vp, i32, ctypes.POINTER(dbl)],
+            "ocn_fit_forward": [vp, vp],
+            "ocn_fit_backward": [vp, vp],
+            "ocn_fit_step": [vp, ctypes.POINTER(OcnAdam), vp],
+            "ocn_fit_get": [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl), vp],
+            "ocn_fit_grad_h0": [vp, i32, ctypes.POINTER(vp)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -506,3 +522,88 @@ class Batch:
         self.hip.sync()
         N = self.members[member].N
         return self.hip.download(self.h0_ptr(member), (N, N, 4))
+
+
+# ---------------------------------------------------------------- differentiable fits
+
+def adam(lr=(0.05, 0.01, 0.02), beta1=0.9, beta2=0.999, eps=1e-8):
+    a = OcnAdam()
+    for i in range(3):
+        a.lr[i] = float(lr[i])
+    a.beta1, a.beta2, a.eps = float(beta1), float(beta2), float(eps)
+    return a
+
+
+class Fit:
+    """Fits (wind speed, wind angle, amplitude) of one or many oceans to target
+    heights at K shared times: forward (loss) and backward (loss + gradients)
+    operators and a device-side Adam step."""
+
+    def __init__(self, members, times, stream=None):
+        arr = (OcnParams * len(members))(*members)
+        ts = np.ascontiguousarray(times, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().ocn_fit_create(arr, len(members), len(ts), ts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   ctypes.byref(h)))
+        self.h = h.value
+        self.members = list(members)
+        self.times = ts
+        self.stream = stream
+        self.hip = Hip()
+
+    def close(self):
+        if self.h:
+            check(lib().ocn_fit_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def targets_ptr(self, member):
+        q = ctypes.c_void_p()
+        check(lib().ocn_fit_targets(self.h, int(member), ctypes.byref(q)))
+        return q.value
+
+    def upload_targets(self, member, targets):
+        N = self.members[member].N
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        assert t.shape == (len(self.times), N, N)
+        self.hip.sync()
+        self.hip.upload(self.targets_ptr(member), t)
+
+    def set_vars(self, member, V, theta, A):
+        v = (ctypes.c_double * 3)(V, theta, A)
+        check(lib().ocn_fit_set_vars(self.h, int(member), v))
+
+    def forward(self):
+        check(lib().ocn_fit_forward(self.h, self.stream))
+
+    def backward(self):
+        check(lib().ocn_fit_backward(self.h, self.stream))
+
+    def step(self, adam_cfg):
+        check(lib().ocn_fit_step(self.h, ctypes.byref(adam_cfg), self.stream))
+
+    def get(self, member):
+        """(vars (V, theta, A), grad, loss) of one member; synchronises."""
+        v = (ctypes.c_double * 3)()
+        g = (ctypes.c_double * 3)()
+        l = ctypes.c_double()
+        check(lib().ocn_fit_get(self.h, int(member), v, g, ctypes.byref(l), self.stream))
+        return np.array(v[:]), np.array(g[:]), l.value
+
+    def grad_h0(self, member):
+        q = ctypes.c_void_p()
+        check(lib().ocn_fit_grad_h0(self.h, int(member), ctypes.byref(q)))
+        self.hip.sync()
+        N = self.members[member].N
+        return self.hip.download(q.value, (N, N, 4))

@@ -99,9 +99,19 @@ typedef struct ocn_fields {
 
 typedef struct ocn_handle ocn_handle;
 typedef struct ocn_batch  ocn_batch;
+typedef struct ocn_fit    ocn_fit;
 
-/* Largest member count of one batch. */
+/* Largest member count of one batch or fit. */
 #define OCN_BATCH_MAX_MEMBERS 4096
+/* Largest number of frames (times) of one fit. */
+#define OCN_FIT_MAX_FRAMES 16
+
+/* Adam settings of ocn_fit_step: per-variable learning rates for
+ * (wind speed, wind angle, amplitude), beta1, beta2, eps. */
+typedef struct ocn_adam {
+    double lr[3];
+    double beta1, beta2, eps;
+} ocn_adam;
 
 /* ---------------------------------------------------------------- lifecycle */
 
@@ -261,6 +271,48 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
  * and its h0 spectrum (float4[N*N]). */
 int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out);
 int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0);
+
+/* ---------------------------------------------------------------- differentiable fits
+ *
+ * No reference counterpart (the reference only renders the forward path).
+ * A fit holds one or many oceans (members; N a power of two in [64, 1024]),
+ * K <= OCN_FIT_MAX_FRAMES frame times shared by all members, one target height
+ * field per member and frame, and per member the fitted variables
+ * (wind speed V, wind angle theta with direction (cos theta, sin theta),
+ * Phillips amplitude A), starting from the member's ocn_params.  N, L,
+ * suppression, gravity, the Philox noise (seed, ocean_id, cascade_id) and
+ * conj_mode stay fixed.  Loss per member:
+ *   L = 1/(K N^2) sum_k sum_x (h_k(x) - target_k(x))^2,
+ * h_k the height that ocn_update writes at times[k] for the current variables.
+ * Gradients are dL/d(V, theta, A) and dL/dh0 (float4 per texel: d/dRe and
+ * d/dIm of h0k, then of h0mk).  Each call recomputes the spectrum from the
+ * noise; no per-frame field is kept between calls.  Calls on one fit must not
+ * overlap in time. */
+
+int ocn_fit_create(const ocn_params* members, int count, int K, const double* times, ocn_fit** out);
+int ocn_fit_destroy(ocn_fit* f);
+
+/* Device buffer of one member's targets: K * N * N floats, frame-major, [y][x]
+ * rows as the height texture (zero after creation). */
+int ocn_fit_targets(const ocn_fit* f, int member, float** targets);
+
+/* Sets (V, theta, A) of one member and resets its Adam state (synchronises). */
+int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3]);
+
+/* Forward operator: the loss of every member at its current variables. */
+int ocn_fit_forward(ocn_fit* f, void* stream);
+/* Backward operator: forward recomputed, then the adjoint: loss, dL/d(V, theta, A)
+ * and dL/dh0 of every member. */
+int ocn_fit_backward(ocn_fit* f, void* stream);
+/* ocn_fit_backward followed by one Adam step on every member's variables, on the
+ * device (no host round trip; a sequence of steps can be queued at once). */
+int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream);
+
+/* After the work on `stream` has finished (the call synchronises it): one
+ * member's variables, last gradient and last loss (any pointer may be NULL). */
+int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* loss, void* stream);
+/* Device pointer of one member's dL/dh0 (float4[N*N]) from the last backward. */
+int ocn_fit_grad_h0(const ocn_fit* f, int member, void** grad_h0);
 
 /* ---------------------------------------------------------------- host utilities */
 

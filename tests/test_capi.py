@@ -132,6 +132,41 @@ def test_batch_null_pointers_rejected():
     assert L.ocn_batch_destroy(None) == pyocean.OCN_OK
 
 
+@pytest.mark.parametrize("case", ["N2048", "N32", "K0", "K17", "nan_time", "bad_member", "null_times"])
+def test_fit_create_rejects_bad_arguments_before_hip(case):
+    members = [pyocean.make_params(N=256), pyocean.make_params(N=512)]
+    times = [0.5, 1.0]
+    if case == "N2048":
+        members[1] = pyocean.make_params(N=2048)
+    elif case == "N32":
+        members[1] = pyocean.make_params(N=32)
+    elif case == "K0":
+        times = []
+    elif case == "K17":
+        times = [0.1 * k for k in range(17)]
+    elif case == "nan_time":
+        times = [0.5, float("nan")]
+    elif case == "bad_member":
+        members[1] = pyocean.make_params(N=512, amplitude=float("inf"))
+    arr = (pyocean.OcnParams * 2)(*members)
+    ts = (ctypes.c_double * max(len(times), 1))(*times)
+    h = ctypes.c_void_p()
+    rc = pyocean.lib().ocn_fit_create(arr, 2, len(times), None if case == "null_times" else ts, ctypes.byref(h))
+    assert rc == pyocean.OCN_EINVAL and h.value is None
+    if case in ("N2048", "N32", "bad_member"):
+        assert "member 1" in pyocean.lib().ocn_last_error().decode()
+
+
+def test_fit_null_pointers_rejected():
+    L = pyocean.lib()
+    assert L.ocn_fit_forward(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_backward(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_step(None, None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_get(None, 0, None, None, None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_targets(None, 0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_destroy(None) == pyocean.OCN_OK
+
+
 def test_cascade_params_match_the_batched_workload():
     # 64 oceans x 3 cascades (L, N) = (2000, 256), (250, 512), (30, 1024); per-ocean draws shared by its cascades
     ms = pyocean.cascade_params()

@@ -1,0 +1,144 @@
+"""GPU parity of the differentiable fit (ocn_fit_*): loss, dL/d(V, theta, A)
+and dL/dh0 from the FP32 adjoint kernels against the FP64 oracle adjoint
+(itself pinned by FP64 central differences in test_oracle_adjoint.py), with
+relative error <= 1e-3 on the gradients; batched members against single fits;
+and the 200-step Adam fit of the N=512 workload."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import pyocean
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+GRAD_RTOL = 1e-3
+HIDDEN = (22.0, math.radians(30.0), 3.5)
+TIMES8 = [0.5 * k for k in range(1, 9)]
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hip():
+    h = pyocean.Hip()
+    assert h.device_count() >= 1, "no HIP device visible"
+    return h
+
+
+def _kw(N, L, V, th, A, conj=0, **extra):
+    return dict(N=N, L=L, wind_speed=V, wind_dir=(math.cos(th), math.sin(th)), amplitude=A, conj_mode=conj,
+                **extra)
+
+
+def _targets(N, L, times, conj=0, sigma=0.05, key=1234, **extra):
+    """Heights at the hidden variables (FP64 oracle) plus N(0, sigma^2) noise."""
+    op = oracle.make_params(**_kw(N, L, *HIDDEN, conj=conj, **extra))
+    h0 = oracle.h0(op)
+    h = np.stack([oracle.frame(op, h0, t, nthreads=NTHREADS, want=("height",))["height"] for t in times])
+    rng = np.random.Generator(np.random.Philox(key=key))
+    return (h + sigma * rng.standard_normal(h.shape)).astype(np.float32)
+
+
+def _oracle_grads(kw, times, targets):
+    op = oracle.make_params(**kw)
+    loss, gh = oracle.fit_eval(op, oracle.h0(op), times, targets.astype(np.float64), nthreads=NTHREADS)
+    return loss, oracle.fit_param_grad(op, gh), gh
+
+
+CASES = [
+    ("c4_n512_k8", 512, 1000.0, TIMES8, 0),
+    ("n512_k8_textbook", 512, 1000.0, TIMES8, 1),
+    ("n64_k3", 64, 100.0, [0.25, 1.0, 3.0], 0),
+    ("n1024_k2", 1024, 2000.0, [1.0, 2.0], 0),
+    ("n128_k1_textbook", 128, 250.0, [1.5], 1),
+]
+
+
+@pytest.mark.parametrize("name,N,L,times,conj", CASES, ids=[c[0] for c in CASES])
+def test_fit_gradients_match_oracle_adjoint(hip, name, N, L, times, conj):
+    start = (18.0, math.radians(15.0), 2.5)
+    kw = _kw(N, L, *start, conj=conj)
+    tg = _targets(N, L, times, conj=conj)
+    _say("fit %s: N=%d K=%d" % (name, N, len(times)))
+    with pyocean.Fit([pyocean.make_params(**kw)], times) as f:
+        f.upload_targets(0, tg)
+        f.forward()
+        _, _, loss_fwd = f.get(0)
+        f.backward()
+        x, g, loss = f.get(0)
+        gh = f.grad_h0(0)
+    np.testing.assert_allclose(x, start, rtol=0, atol=0)   # backward does not move the variables
+    loss_ref, g_ref, gh_ref = _oracle_grads(kw, times, tg)
+    _say("  loss %.6e ref %.6e; grad %s ref %s" % (loss, loss_ref, g, g_ref))
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    assert abs(loss_fwd - loss) <= 1e-6 * loss
+    np.testing.assert_allclose(g, g_ref, rtol=GRAD_RTOL, atol=1e-6 * np.abs(g_ref).max())
+    # dL/dh0 over every texel: relative L2 and max-abs against the largest entry
+    err = gh.astype(np.float64) - gh_ref
+    assert np.sqrt((err ** 2).sum() / (gh_ref ** 2).sum()) <= GRAD_RTOL
+    assert np.abs(err).max() <= GRAD_RTOL * np.abs(gh_ref).max()
+
+
+def test_fit_members_equal_single_fits(hip):
+    """Members of different N in one fit give exactly the single-member results."""
+    times = [0.5, 1.0, 1.5]
+    specs = [(64, 100.0, (12.0, 0.3, 2.0), 0), (256, 500.0, (25.0, -1.0, 4.0), 1), (128, 250.0, (8.0, 2.5, 3.0), 0)]
+    members, tgs = [], []
+    for i, (N, L, v, conj) in enumerate(specs):
+        members.append(pyocean.make_params(**_kw(N, L, *v, conj=conj, ocean_id=i, seed=42 + i)))
+        tgs.append(_targets(N, L, times, conj=conj, key=100 + i, ocean_id=i, seed=42 + i))
+    with pyocean.Fit(members, times) as f:
+        for i in range(len(members)):
+            f.upload_targets(i, tgs[i])
+        f.backward()
+        together = [f.get(i) + (f.grad_h0(i),) for i in range(len(members))]
+    for i, m in enumerate(members):
+        with pyocean.Fit([m], times) as f1:
+            f1.upload_targets(0, tgs[i])
+            f1.backward()
+            x, g, loss = f1.get(0)
+            gh = f1.grad_h0(0)
+        np.testing.assert_array_equal(together[i][1], g)
+        assert together[i][2] == loss
+        np.testing.assert_array_equal(together[i][3], gh)
+
+
+def test_fit_adam_recovers_hidden_variables(hip):
+    """The N=512 workload: targets from (22 m/s, 30 deg, 3.5) plus N(0, 0.05^2)
+    noise at t = 0.5..4.0 s; 200 Adam steps from (18 m/s, 15 deg, 2.5)."""
+    N, L = 512, 1000.0
+    tg = _targets(N, L, TIMES8)
+    start = (18.0, math.radians(15.0), 2.5)
+    with pyocean.Fit([pyocean.make_params(**_kw(N, L, *start))], TIMES8) as f:
+        f.upload_targets(0, tg)
+        cfg = pyocean.adam()
+        for _ in range(200):
+            f.step(cfg)
+        x, g, loss = f.get(0)
+    _say("adam: V %.4f theta %.4f deg A %.4f loss %.6e" % (x[0], math.degrees(x[1]), x[2], loss))
+    # the loss floor is the target noise variance 0.0025
+    assert loss < 0.0026
+    assert abs(x[0] - HIDDEN[0]) < 0.3
+    assert abs(math.degrees(x[1] - HIDDEN[1])) < 0.3
+    assert abs(x[2] - HIDDEN[2]) < 0.05
+
+
+def test_fit_rejects_bad_arguments(hip):
+    p = pyocean.make_params(N=2048)
+    with pytest.raises(pyocean.OceanError):
+        pyocean.Fit([p], [1.0])
+    with pytest.raises(pyocean.OceanError):
+        pyocean.Fit([pyocean.make_params(N=64)], [float(t) for t in range(17)])
+    with pyocean.Fit([pyocean.make_params(N=64)], [1.0]) as f:
+        with pytest.raises(pyocean.OceanError):
+            f.set_vars(0, -1.0, 0.0, 1.0)
+        with pytest.raises(pyocean.OceanError):
+            f.step(pyocean.adam(beta1=1.0))
