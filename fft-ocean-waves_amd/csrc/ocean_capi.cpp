@@ -1037,6 +1037,33 @@ int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0)
     return OCN_OK;
 }
 
+int ocn_batch_copy_fields_to_host(ocn_batch* b, int member, float* height, float* dx, float* dz, float* normals,
+                                  float* jacobian, void* stream)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const ocn::BatchMember& m  = b->members[member];
+    const size_t            nn = (size_t)b->params[member].N * b->params[member].N;
+    hipStream_t             s  = (hipStream_t)stream;
+    const struct {
+        float*      dst;
+        const void* src;
+        size_t      bytes;
+    } copies[] = {{height, m.height, nn * 4}, {dx, m.dx, nn * 4}, {dz, m.dz, nn * 4}, {normals, m.normals, nn * 16},
+                  {jacobian, m.jac, nn * 4}};
+    for (const auto& c : copies) {
+        if (!c.dst) continue;
+        if ((e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, s)) != hipSuccess)
+            return hip_fail(e, "hipMemcpyAsync(batch fields)");
+    }
+    e = hipStreamSynchronize(s);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
 // ---------------------------------------------------------------- fits
 
 int ocn_fit_create(const ocn_params* params, int count, int K, const double* times, ocn_fit** out)
@@ -1160,6 +1187,20 @@ int ocn_fit_targets(const ocn_fit* f, int member, float** targets)
     if (!targets) return fail(OCN_EINVAL, "targets is NULL");
     *targets = const_cast<float*>(f->members[member].targets);
     return OCN_OK;
+}
+
+int ocn_fit_upload_targets(ocn_fit* f, int member, const float* targets, void* stream)
+{
+    int rc = check_fit(f, member);
+    if (rc) return rc;
+    if (!targets) return fail(OCN_EINVAL, "targets is NULL");
+    const ocn::FitMember& m = f->members[member];
+    const size_t          bytes = sizeof(float) * (size_t)m.K * m.N * m.N;
+    hipError_t            e = hipSetDevice(f->device);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(const_cast<float*>(m.targets), targets, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpyAsync(fit targets)");
 }
 
 int ocn_fit_grad_h0(const ocn_fit* f, int member, void** grad_h0)

@@ -33,7 +33,7 @@ EXPORTS = [
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
-    "ocn_batch_get_fields", "ocn_batch_get_h0",
+    "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
@@ -125,6 +125,8 @@ def lib():
             "ocn_batch_update_graph": [vp, dbl, vp],
             "ocn_batch_get_fields": [vp, i32, F],
             "ocn_batch_get_h0": [vp, i32, ctypes.POINTER(vp)],
+            "ocn_batch_copy_fields_to_host": [vp, i32] + [vp] * 6,
+            "ocn_fit_upload_targets": [vp, i32, vp, vp],
             "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
             "ocn_fit_destroy": [vp],
             "ocn_fit_targets": [vp, i32, ctypes.POINTER(vp)],
@@ -577,8 +579,7 @@ class Fit:
         N = self.members[member].N
         t = np.ascontiguousarray(targets, dtype=np.float32)
         assert t.shape == (len(self.times), N, N)
-        self.hip.sync()
-        self.hip.upload(self.targets_ptr(member), t)
+        check(lib().ocn_fit_upload_targets(self.h, int(member), t.ctypes.data, self.stream))
 
     def set_vars(self, member, V, theta, A):
         v = (ctypes.c_double * 3)(V, theta, A)

@@ -136,4 +136,113 @@ private:
     void*       stream_;
 };
 
+// Many oceans / cascades (uneven N) advanced together: one column launch and
+// one row launch per frame (or a replayed two-node HIP graph).  No reference
+// counterpart; each member's outputs equal what Ocean::update_into writes.
+class Batch {
+public:
+    Batch(const std::vector<ocn_params>& members, void* stream = nullptr) : stream_(stream)
+    {
+        check(ocn_batch_create(members.data(), static_cast<int>(members.size()), &b_));
+    }
+    ~Batch() { ocn_batch_destroy(b_); }
+    Batch(const Batch&)            = delete;
+    Batch& operator=(const Batch&) = delete;
+
+    void       tilde_h0_k() { check(ocn_batch_tilde_h0_k(b_, stream_)); }
+    void       update(double t) { check(ocn_batch_update(b_, t, stream_)); }
+    void       update_graph(double t) { check(ocn_batch_update_graph(b_, t, stream_)); }
+    ocn_fields fields(int member) const
+    {
+        ocn_fields f;
+        check(ocn_batch_get_fields(b_, member, &f));
+        return f;
+    }
+    int size() const
+    {
+        int n = 0;
+        check(ocn_batch_size(b_, &n));
+        return n;
+    }
+    // Synchronises the stream and copies one member's outputs to the host.
+    HostFrame download(int member) const
+    {
+        ocn_params p;
+        check(ocn_batch_get_params(b_, member, &p));
+        const size_t nn = static_cast<size_t>(p.N) * p.N;
+        HostFrame f{std::vector<float>(nn), std::vector<float>(nn), std::vector<float>(nn),
+                    std::vector<float>(nn * 4), std::vector<float>(nn)};
+        check(ocn_batch_copy_fields_to_host(b_, member, f.height.data(), f.dx.data(), f.dz.data(), f.normals.data(),
+                                            f.jacobian.data(), stream_));
+        return f;
+    }
+
+private:
+    ocn_batch* b_ = nullptr;
+    void*      stream_;
+};
+
+// The differentiable fit of (wind speed, wind angle, amplitude) to target
+// heights: a forward operator (loss) and a backward operator (loss and
+// gradients, the spectrum recomputed), plus a device-side Adam step.  No
+// reference counterpart.
+struct FitResult {
+    double vars[3];  // wind speed, wind angle (rad), amplitude
+    double grad[3];  // dL/d(vars)
+    double loss;
+};
+
+class Fit {
+public:
+    Fit(const std::vector<ocn_params>& members, const std::vector<double>& times, void* stream = nullptr)
+        : stream_(stream)
+    {
+        check(ocn_fit_create(members.data(), static_cast<int>(members.size()), static_cast<int>(times.size()),
+                             times.data(), &f_));
+        for (const ocn_params& p : members) target_sizes_.push_back(times.size() * p.N * p.N);
+    }
+    ~Fit() { ocn_fit_destroy(f_); }
+    Fit(const Fit&)            = delete;
+    Fit& operator=(const Fit&) = delete;
+
+    float* targets(int member) const
+    {
+        float* t = nullptr;
+        check(ocn_fit_targets(f_, member, &t));
+        return t;
+    }
+    // K frames of N x N target heights, frame-major.
+    void upload_targets(int member, const std::vector<float>& host)
+    {
+        if (member < 0 || member >= static_cast<int>(target_sizes_.size()) || host.size() != target_sizes_[member])
+            throw Error(OCN_EINVAL, "targets must hold K * N * N floats of an existing member");
+        check(ocn_fit_upload_targets(f_, member, host.data(), stream_));
+    }
+    void forward() { check(ocn_fit_forward(f_, stream_)); }
+    void backward() { check(ocn_fit_backward(f_, stream_)); }
+    void step(const ocn_adam& a) { check(ocn_fit_step(f_, &a, stream_)); }
+    void set_vars(int member, double V, double theta, double A)
+    {
+        const double v[3] = {V, theta, A};
+        check(ocn_fit_set_vars(f_, member, v));
+    }
+    FitResult result(int member) const
+    {
+        FitResult r{};
+        check(ocn_fit_get(f_, member, r.vars, r.grad, &r.loss, stream_));
+        return r;
+    }
+    void* grad_h0(int member) const
+    {
+        void* g = nullptr;
+        check(ocn_fit_grad_h0(f_, member, &g));
+        return g;
+    }
+
+private:
+    ocn_fit*            f_ = nullptr;
+    void*               stream_;
+    std::vector<size_t> target_sizes_;
+};
+
 }  // namespace ocean

@@ -271,6 +271,10 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
  * and its h0 spectrum (float4[N*N]). */
 int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out);
 int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0);
+/* Host copies of one member's outputs after the work on `stream` (synchronises
+ * it); any pointer may be NULL; sizes as ocn_copy_fields_to_host. */
+int ocn_batch_copy_fields_to_host(ocn_batch* b, int member, float* height, float* dx, float* dz, float* normals,
+                                  float* jacobian, void* stream);
 
 /* ---------------------------------------------------------------- differentiable fits
  *
@@ -295,6 +299,10 @@ int ocn_fit_destroy(ocn_fit* f);
 /* Device buffer of one member's targets: K * N * N floats, frame-major, [y][x]
  * rows as the height texture (zero after creation). */
 int ocn_fit_targets(const ocn_fit* f, int member, float** targets);
+
+/* Copies K * N * N host floats into one member's targets, ordered on `stream`
+ * (synchronises it). */
+int ocn_fit_upload_targets(ocn_fit* f, int member, const float* targets, void* stream);
 
 /* Sets (V, theta, A) of one member and resets its Adam state (synchronises). */
 int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3]);

@@ -3,6 +3,9 @@
 // passes (diharaw/fft-ocean-waves src/main.cpp:199-255).
 //
 //   ocean_api_example --validate        argument checks only (no GPU needed)
+//   ocean_api_example --batch-fit       a two-member batch against single
+//       oceans, then a noiseless fit of (wind speed, angle, amplitude) to
+//       heights of known variables; prints both results, exit 0 on success.
 //   ocean_api_example OUT.bin N L V wx wy A l lambda seed t
 //       one frame, fused and pass by pass; writes the fused frame's height,
 //       dx, dz, normals, jacobian (float32, in that order) to OUT.bin and
@@ -44,9 +47,77 @@ static int validate_only()
     return failures ? 1 : 0;
 }
 
+static int batch_and_fit()
+{
+    ocn_params a = ocean::default_params();
+    a.N = 64;
+    a.L = 100.0;
+    a.wind_speed = 12.0;
+    ocn_params b = a;
+    b.N = 128;
+    b.L = 400.0;
+    b.wind_speed = 20.0;
+    b.cascade_id = 1;
+    // batch of two cascades against the single-ocean path
+    ocean::Batch batch({a, b});
+    batch.tilde_h0_k();
+    batch.update(1.25);
+    double worst = 0.0;
+    for (int i = 0; i < batch.size(); ++i) {
+        ocean::Ocean one(i == 0 ? a : b);
+        one.init();
+        one.update(1.25);
+        const ocean::HostFrame x = one.download(), y = batch.download(i);
+        double d = 0.0, hmax = 0.0;
+        for (size_t k = 0; k < x.height.size(); ++k) {
+            d    = std::fmax(d, std::fabs(x.height[k] - y.height[k]));
+            hmax = std::fmax(hmax, std::fabs(x.height[k]));
+        }
+        worst = std::fmax(worst, d / hmax);
+        std::printf("batch member %d N=%d: max|height| %.4e, max diff vs single %.3e\n", i, i == 0 ? a.N : b.N, hmax,
+                    d);
+    }
+    // noiseless fit: targets are heights of member a at two times
+    const std::vector<double> times = {0.5, 1.5};
+    std::vector<float>        targets;
+    {
+        ocean::Ocean hidden(a);
+        hidden.init();
+        for (double t : times) {
+            hidden.update(t);
+            const ocean::HostFrame f = hidden.download();
+            targets.insert(targets.end(), f.height.begin(), f.height.end());
+        }
+    }
+    ocn_params start = a;
+    start.wind_speed = 10.0;
+    start.amplitude  = 0.8 * a.amplitude;
+    ocean::Fit fit({start}, times);
+    fit.upload_targets(0, targets);
+    fit.forward();
+    const double loss0 = fit.result(0).loss;
+    ocn_adam adam{{0.1, 0.01, 0.05 * a.amplitude}, 0.9, 0.999, 1e-8};
+    for (int s = 0; s < 300; ++s) fit.step(adam);
+    const ocean::FitResult r = fit.result(0);
+    std::printf("fit: loss %.3e -> %.3e, V %.4f (true %.4f), theta %.5f (true %.5f), A %.5g (true %.5g)\n", loss0,
+                r.loss, r.vars[0], a.wind_speed, r.vars[1], std::atan2(a.wind_dir_y, a.wind_dir_x), r.vars[2],
+                a.amplitude);
+    const bool ok = worst <= 1e-5 && r.loss < 1e-3 * loss0 && std::fabs(r.vars[0] - a.wind_speed) < 0.2 &&
+                    std::fabs(r.vars[2] / a.amplitude - 1.0) < 0.02;
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
     if (argc == 2 && std::strcmp(argv[1], "--validate") == 0) return validate_only();
+    if (argc == 2 && std::strcmp(argv[1], "--batch-fit") == 0) {
+        try {
+            return batch_and_fit();
+        } catch (const ocean::Error& e) {
+            std::fprintf(stderr, "ocean error: %s\n", e.what());
+            return 1;
+        }
+    }
     if (argc != 12) {
         std::fprintf(stderr, "usage: %s OUT.bin N L V wx wy A l lambda seed t | --validate\n", argv[0]);
         return 2;

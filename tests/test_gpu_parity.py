@@ -338,3 +338,14 @@ def test_cpp_host_api_frame_matches_oracle(hip, tmp_path):
     ref = oracle.frame(op, oracle.h0(op), 1.0, nthreads=NTHREADS)
     for fld in got:
         assert_field_close(fld, got[fld], ref[fld])
+
+
+def test_cpp_host_api_batch_and_fit(hip):
+    """ocean::Batch against ocean::Ocean and a noiseless ocean::Fit through the
+    C++ host API (tests/cpp/ocean_api_example --batch-fit, a child process)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "ocean_api_example")
+    assert os.path.exists(exe), "build with __graft_entry__.build()"
+    r = subprocess.run([exe, "--batch-fit"], capture_output=True, text=True, timeout=120)
+    _say(r.stdout.strip())
+    assert r.returncode == 0, r.stdout + r.stderr
