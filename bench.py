@@ -11,12 +11,14 @@ Prints one JSON line (metric, value in frames/s, roofline of the dominant
 kernel measured with HIP events, CPU baseline of the FP64 oracle).
 
 Beside it, the line carries the batched-cascade comparison (64 oceans x 3
-cascades in two launches per frame against per-ocean launches).
+cascades in two launches per frame against per-ocean launches) and the
+differentiable fit (forward and forward+backward cost, a 200-step Adam fit).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S] [--cascade-frames F]
   python bench.py --mode copy      (device-to-device copy used to calibrate
                                     rocprofv3 FETCH_SIZE / WRITE_SIZE)
   python bench.py --mode cascades  (the batched-cascade comparison alone)
+  python bench.py --mode fit       (the differentiable fit alone)
 """
 import argparse
 import json
@@ -154,6 +156,67 @@ def cascades_bench(hip, frames, warmup):
     return out
 
 
+def fit_bench(hip, steps=200):
+    """The differentiable fit.  N=512, L=1000 m, l=0.5: targets are the heights
+    at (22 m/s, 30 deg, A=3.5) at t = 0.5..4.0 s (K=8) plus N(0, 0.05^2) noise;
+    the forward (loss) and forward+backward (loss, gradients, spectrum
+    recomputed) operators are timed with HIP events over graph replays of 20
+    calls each (host launch cost excluded), then `steps` Adam steps run from
+    (18 m/s, 15 deg, 2.5)."""
+    import pyocean
+    N, L = 512, 1000.0
+    times = [0.5 * k for k in range(1, 9)]
+    hidden = (22.0, math.radians(30.0), 3.5)
+    start = (18.0, math.radians(15.0), 2.5)
+
+    def params(v, **kw):
+        return pyocean.make_params(N=N, L=L, wind_speed=v[0], wind_dir=(math.cos(v[1]), math.sin(v[1])),
+                                   amplitude=v[2], **kw)
+
+    stream = hip.stream()
+    tg = np.empty((len(times), N, N), dtype=np.float32)
+    with pyocean.Ocean(params(hidden), stream=stream) as oc:
+        oc.tilde_h0_k()
+        for k, t in enumerate(times):
+            oc.update(t)
+            tg[k] = oc.download_fields()["height"]
+    rng = np.random.Generator(np.random.Philox(key=1234))
+    tg += (0.05 * rng.standard_normal(tg.shape)).astype(np.float32)
+    e0, e1 = hip.event(), hip.event()
+    out = {"workload": "N=512, L=1000 m, K=8 frames t=0.5..4.0 s, hidden (22 m/s, 30 deg, A=3.5), "
+                       "target noise N(0, 0.05^2), start (18 m/s, 15 deg, 2.5)"}
+    with pyocean.Fit([params(start)], times, stream=stream) as f:
+        f.upload_targets(0, tg)
+        for name, op in (("forward", f.forward), ("forward_backward", f.backward)):
+            g, ex = hip.capture(stream, lambda: [op() for _ in range(20)])
+            try:
+                hip.graph_launch(ex, stream)
+                hip.record(e0, stream)
+                for _ in range(5):
+                    hip.graph_launch(ex, stream)
+                hip.record(e1, stream)
+                out[name + "_us"] = round(hip.elapsed_ms(e0, e1) * 1e3 / 100, 2)
+            finally:
+                hip.stream_sync(stream)
+                hip.graph_destroy(g, ex)
+        out["ratio_fwd_bwd_to_fwd"] = round(out["forward_backward_us"] / out["forward_us"], 3)
+        cfg = pyocean.adam()
+        hip.record(e0, stream)
+        for _ in range(steps):
+            f.step(cfg)
+        hip.record(e1, stream)
+        out["adam_steps"] = steps
+        out["adam_steps_per_s"] = round(steps / (hip.elapsed_ms(e0, e1) * 1e-3), 1)
+        x, _, loss = f.get(0)
+    out["final"] = {"wind_speed": round(x[0], 4), "wind_angle_deg": round(math.degrees(x[1]), 4),
+                    "amplitude": round(x[2], 4), "loss": loss}
+    out["abs_error"] = {"wind_speed": round(abs(x[0] - hidden[0]), 4),
+                        "wind_angle_deg": round(abs(math.degrees(x[1] - hidden[1])), 4),
+                        "amplitude": round(abs(x[2] - hidden[2]), 4)}
+    hip.stream_destroy(stream)
+    return out
+
+
 def run_copy(hip):
     """Device-to-device copies of 256 MiB (rocprofv3 counter calibration)."""
     nbytes = 256 << 20
@@ -173,9 +236,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["frames", "copy", "cascades"], default="frames")
+    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit"], default="frames")
     ap.add_argument("--cascade-frames", type=int, default=60,
                     help="frames of the batched-cascade comparison (0 skips it)")
+    ap.add_argument("--fit", type=int, default=1, help="1: time the differentiable fit too (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the ocean path runs on a single MI355X); "
@@ -189,6 +253,9 @@ def main():
         sys.exit(1)
     if args.mode == "copy":
         run_copy(hip)
+        return
+    if args.mode == "fit":
+        print(json.dumps({"mode": "fit", **fit_bench(hip)}), flush=True)
         return
     if args.mode == "cascades":
         print(json.dumps({"mode": "cascades", **cascades_bench(hip, max(1, args.cascade_frames), 10)}), flush=True)
@@ -266,6 +333,7 @@ def main():
     hip.stream_destroy(stream)
 
     casc = cascades_bench(hip, args.cascade_frames, 10) if args.cascade_frames > 0 else None
+    fit = fit_bench(hip) if args.fit else None
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
     out = {
         "metric": "ocean-frames/s + HBM GB/s (% peak) at N=1024; batched & fwd+bwd vs plain on GPU",
@@ -306,6 +374,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched_cascades": casc,
+        "fit": fit,
     }
     print(json.dumps(out), flush=True)
 

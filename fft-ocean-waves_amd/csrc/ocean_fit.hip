@@ -115,10 +115,13 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         const int c = k % C, ty = k / C;
         const int txc = min(strip * C + c, N / 2);
         const int mxc = (N - txc) & (N - 1), my = (N - ty) & (N - 1);
-        const float  s = phillips_amp(sc, txc, ty);  // P(-k) = P(k)
+        // The mirror's amplitude is evaluated on its own: on the Nyquist row or
+        // column its centred index is (-nx, ny) or (nx, -ny), not -n, so the
+        // wind factor d^2 differs from the texel's.
+        const float  s = phillips_amp(sc, txc, ty), sm = phillips_amp(sc, mxc, my);
         const float4 a = m.xi[(size_t)ty * N + txc], b = m.xi[(size_t)my * N + mxc];
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
-        hq[it] = make_float4(s * b.x, s * b.y, s * b.z, s * b.w);
+        hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         const int nx = txc - N / 2, ny = ty - N / 2;
         m2[it] = nx * nx + ny * ny;
     }

@@ -528,7 +528,9 @@ class Batch:
 
 # ---------------------------------------------------------------- differentiable fits
 
-def adam(lr=(0.05, 0.01, 0.02), beta1=0.9, beta2=0.999, eps=1e-8):
+def adam(lr=(0.2, 0.02, 0.05), beta1=0.9, beta2=0.999, eps=1e-8):
+    """Adam settings; the default rates (m/s, rad, amplitude per step) fit the
+    N=512 workload in 200 steps."""
     a = OcnAdam()
     for i in range(3):
         a.lr[i] = float(lr[i])
