@@ -19,6 +19,7 @@ differentiable fit (forward and forward+backward cost, a 200-step Adam fit).
                                     rocprofv3 FETCH_SIZE / WRITE_SIZE)
   python bench.py --mode cascades  (the batched-cascade comparison alone)
   python bench.py --mode fit       (the differentiable fit alone)
+  python bench.py --mode batched-fit (256 oceans fitted together)
 """
 import argparse
 import json
@@ -217,6 +218,69 @@ def fit_bench(hip, steps=200):
     return out
 
 
+def batched_fit_bench(hip, steps=200, n_oceans=256):
+    """The batched fit: 256 independent oceans at N=256, L=500 m, each with
+    hidden (V ~ U(5,30) m/s, angle ~ U(0,2pi), A ~ U(2,5)), targets = its heights
+    at t = 0.5, 1.0, 1.5, 2.0 s plus N(0, 0.05^2) noise, all starting from
+    (17.5 m/s, 0 rad, 3.5); `steps` Adam steps of all oceans, each step one
+    column, one row, one adjoint-column and one reduction launch for all
+    members.  Reports steps/s and the final parameter errors (the angle modulo
+    pi: the Phillips spectrum depends on (k.w)^2, so w and -w are the same ocean)."""
+    import pyocean
+    times = [0.5, 1.0, 1.5, 2.0]
+    hidden, starts = pyocean.fit_batch_params(n_oceans)
+    N = starts[0].N
+    nn = N * N
+    stream = hip.stream()
+    rng = np.random.Generator(np.random.Philox(key=777))
+    with pyocean.Fit(starts, times, stream=stream) as f:
+        with pyocean.Batch(hidden, stream=stream) as b:
+            b.tilde_h0_k()
+            for k, t in enumerate(times):
+                b.update(t)
+                hip.stream_sync(stream)
+                for o in range(n_oceans):
+                    hip.ok(hip.rt.hipMemcpy(f.targets_ptr(o) + k * nn * 4, b.fields(o).height, nn * 4, hip.D2D),
+                           "hipMemcpy D2D")
+        for o in range(n_oceans):
+            tg = hip.download(f.targets_ptr(o), (len(times), N, N))
+            f.upload_targets(o, tg + (0.05 * rng.standard_normal(tg.shape)).astype(np.float32))
+        cfg = pyocean.adam()
+        f.step(cfg)
+        for o in range(n_oceans):
+            f.set_vars(o, starts[o].wind_speed, math.atan2(starts[o].wind_dir_y, starts[o].wind_dir_x),
+                       starts[o].amplitude)
+        e0, e1 = hip.event(), hip.event()
+        hip.record(e0, stream)
+        for _ in range(steps):
+            f.step(cfg)
+        hip.record(e1, stream)
+        ms = hip.elapsed_ms(e0, e1)
+        res = [f.get(o) for o in range(n_oceans)]
+    hip.stream_destroy(stream)
+    dv = np.array([abs(r[0][0] - h.wind_speed) for r, h in zip(res, hidden)])
+    dth = np.array([abs((r[0][1] - math.atan2(h.wind_dir_y, h.wind_dir_x) + math.pi / 2) % math.pi - math.pi / 2)
+                    for r, h in zip(res, hidden)])
+    da = np.array([abs(r[0][2] / h.amplitude - 1.0) for r, h in zip(res, hidden)])
+    loss = np.array([r[2] for r in res])
+    return {
+        "workload": "%d oceans, N=%d, L=500 m, K=4 (t=0.5..2.0 s), hidden V~U(5,30), angle~U(0,2pi), A~U(2,5), "
+                    "target noise N(0, 0.05^2), start (17.5 m/s, 0 rad, 3.5)" % (n_oceans, N),
+        "steps": steps,
+        "ms_per_step": round(ms / steps, 4),
+        "fit_steps_per_s": round(steps / (ms * 1e-3), 1),
+        "ocean_steps_per_s": round(n_oceans * steps / (ms * 1e-3), 1),
+        "final_abs_error_wind_speed": {"median": round(float(np.median(dv)), 4), "p90": round(float(np.percentile(dv, 90)), 4),
+                                       "max": round(float(dv.max()), 4)},
+        "final_abs_error_angle_deg_mod_180": {"median": round(math.degrees(float(np.median(dth))), 4),
+                                              "p90": round(math.degrees(float(np.percentile(dth, 90))), 4),
+                                              "max": round(math.degrees(float(dth.max())), 4)},
+        "final_rel_error_amplitude": {"median": round(float(np.median(da)), 4), "p90": round(float(np.percentile(da, 90)), 4),
+                                      "max": round(float(da.max()), 4)},
+        "final_loss": {"median": float(np.median(loss)), "max": float(loss.max())},
+    }
+
+
 def run_copy(hip):
     """Device-to-device copies of 256 MiB (rocprofv3 counter calibration)."""
     nbytes = 256 << 20
@@ -236,7 +300,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit"], default="frames")
+    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit", "batched-fit"], default="frames")
     ap.add_argument("--cascade-frames", type=int, default=60,
                     help="frames of the batched-cascade comparison (0 skips it)")
     ap.add_argument("--fit", type=int, default=1, help="1: time the differentiable fit too (0 skips it)")
@@ -256,6 +320,9 @@ def main():
         return
     if args.mode == "fit":
         print(json.dumps({"mode": "fit", **fit_bench(hip)}), flush=True)
+        return
+    if args.mode == "batched-fit":
+        print(json.dumps({"mode": "batched-fit", **batched_fit_bench(hip)}), flush=True)
         return
     if args.mode == "cascades":
         print(json.dumps({"mode": "cascades", **cascades_bench(hip, max(1, args.cascade_frames), 10)}), flush=True)
@@ -334,6 +401,7 @@ def main():
 
     casc = cascades_bench(hip, args.cascade_frames, 10) if args.cascade_frames > 0 else None
     fit = fit_bench(hip) if args.fit else None
+    bfit = batched_fit_bench(hip) if args.fit else None
     cpu = cpu_baseline(args.cpu_seconds) if args.cpu_seconds > 0 else None
     out = {
         "metric": "ocean-frames/s + HBM GB/s (% peak) at N=1024; batched & fwd+bwd vs plain on GPU",
@@ -375,6 +443,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_cascades": casc,
         "fit": fit,
+        "batched_fit": bfit,
     }
     print(json.dumps(out), flush=True)
 

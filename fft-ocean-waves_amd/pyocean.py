@@ -538,6 +538,23 @@ def adam(lr=(0.2, 0.02, 0.05), beta1=0.9, beta2=0.999, eps=1e-8):
     return a
 
 
+def fit_batch_params(n_oceans=256, N=256, L=500.0, seed=42, start=(17.5, 0.0, 3.5)):
+    """Hidden and starting members of a batched fit of independent oceans.
+    Ocean o draws its hidden wind speed ~ U(5, 30) m/s, angle ~ U(0, 2 pi) and
+    amplitude ~ U(2, 5) from numpy's Philox generator keyed by seed + o; its
+    noise seed is seed + o with ocean_id o; suppression L/2000.  Every ocean
+    starts from `start` (speed, angle, amplitude).  Returns (hidden, starts)."""
+    hidden, starts = [], []
+    for o in range(n_oceans):
+        rng = np.random.Generator(np.random.Philox(key=seed + o))
+        v, th, a = float(rng.uniform(5.0, 30.0)), float(rng.uniform(0.0, 2.0 * np.pi)), float(rng.uniform(2.0, 5.0))
+        common = dict(N=N, L=L, seed=seed + o, ocean_id=o)
+        hidden.append(make_params(wind_speed=v, wind_dir=(np.cos(th), np.sin(th)), amplitude=a, **common))
+        starts.append(make_params(wind_speed=start[0], wind_dir=(np.cos(start[1]), np.sin(start[1])),
+                                  amplitude=start[2], **common))
+    return hidden, starts
+
+
 class Fit:
     """Fits (wind speed, wind angle, amplitude) of one or many oceans to target
     heights at K shared times: forward (loss) and backward (loss + gradients)
