@@ -101,9 +101,9 @@ __device__ __forceinline__ T block_sum(T x, T* red)
 template <int N>
 __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)
 {
-    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = 2 * C * T, LEN = lds_len(N);
+    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
     constexpr int IT = C * N / NT;
-    static_assert(NT == kFitColsThreads && (C * N) % NT == 0 && T <= 64, "column block shape");
+    static_assert(2 * C * T == kFitFftThreads && (C * N) % NT == 0 && T <= 64, "column block shape");
     const SpecConst sc = fit_spec(m, *m.vars);
     const size_t    nn = (size_t)N * N;
 
@@ -135,7 +135,7 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
             lds[(C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], m2[it], tr2);
         }
         __syncthreads();
-        {
+        if (threadIdx.x < kFitFftThreads) {  // whole waves: 2C transforms of T threads
             const int g = threadIdx.x / T, j = threadIdx.x % T;
             float2*   buf = lds + g * LEN;
             float2    v[E];
@@ -217,9 +217,9 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
 template <int N>
 __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, float2* lds, float* red)
 {
-    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = 2 * C * T, LEN = lds_len(N);
-    constexpr int IT = 2 * C * N / NT;  // texels per thread (= N / T = 16)
-    static_assert(NT == kFitColsThreads && T <= 64, "column block shape");
+    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
+    constexpr int IT = 2 * C * N / NT;  // texels per thread (= N / (2T) = 8)
+    static_assert(2 * C * T == kFitFftThreads && (2 * C * N) % NT == 0 && T <= 64, "column block shape");
     const FitVars   vars = *m.vars;
     const SpecConst sc = fit_spec(m, vars);
     const size_t    nn = (size_t)N * N;
@@ -244,7 +244,7 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
             lds[slot * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
         }
         __syncthreads();
-        {
+        if (threadIdx.x < kFitFftThreads) {  // whole waves: 2C transforms of T threads
             const int g = threadIdx.x / T, j = threadIdx.x % T;
             float2*   buf = lds + g * LEN;
             float2    v[E];
