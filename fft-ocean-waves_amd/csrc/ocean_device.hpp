@@ -76,16 +76,26 @@ __device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int t
 // an FP32 estimate w0 plus one Newton step on w^4 = g^2 |k|^2 (|k|^2 is exact
 // from the integer wave numbers), then the product with t / (2 pi) is reduced
 // to one turn before the FP32 sincos.  m = nx^2 + ny^2.
-__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
+__device__ __forceinline__ double omega_fp64(const SpecConst& sc, int m, float& kf)
 {
     kf = fmaxf(sc.kscale * __builtin_amdgcn_sqrtf((float)m), 0.00001f);
     const float  w0 = __builtin_amdgcn_sqrtf(sc.g * kf);
     const double Q  = fmax(sc.k2_d * (double)m, 1e-10) * sc.g2_d;  // g^2 |k|_clamped^2
     const double w  = (double)w0;
     const double w2 = w * w;
-    const double w1 = w - (w2 * w2 - Q) * (double)(0.25f * __builtin_amdgcn_rcpf(w0 * w0 * w0));
-    const double rev = w1 * t_rev;
+    return w - (w2 * w2 - Q) * (double)(0.25f * __builtin_amdgcn_rcpf(w0 * w0 * w0));
+}
+
+// w t / (2 pi) reduced to [-1/2, 1/2] turns, w from omega_fp64.
+__device__ __forceinline__ float turns(double w, double t_rev)
+{
+    const double rev = w * t_rev;
     return (float)(rev - rint(rev));
+}
+
+__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
+{
+    return turns(omega_fp64(sc, m, kf), t_rev);
 }
 
 // sin and cos of 2 pi fr for fr in [-1/2, 1/2]: reduction by quarter turns
@@ -135,6 +145,20 @@ __device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 
     // -i m H = (m H.y, -m H.x);  conj(-i g G) = (g G.y, g G.x)
     ax = make_float2(0.5f * (mx * H.y + gx * G.y), 0.5f * (gx * G.x - mx * H.x));
     az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
+}
+
+// ------------------------------------------------------------------ intermediate column order
+
+// Position of column x in a row of the column-pass intermediates (Z1, Z2):
+// columns 0..N/2 in place, columns N/2+1..N-1 reversed after them (column
+// N - u at N/2 + u).  A strip of half-plane columns tx and its mirror columns
+// N - tx then occupy two aligned runs of equal length, so neighbouring strips
+// complete whole cache lines instead of splitting the mirror run across lines
+// written by different strips.
+template <int N>
+__device__ __forceinline__ int zpos(int x)
+{
+    return x <= N / 2 ? x : 3 * N / 2 - x;
 }
 
 // ------------------------------------------------------------------ XCD-aware block order
@@ -216,12 +240,12 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
                 const float2 gx  = lds[(1 * C + c) * LEN + lds_pad(y)];
                 const size_t row = (size_t)y * N;
                 z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
-                if (mxc != tx) z1[row + mxc] = make_float2(gy.x + gx.y, gx.x - gy.y);
+                if (mxc != tx) z1[row + zpos<N>(mxc)] = make_float2(gy.x + gx.y, gx.x - gy.y);
                 if (y < N / 2) {  // Z2 row y pairs G_dz rows 2y and 2y+1
                     const float2 lo = lds[(2 * C + c) * LEN + lds_pad(2 * y)];
                     const float2 hi = lds[(2 * C + c) * LEN + lds_pad(2 * y + 1)];
                     z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
-                    if (mxc != tx) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
+                    if (mxc != tx) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
                 }
             }
         }
@@ -257,7 +281,7 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
     // inverse transform of one Z row into v (natural order, thread j holds x = j + r*T)
     auto fft_row = [&](const float2* src, float2 (&v)[E]) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+        for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
         fft_inverse<N, E>(v, j, buf, tw, Sync{});
     };
     // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y

@@ -11,8 +11,7 @@ constexpr int kFitMaxK       = 16;    // frames (times) per fit
 constexpr int kFitMinN       = 64;
 constexpr int kFitMaxN       = 1024;
 constexpr int kFitColsWidth  = 1024;  // C * N: columns per column block times N
-constexpr int kFitFftThreads  = kFitColsWidth / 8;  // 2 * C * (N/16): the threads of 2C column transforms
-constexpr int kFitColsThreads = 2 * kFitFftThreads;  // column block: every thread evaluates texels
+constexpr int kFitColsThreads = kFitColsWidth / 4;  // 2 frame pairs x 2C column transforms of N/16 threads
 constexpr int kFitRowsThreads = 256;  // (256 / T) rows of T = N/8 threads
 constexpr int kFitReduceThreads = 256;
 

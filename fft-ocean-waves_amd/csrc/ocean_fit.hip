@@ -15,7 +15,8 @@
 // One fit step, P = ceil(K/2) frame pairs:
 //   k_fit_cols_fwd  item (member, strip): h0 = s(V, theta, A) xi for the strip,
 //                   per pair the folded spectra of both frames, column IFFTs,
-//                   Z_p = G_k1 + i G_k2 (columns tx and N - tx)
+//                   Z_p = G_k1 + i G_k2 (columns tx and N - tx; rows of Z and Y
+//                   in the zpos column order)
 //   k_fit_rows      item (member, rows x pair): row IFFT -> h_k1, h_k2,
 //                   residual against the targets, loss partial; in gradient
 //                   mode the forward row DFT of the signed residual pair -> Y_p
@@ -36,7 +37,7 @@ namespace ocn {
 
 namespace {
 
-constexpr int kColsLds = 2 * (kFitColsWidth + kFitColsWidth / 16);  // 2 * C * lds_len(N)
+constexpr int kColsLds = 4 * (kFitColsWidth + kFitColsWidth / 16);  // 2 pairs * 2C * lds_len(N)
 constexpr int kRowsLds = 2176;                                       // (256 / T) * lds_len(N), T = N/8
 
 // Spectrum constants of the member at its current variables.
@@ -59,11 +60,11 @@ __device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars&
 }
 
 // Hermitian part (H(t) + conj H(t'))/2 of the height spectrum at one time;
-// p = h0 at t, q = h0 at its mirror t', m2 = |n|^2 (shared by t and t').
-__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, int m2, double t_rev)
+// p = h0 at t, q = h0 at its mirror t', w = omega_fp64 (shared by t and t').
+__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, double w, double t_rev)
 {
-    float kf, s, c;
-    sincos_turns(phase_turns(sc, m2, t_rev, kf), s, c);
+    float s, c;
+    sincos_turns(turns(w, t_rev), s, c);
     const float2 H = evolve_amp(sc, p, s, c);
     const float2 G = evolve_amp(sc, q, s, c);
     return make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
@@ -71,11 +72,11 @@ __device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, flo
 
 // dL/dh0 += the pull-back of the frame spectrum gradient G through
 // H = h0k e^{i phi} + C(h0mk) e^{-i phi}: ga += e^{-i phi} G, gb += e^{+i phi} G
-// (identity C) or e^{-i phi} conj(G) (conjugation).
-__device__ __forceinline__ void pull_back(const SpecConst& sc, float2 G, int m2, double t_rev, float2& ga, float2& gb)
+// (identity C) or e^{-i phi} conj(G) (conjugation).  w = omega_fp64 of the texel.
+__device__ __forceinline__ void pull_back(const SpecConst& sc, float2 G, double w, double t_rev, float2& ga, float2& gb)
 {
-    float kf, s, c;
-    sincos_turns(phase_turns(sc, m2, t_rev, kf), s, c);
+    float s, c;
+    sincos_turns(turns(w, t_rev), s, c);
     ga.x += c * G.x + s * G.y;
     ga.y += c * G.y - s * G.x;
     gb.x += c * G.x - s * G.y;
@@ -98,17 +99,20 @@ __device__ __forceinline__ T block_sum(T x, T* red)
 
 // ---------------------------------------------------------------- forward columns
 
+// Column transforms run two frame pairs per round: 2 pairs x 2C columns of T
+// threads fill the 256-thread block.
 template <int N>
 __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)
 {
     constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
-    constexpr int IT = C * N / NT;
-    static_assert(2 * C * T == kFitFftThreads && (C * N) % NT == 0 && T <= 64, "column block shape");
+    constexpr int IT = C * N / NT;  // texel pairs (t, t') per thread
+    constexpr int S  = 2 * C;       // transforms per frame pair
+    static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
     const SpecConst sc = fit_spec(m, *m.vars);
     const size_t    nn = (size_t)N * N;
 
     float4 hp[IT], hq[IT];
-    int    m2[IT];
+    double w[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int k = threadIdx.x + it * NT;
@@ -123,19 +127,26 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         const int nx = txc - N / 2, ny = ty - N / 2;
-        m2[it] = nx * nx + ny * ny;
+        float kf;
+        w[it] = omega_fp64(sc, nx * nx + ny * ny, kf);  // |k| is shared by t and t'
     }
-    for (int p = 0; p < m.P; ++p) {
-        const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[min(2 * p + 1, m.K - 1)];
+    for (int p0 = 0; p0 < m.P; p0 += 2) {
+        const int np = min(2, m.P - p0);
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int k = threadIdx.x + it * NT;
-            const int c = k % C, ty = k / C;
-            lds[c * LEN + lds_pad(ty)]       = fold_height(sc, hp[it], hq[it], m2[it], tr1);
-            lds[(C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], m2[it], tr2);
+        for (int q = 0; q < 2; ++q) {
+            if (q >= np) break;
+            const int    p   = p0 + q;
+            const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[min(2 * p + 1, m.K - 1)];
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int k = threadIdx.x + it * NT;
+                const int c = k % C, ty = k / C;
+                lds[(q * S + c) * LEN + lds_pad(ty)]     = fold_height(sc, hp[it], hq[it], w[it], tr1);
+                lds[(q * S + C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], w[it], tr2);
+            }
         }
         __syncthreads();
-        if (threadIdx.x < kFitFftThreads) {  // whole waves: 2C transforms of T threads
+        if ((int)threadIdx.x < np * S * T) {  // whole waves: np * 2C transforms of T threads
             const int g = threadIdx.x / T, j = threadIdx.x % T;
             float2*   buf = lds + g * LEN;
             float2    v[E];
@@ -147,18 +158,22 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
             for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
         }
         __syncthreads();
-        float2* zp = m.z + (size_t)p * nn;
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int k = threadIdx.x + it * NT;
-            const int c = k % C, y = k / C;
-            const int tx = strip * C + c;
-            if (tx <= N / 2) {
-                const int    mxc = (N - tx) & (N - 1);
-                const float2 g1  = lds[c * LEN + lds_pad(y)];
-                const float2 g2  = lds[(C + c) * LEN + lds_pad(y)];
-                zp[(size_t)y * N + tx] = make_float2(g1.x - g2.y, g1.y + g2.x);
-                if (mxc != tx) zp[(size_t)y * N + mxc] = make_float2(g1.x + g2.y, g2.x - g1.y);
+        for (int q = 0; q < 2; ++q) {
+            if (q >= np) break;
+            float2* zp = m.z + (size_t)(p0 + q) * nn;
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int k = threadIdx.x + it * NT;
+                const int c = k % C, y = k / C;
+                const int tx = strip * C + c;
+                if (tx <= N / 2) {
+                    const int    mxc = (N - tx) & (N - 1);
+                    const float2 g1  = lds[(q * S + c) * LEN + lds_pad(y)];
+                    const float2 g2  = lds[(q * S + C + c) * LEN + lds_pad(y)];
+                    zp[(size_t)y * N + tx] = make_float2(g1.x - g2.y, g1.y + g2.x);
+                    if (mxc != tx) zp[(size_t)y * N + zpos<N>(mxc)] = make_float2(g1.x + g2.y, g2.x - g1.y);
+                }
             }
         }
         __syncthreads();
@@ -182,7 +197,7 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
     float2 v[E];
     const float2* src = m.z + (size_t)p * nn + (size_t)y * N;
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+    for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
     fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
 
     const int   k1 = 2 * p, k2 = 2 * p + 1;
@@ -206,7 +221,7 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
         fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
         float2* dst = m.y + (size_t)p * nn + (size_t)y * N;
 #pragma unroll
-        for (int r = 0; r < E; ++r) dst[j + r * T] = make_float2(v[r].x, -v[r].y);
+        for (int r = 0; r < E; ++r) dst[zpos<N>(j + r * T)] = make_float2(v[r].x, -v[r].y);
     }
     lsum = block_sum(lsum, red);
     if (threadIdx.x == 0) m.loss_part[local] = lsum;
@@ -218,8 +233,9 @@ template <int N>
 __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, float2* lds, float* red)
 {
     constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
-    constexpr int IT = 2 * C * N / NT;  // texels per thread (= N / (2T) = 8)
-    static_assert(2 * C * T == kFitFftThreads && (2 * C * N) % NT == 0 && T <= 64, "column block shape");
+    constexpr int IT = 2 * C * N / NT;  // texels per thread
+    constexpr int S  = 2 * C;           // transforms per frame pair: the strip's columns and their mirrors
+    static_assert(2 * S * T == NT && (2 * C * N) % NT == 0 && T <= 64, "column block shape");
     const FitVars   vars = *m.vars;
     const SpecConst sc = fit_spec(m, vars);
     const size_t    nn = (size_t)N * N;
@@ -230,21 +246,34 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         return slot < C ? txc : (N - txc) & (N - 1);
     };
     float2 ga[IT], gb[IT];
+    double w[IT];
 #pragma unroll
-    for (int it = 0; it < IT; ++it) ga[it] = gb[it] = make_float2(0.0f, 0.0f);
+    for (int it = 0; it < IT; ++it) {
+        ga[it] = gb[it] = make_float2(0.0f, 0.0f);
+        const int kk = threadIdx.x + it * NT;
+        const int slot = kk % S, ty = kk / S;
+        const int nx = column(slot) - N / 2, ny = ty - N / 2;
+        float kf;
+        w[it] = omega_fp64(sc, nx * nx + ny * ny, kf);
+    }
 
-    for (int p = 0; p < m.P; ++p) {
-        const float2* yp = m.y + (size_t)p * nn;
+    for (int p0 = 0; p0 < m.P; p0 += 2) {
+        const int np = min(2, m.P - p0);
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int kk = threadIdx.x + it * NT;
-            const int slot = kk % (2 * C), ty = kk / (2 * C);
-            const int txc = strip * C + slot % C;
-            const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + column(slot)] : make_float2(0.0f, 0.0f);
-            lds[slot * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
+        for (int q = 0; q < 2; ++q) {
+            if (q >= np) break;
+            const float2* yp = m.y + (size_t)(p0 + q) * nn;
+#pragma unroll
+            for (int it = 0; it < IT; ++it) {
+                const int kk = threadIdx.x + it * NT;
+                const int slot = kk % S, ty = kk / S;
+                const int txc = strip * C + slot % C;
+                const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + zpos<N>(column(slot))] : make_float2(0.0f, 0.0f);
+                lds[(q * S + slot) * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
+            }
         }
         __syncthreads();
-        if (threadIdx.x < kFitFftThreads) {  // whole waves: 2C transforms of T threads
+        if ((int)threadIdx.x < np * S * T) {  // whole waves: np * 2C transforms of T threads
             const int g = threadIdx.x / T, j = threadIdx.x % T;
             float2*   buf = lds + g * LEN;
             float2    v[E];
@@ -256,23 +285,25 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
             for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
         }
         __syncthreads();
-        const int    k1 = 2 * p, k2 = 2 * p + 1;
-        const bool   two = k2 < m.K;
-        const double tr1 = m.t_rev[k1], tr2 = m.t_rev[two ? k2 : k1];
+#pragma unroll 1
+        for (int q = 0; q < 2; ++q) {
+            if (q >= np) break;
+            const int    k1 = 2 * (p0 + q), k2 = k1 + 1;
+            const bool   two = k2 < m.K;
+            const double tr1 = m.t_rev[k1], tr2 = m.t_rev[two ? k2 : k1];
 #pragma unroll
-        for (int it = 0; it < IT; ++it) {
-            const int kk = threadIdx.x + it * NT;
-            const int slot = kk % (2 * C), ty = kk / (2 * C);
-            const int mslot = slot < C ? slot + C : slot - C, my = (N - ty) & (N - 1);
-            // F = conj of the stored transform; G1 = (F(u) + conj F(-u))/2, G2 = (F(u) - conj F(-u))/(2i)
-            const float2 fu = lds[slot * LEN + lds_pad(ty)];
-            const float2 fm = lds[mslot * LEN + lds_pad(my)];
-            const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
-            const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
-            const int nx = column(slot) - N / 2, ny = ty - N / 2;
-            const int m2 = nx * nx + ny * ny;
-            pull_back(sc, G1, m2, tr1, ga[it], gb[it]);
-            if (two) pull_back(sc, G2, m2, tr2, ga[it], gb[it]);
+            for (int it = 0; it < IT; ++it) {
+                const int kk = threadIdx.x + it * NT;
+                const int slot = kk % S, ty = kk / S;
+                const int mslot = slot < C ? slot + C : slot - C, my = (N - ty) & (N - 1);
+                // F = conj of the stored transform; G1 = (F(u) + conj F(-u))/2, G2 = (F(u) - conj F(-u))/(2i)
+                const float2 fu = lds[(q * S + slot) * LEN + lds_pad(ty)];
+                const float2 fm = lds[(q * S + mslot) * LEN + lds_pad(my)];
+                const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
+                const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
+                pull_back(sc, G1, w[it], tr1, ga[it], gb[it]);
+                if (two) pull_back(sc, G2, w[it], tr2, ga[it], gb[it]);
+            }
         }
         __syncthreads();
     }
@@ -284,7 +315,7 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int kk = threadIdx.x + it * NT;
-        const int slot = kk % (2 * C), ty = kk / (2 * C);
+        const int slot = kk % S, ty = kk / S;
         const int txc = strip * C + slot % C;
         if (txc > N / 2 || (slot >= C && (txc == 0 || txc == N / 2))) continue;  // not this block's texel
         const int    tx = column(slot);
@@ -298,10 +329,10 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         const float  kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
         const float  d    = (kx * sc.wx + ky * sc.wy) / kmag;
         const float  dp   = (ky * sc.wx - kx * sc.wy) / kmag;
-        const float  w    = dLds * s;
-        dV = fmaf(w, dv_scale / (kmag * kmag), dV);
-        dT = fmaf(w, dp / d, dT);
-        dA = fmaf(w, da_scale, dA);
+        const float  wgt  = dLds * s;
+        dV = fmaf(wgt, dv_scale / (kmag * kmag), dV);
+        dT = fmaf(wgt, dp / d, dT);
+        dA = fmaf(wgt, da_scale, dA);
     }
     dV = block_sum(dV, red);
     dT = block_sum(dT, red);

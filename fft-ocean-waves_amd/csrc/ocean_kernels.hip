@@ -144,7 +144,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
     for (int r = 0; r < E; ++r) {
         const size_t row = (size_t)(j + r * T) * N;
         z1[row + tx] = make_float2(ay[r].x - ax[r].y, ay[r].y + ax[r].x);
-        if (mirror) z1[row + mxc] = make_float2(ay[r].x + ax[r].y, ax[r].x - ay[r].y);
+        if (mirror) z1[row + zpos<N>(mxc)] = make_float2(ay[r].x + ax[r].y, ax[r].x - ay[r].y);
     }
 #pragma unroll
     for (int r = 0; r < E / 2; ++r) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
         const size_t row = (size_t)p * N;
         const float2 lo = buf[lds_pad(2 * p)], hi = buf[lds_pad(2 * p + 1)];  // rows 2p and 2p+1
         z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
-        if (mirror) z2[row + mxc] = make_float2(lo.x + hi.y, hi.x - lo.y);
+        if (mirror) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
     }
 }
 
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
     const float2* src = g == 0 ? z1 + (size_t)(2 * p) * N : g == 1 ? z1 + (size_t)(2 * p + 1) * N : z2 + (size_t)p * N;
     float2 v[E];
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[j + r * T];
+    for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
     fft_inverse<N, E>(v, j, lds + g * lds_len(N), tw, BlockSync{});
     constexpr float scale = 1.0f / ((float)N * (float)N);
     const size_t row0 = (size_t)(2 * p) * N, row1 = row0 + N;
