@@ -161,6 +161,17 @@ __device__ __forceinline__ int zpos(int x)
     return x <= N / 2 ? x : 3 * N / 2 - x;
 }
 
+// zpos of element r of a row transform held as x = j + r*T (T = N/E, j < T),
+// with the half the element falls in known at compile time.
+template <int N, int E>
+__device__ __forceinline__ int zpos_elem(int j, int r)
+{
+    constexpr int T = N / E;
+    if (2 * r < E) return j + r * T;
+    if (2 * r > E) return 3 * N / 2 - (j + r * T);
+    return j == 0 ? N / 2 : N - j;
+}
+
 // ------------------------------------------------------------------ XCD-aware block order
 
 // MI355X spreads the workgroups of a launch round-robin over its 8 XCDs
@@ -281,7 +292,7 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
     // inverse transform of one Z row into v (natural order, thread j holds x = j + r*T)
     auto fft_row = [&](const float2* src, float2 (&v)[E]) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
+        for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
         fft_inverse<N, E>(v, j, buf, tw, Sync{});
     };
     // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y

@@ -197,7 +197,7 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
     float2 v[E];
     const float2* src = m.z + (size_t)p * nn + (size_t)y * N;
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
+    for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
     fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
 
     const int   k1 = 2 * p, k2 = 2 * p + 1;
@@ -221,7 +221,7 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
         fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
         float2* dst = m.y + (size_t)p * nn + (size_t)y * N;
 #pragma unroll
-        for (int r = 0; r < E; ++r) dst[zpos<N>(j + r * T)] = make_float2(v[r].x, -v[r].y);
+        for (int r = 0; r < E; ++r) dst[zpos_elem<N, E>(j, r)] = make_float2(v[r].x, -v[r].y);
     }
     lsum = block_sum(lsum, red);
     if (threadIdx.x == 0) m.loss_part[local] = lsum;

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
     const float2* src = g == 0 ? z1 + (size_t)(2 * p) * N : g == 1 ? z1 + (size_t)(2 * p + 1) * N : z2 + (size_t)p * N;
     float2 v[E];
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[zpos<N>(j + r * T)];
+    for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
     fft_inverse<N, E>(v, j, lds + g * lds_len(N), tw, BlockSync{});
     constexpr float scale = 1.0f / ((float)N * (float)N);
     const size_t row0 = (size_t)(2 * p) * N, row1 = row0 + N;
