@@ -84,9 +84,23 @@ def cpu_baseline(seconds):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return {"value": round(n / el, 4), "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": "%d whole N=1024 frames (spectrum, 3 radix-2 butterfly IFFTs + inversion, normals, "
-                      "Jacobian) of the FP64 oracle in %.1f s, 1 thread" % (n, el)}
+    out = {"value": round(n / el, 4), "unit": "frames/s", "cores": 1, "kind": "port",
+           "sample": "%d whole N=1024 frames (spectrum, 3 radix-2 butterfly IFFTs + inversion, normals, "
+                     "Jacobian) of the FP64 oracle in %.1f s, 1 thread" % (n, el)}
+    # the same frames with the oracle's butterfly stages spread over the host's
+    # cores (one pthread per row range; spectrum, inversion, normals and
+    # Jacobian stay single-threaded)
+    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
+    n2, t0 = 0, time.perf_counter()
+    while True:
+        oracle.frame(op, h0, n2 / 60.0, nthreads=cores)
+        n2 += 1
+        el2 = time.perf_counter() - t0
+        if el2 >= seconds / 2:
+            break
+    out["threaded"] = {"value": round(n2 / el2, 4), "unit": "frames/s", "cores": cores,
+                       "sample": "%d frames in %.1f s, butterfly stages on %d threads" % (n2, el2, cores)}
+    return out
 
 
 def cascades_bench(hip, frames, warmup):

@@ -63,6 +63,7 @@ int validate(const ocn_params* p)
         return fail(OCN_EINVAL, "suppression must be finite and >= 0");
     if (!std::isfinite(p->choppiness)) return fail(OCN_EINVAL, "choppiness must be finite");
     if (!(std::isfinite(p->gravity) && p->gravity > 0.0)) return fail(OCN_EINVAL, "gravity must be finite and > 0");
+    if (!std::isfinite(p->displacement_scale)) return fail(OCN_EINVAL, "displacement_scale must be finite");
     if (p->conj_mode != OCN_CONJ_REFERENCE && p->conj_mode != OCN_CONJ_TEXTBOOK)
         return fail(OCN_EINVAL, "conj_mode must be OCN_CONJ_REFERENCE or OCN_CONJ_TEXTBOOK");
     if (p->normal_mode != OCN_NORMALS_NEAREST && p->normal_mode != OCN_NORMALS_BILINEAR)
@@ -285,8 +286,11 @@ void frame_args(const ocn_handle* h, const double* t, int nframes, const ocn_fie
     a->dz            = out->dz;
     a->normals       = reinterpret_cast<float4*>(out->normals);
     a->jacobian      = out->jacobian;
+    a->disp          = reinterpret_cast<float4*>(out->displacement);
+    a->rnorm         = reinterpret_cast<float4*>(out->render_normals);
     a->normal_mode   = h->p.normal_mode;
     a->lambda        = (float)h->p.choppiness;
+    a->dscale        = (float)h->p.displacement_scale;
     a->inv2d         = (float)((double)h->p.N / (2.0 * h->p.L));
 }
 
@@ -481,6 +485,7 @@ int ocn_params_default(ocn_params* p)
     p->suppression = 0.1;    // m_suppression_factor (main.cpp:1643)
     p->choppiness  = 0.75;   // m_choppiness (main.cpp:1633)
     p->gravity     = 9.81;   // g (tilde_h0_k_cs.glsl:26)
+    p->displacement_scale = 0.5;  // m_displacement_scale (main.cpp:1632)
     p->seed        = 42;
     p->conj_mode   = OCN_CONJ_REFERENCE;
     p->normal_mode = OCN_NORMALS_NEAREST;
@@ -644,9 +649,13 @@ int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn
     pick_schedule(h, nframes, &cv, &rv);
     for (int f0 = 0; f0 < nframes; f0 += ocn::kMaxFrames) {
         const int nf = std::min(ocn::kMaxFrames, nframes - f0);
-        ocn_fields s{out->height + f0 * nn, out->dx + f0 * nn, out->dz + f0 * nn,
+        ocn_fields s{out->height + f0 * nn,
+                     out->dx + f0 * nn,
+                     out->dz + f0 * nn,
                      out->normals ? out->normals + f0 * nn * 4 : nullptr,
-                     out->jacobian ? out->jacobian + f0 * nn : nullptr};
+                     out->jacobian ? out->jacobian + f0 * nn : nullptr,
+                     out->displacement ? out->displacement + f0 * nn * 4 : nullptr,
+                     out->render_normals ? out->render_normals + f0 * nn * 4 : nullptr};
         if ((rc = run_frames(h, t_host + f0, nf, &s, nn, stage, cv, rv, (hipStream_t)stream))) return rc;
     }
     return OCN_OK;

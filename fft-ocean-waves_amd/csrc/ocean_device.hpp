@@ -276,13 +276,17 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
 // normal_map_cs.glsl:37-53; J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 dDx/dz dDz/dx
 // for the render displacement (-l Dx, -l Dz) (grid_tes.glsl:60-62), central
 // differences with spacing L/N.  All pointers are at the frame's data;
-// normals or jac may be null.
-template <int N, int E, int R, class Sync>
+// normals or jac may be null.  RENDER adds the render export (disp, rnorm,
+// either may be null): (-l Dx, s h, -l Dz, J) and the unit normal of the
+// displaced surface, dP/dv x dP/du.
+template <int N, int E, int R, class Sync, bool RENDER = false>
 __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, const float2* __restrict__ z2,
                                               float* __restrict__ height, float* __restrict__ dx,
                                               float* __restrict__ dz, float4* __restrict__ normals,
                                               float* __restrict__ jac, const float2* __restrict__ tw, float2* buf,
-                                              int j, int band, float lambda, float inv2d)
+                                              int j, int band, float lambda, float inv2d,
+                                              float4* __restrict__ disp = nullptr, float4* __restrict__ rnorm = nullptr,
+                                              float dscale = 0.0f)
 {
     constexpr int T = N / E;
     static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
@@ -369,7 +373,19 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 normals[row + x] = make_float4(nx * inv, inv, nz * inv, 1.0f);
             }
         }
-        if (jac) {
+        if (jac || (RENDER && (disp || rnorm))) {
+            float dhdx[E];
+            if (RENDER && rnorm) {  // x-derivative of the height for the displaced-surface normal
+                Sync{}();
+#pragma unroll
+                for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = make_float2(hB[r], 0.0f);
+                Sync{}();
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const int x = j + r * T;
+                    dhdx[r] = (buf[lds_pad((x + 1) & (N - 1))].x - buf[lds_pad((x - 1) & (N - 1))].x) * inv2d;
+                }
+            }
             Sync{}();
 #pragma unroll
             for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = make_float2(xB[r], zB[r]);
@@ -383,7 +399,19 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 const float  dzdx = (rt.y - lf.y) * inv2d;
                 const float  dxdz = (sc * v[r].y - xA[r]) * inv2d;
                 const float  dzdz = (zC[r] - zA[r]) * inv2d;
-                jac[row + x] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+                const float  J = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+                if (jac) jac[row + x] = J;
+                if (!RENDER) continue;
+                if (disp) disp[row + x] = make_float4(-lambda * xB[r], dscale * hB[r], -lambda * zB[r], J);
+                if (rnorm) {
+                    // n = dP/dv x dP/du with dP/du = (1 - l dDx/dx, s dh/dx, -l dDz/dx),
+                    // dP/dv = (-l dDx/dz, s dh/dz, 1 - l dDz/dz); its y component is J
+                    const float dhdz = (sc * v[r].x - hA[r]) * inv2d;
+                    const float nx = -dscale * (lambda * dhdz * dzdx + (1.0f - lambda * dzdz) * dhdx[r]);
+                    const float nz = -dscale * (lambda * dxdz * dhdx[r] + dhdz * (1.0f - lambda * dxdx));
+                    const float inv = rsqrtf(fmaf(nx, nx, fmaf(J, J, nz * nz)));
+                    rnorm[row + x] = make_float4(nx * inv, J * inv, nz * inv, 1.0f);
+                }
             }
         }
 #pragma unroll

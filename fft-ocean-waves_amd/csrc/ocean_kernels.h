@@ -54,8 +54,11 @@ struct FrameArgs {
     float*        dz;
     float4*       normals;    // may be null
     float*        jacobian;   // may be null
+    float4*       disp;       // render export, may be null
+    float4*       rnorm;      // render export, may be null
     int           normal_mode;
     float         lambda;
+    float         dscale;     // render height scale
     float         inv2d;      // N / (2 L)
 };
 

@@ -234,11 +234,12 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
 // lie in one wave when T <= 64 (wave-level ordering), otherwise a block is one
 // group.  1-D grid of ceil(bands*frames / groups per block) blocks; after the
 // XCD remap, neighbouring bands (which share halo rows) run on the same XCD.
-template <int N, int E, int R>
+template <int N, int E, int R, bool RENDER>
 __global__ __launch_bounds__(N / E >= 64 ? N / E : 64) void k_rows_nrm(
     const float2* __restrict__ z1, const float2* __restrict__ z2, float* __restrict__ height,
     float* __restrict__ dx, float* __restrict__ dz, float4* __restrict__ normals, float* __restrict__ jac,
-    const float2* __restrict__ tw, FrameBatch fb, float lambda, float inv2d)
+    const float2* __restrict__ tw, FrameBatch fb, float lambda, float inv2d, float4* __restrict__ disp,
+    float4* __restrict__ rnorm, float dscale)
 {
     constexpr int T   = N / E;
     constexpr int GPB = T >= 64 ? 1 : 64 / T;  // groups per block
@@ -252,9 +253,10 @@ __global__ __launch_bounds__(N / E >= 64 ? N / E : 64) void k_rows_nrm(
     const int f    = L / NB;
     if (f >= fb.nframes) return;  // whole group (a wave part when T <= 64): padding of the last block
     const size_t fo = (size_t)f * fb.out_stride;
-    rows_nrm_band<N, E, R, Sync>(z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo, dx + fo,
-                                 dz + fo, normals ? normals + fo : nullptr, jac ? jac + fo : nullptr, tw,
-                                 lds + g * lds_len(N), j, band, lambda, inv2d);
+    rows_nrm_band<N, E, R, Sync, RENDER>(z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo,
+                                         dx + fo, dz + fo, normals ? normals + fo : nullptr,
+                                         jac ? jac + fo : nullptr, tw, lds + g * lds_len(N), j, band, lambda, inv2d,
+                                         disp ? disp + fo : nullptr, rnorm ? rnorm + fo : nullptr, dscale);
 }
 
 // ------------------------------------------------------------------ A7 normals (+ Jacobian)
@@ -283,7 +285,8 @@ __device__ __forceinline__ float stap(const float (*s)[kNrmLW], int y, int x, in
 __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ height, const float* __restrict__ dx,
                                                  const float* __restrict__ dz, float4* __restrict__ normals,
                                                  float* __restrict__ jac, int N, int mode, float lambda,
-                                                 float inv2d, size_t stride)
+                                                 float inv2d, size_t stride, float4* __restrict__ disp,
+                                                 float4* __restrict__ rnorm, float dscale)
 {
     __shared__ float sh[kNrmLH][kNrmLW], sx[kNrmLH][kNrmLW], sz[kNrmLH][kNrmLW];
     const int tiles_x = N / kNrmW, tiles_y = N / kNrmH;
@@ -297,12 +300,15 @@ __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ heigh
     dz += fo;
     if (normals) normals += fo;
     if (jac) jac += fo;
+    if (disp) disp += fo;
+    if (rnorm) rnorm += fo;
+    const bool need_h = normals || disp || rnorm, need_xz = jac || disp || rnorm;
     const int x0 = bx * kNrmW, y0 = by * kNrmH;
     for (int k = threadIdx.x; k < kNrmLH * kNrmLW; k += 256) {
         const int ly = k / kNrmLW, lx = k % kNrmLW;
         const size_t gi = (size_t)((y0 + ly - kNrmHalo) & (N - 1)) * N + ((x0 + lx - kNrmHalo) & (N - 1));
-        if (normals) sh[ly][lx] = height[gi];
-        if (jac) {
+        if (need_h) sh[ly][lx] = height[gi];
+        if (need_xz) {
             sx[ly][lx] = dx[gi];
             sz[ly][lx] = dz[gi];
         }
@@ -325,12 +331,23 @@ __global__ __launch_bounds__(256) void k_normals(const float* __restrict__ heigh
             const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
             normals[i] = make_float4(nx * inv, inv, nz * inv, 1.0f);
         }
-        if (jac) {
+        if (need_xz) {
             const float dxdx = (sx[cy][cx + 1] - sx[cy][cx - 1]) * inv2d;
             const float dxdz = (sx[cy + 1][cx] - sx[cy - 1][cx]) * inv2d;
             const float dzdx = (sz[cy][cx + 1] - sz[cy][cx - 1]) * inv2d;
             const float dzdz = (sz[cy + 1][cx] - sz[cy - 1][cx]) * inv2d;
-            jac[i] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            const float J    = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            if (jac) jac[i] = J;
+            if (disp) disp[i] = make_float4(-lambda * sx[cy][cx], dscale * sh[cy][cx], -lambda * sz[cy][cx], J);
+            if (rnorm) {
+                // n = dP/dv x dP/du of P = (u - l Dx, s h, v - l Dz); its y component is J
+                const float dhdx = (sh[cy][cx + 1] - sh[cy][cx - 1]) * inv2d;
+                const float dhdz = (sh[cy + 1][cx] - sh[cy - 1][cx]) * inv2d;
+                const float nx   = -dscale * (lambda * dhdz * dzdx + (1.0f - lambda * dzdz) * dhdx);
+                const float nz   = -dscale * (lambda * dxdz * dhdx + dhdz * (1.0f - lambda * dxdx));
+                const float inv  = rsqrtf(fmaf(nx, nx, fmaf(J, J, nz * nz)));
+                rnorm[i] = make_float4(nx * inv, J * inv, nz * inv, 1.0f);
+            }
         }
     }
 }
@@ -450,9 +467,14 @@ static hipError_t launch_rows_v(const FrameArgs& a, hipStream_t s)
     } else {
         constexpr int T = N / E, GPB = T >= 64 ? 1 : 64 / T;
         const int groups = (N / B) * a.nframes;
-        hipLaunchKernelGGL((k_rows_nrm<N, E, B>), dim3((groups + GPB - 1) / GPB), dim3(GPB * T), 0, s, a.z1, a.z2,
-                           a.height, a.dx, a.dz, a.normals, a.jacobian, a.tw[radix_slot(E)], a.fb, a.lambda,
-                           a.inv2d);
+        if (a.disp || a.rnorm)
+            hipLaunchKernelGGL((k_rows_nrm<N, E, B, true>), dim3((groups + GPB - 1) / GPB), dim3(GPB * T), 0, s,
+                               a.z1, a.z2, a.height, a.dx, a.dz, a.normals, a.jacobian, a.tw[radix_slot(E)], a.fb,
+                               a.lambda, a.inv2d, a.disp, a.rnorm, a.dscale);
+        else
+            hipLaunchKernelGGL((k_rows_nrm<N, E, B, false>), dim3((groups + GPB - 1) / GPB), dim3(GPB * T), 0, s,
+                               a.z1, a.z2, a.height, a.dx, a.dz, a.normals, a.jacobian, a.tw[radix_slot(E)], a.fb,
+                               a.lambda, a.inv2d, nullptr, nullptr, 0.0f);
         return hipGetLastError();
     }
 }
@@ -494,9 +516,10 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         }
         if (e != hipSuccess) return e;
     }
-    if ((stages & 4) && !fused && (a.normals || a.jacobian)) {
+    if ((stages & 4) && !fused && (a.normals || a.jacobian || a.disp || a.rnorm)) {
         hipLaunchKernelGGL(k_normals, dim3((N / kNrmW) * (N / kNrmH) * a.nframes), dim3(256), 0, s, a.height, a.dx, a.dz,
-                           a.normals, a.jacobian, N, a.normal_mode, a.lambda, a.inv2d, a.fb.out_stride);
+                           a.normals, a.jacobian, N, a.normal_mode, a.lambda, a.inv2d, a.fb.out_stride, a.disp,
+                           a.rnorm, a.dscale);
         e = hipGetLastError();
     }
     return e;
@@ -590,7 +613,7 @@ hipError_t launch_normals(int N, const float* height, const float* dx, const flo
                           float* jacobian, int mode, float lambda, float inv2d, hipStream_t s)
 {
     hipLaunchKernelGGL(k_normals, dim3((N / kNrmW) * (N / kNrmH)), dim3(256), 0, s, height, dx, dz, normals, jacobian, N, mode,
-                       lambda, inv2d, (size_t)0);
+                       lambda, inv2d, (size_t)0, (float4*)nullptr, (float4*)nullptr, 0.0f);
     return hipGetLastError();
 }
 

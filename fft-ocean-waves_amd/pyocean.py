@@ -57,12 +57,14 @@ class OcnParams(ctypes.Structure):
         ("suppression", ctypes.c_double),
         ("choppiness", ctypes.c_double),
         ("gravity", ctypes.c_double),
+        ("displacement_scale", ctypes.c_double),
     ]
 
 
 class OcnFields(ctypes.Structure):
     _fields_ = [("height", ctypes.c_void_p), ("dx", ctypes.c_void_p), ("dz", ctypes.c_void_p),
-                ("normals", ctypes.c_void_p), ("jacobian", ctypes.c_void_p)]
+                ("normals", ctypes.c_void_p), ("jacobian", ctypes.c_void_p),
+                ("displacement", ctypes.c_void_p), ("render_normals", ctypes.c_void_p)]
 
 
 class OcnAdam(ctypes.Structure):
@@ -164,7 +166,7 @@ def default_params():
 
 def make_params(N=256, L=1000.0, wind_speed=40.0, wind_dir=(1.0, 1.0), amplitude=4.0, suppression=None,
                 choppiness=1.0, gravity=9.81, seed=42, ocean_id=0, cascade_id=0,
-                conj_mode=OCN_CONJ_REFERENCE, normal_mode=OCN_NORMALS_NEAREST):
+                conj_mode=OCN_CONJ_REFERENCE, normal_mode=OCN_NORMALS_NEAREST, displacement_scale=0.5):
     """Parameter block; suppression defaults to L/2000 (the small-wave cutoff rule)."""
     p = default_params()
     p.N, p.L, p.wind_speed = int(N), float(L), float(wind_speed)
@@ -174,6 +176,7 @@ def make_params(N=256, L=1000.0, wind_speed=40.0, wind_dir=(1.0, 1.0), amplitude
     p.choppiness, p.gravity = float(choppiness), float(gravity)
     p.seed, p.ocean_id, p.cascade_id = int(seed), int(ocean_id), int(cascade_id)
     p.conj_mode, p.normal_mode = int(conj_mode), int(normal_mode)
+    p.displacement_scale = float(displacement_scale)
     return p
 
 

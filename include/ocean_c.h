@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define OCN_ABI_VERSION 1
+#define OCN_ABI_VERSION 2
 
 /* status codes */
 #define OCN_OK        0
@@ -81,20 +81,36 @@ typedef struct ocn_params {
     double   wind_dir_y;
     double   amplitude;    /* Phillips A, >= 0 */
     double   suppression;  /* small-wave cutoff l (m), >= 0 (u_SuppressFactor) */
-    double   choppiness;   /* lambda; enters only the Jacobian (the reference applies it at render) */
+    double   choppiness;   /* lambda; enters only the Jacobian and the render export (the reference
+                              applies it at render, u_Choppiness) */
     double   gravity;      /* g (m/s^2), > 0 */
+    double   displacement_scale; /* s, height scale of the render export (m_displacement_scale 0.5,
+                                    main.cpp:1632; u_DisplacementScale, grid_tes.glsl:58) */
 } ocn_params;
 
 /* Output fields of one frame (device pointers).  height/dx/dz are unscaled
  * displacement (the render applies (-lambda*dx, scale*height, -lambda*dz),
- * grid_tes.glsl:60-62).  normals is RGBA32F (nx, ny, nz, 1).  jacobian is
- * optional (NULL skips it); normals may be NULL to skip the normal map. */
+ * grid_tes.glsl:58-60).  normals is RGBA32F (nx, ny, nz, 1).  jacobian is
+ * optional (NULL skips it); normals may be NULL to skip the normal map.
+ *
+ * The render export (both optional, NULL skips them; no reference
+ * counterpart, it fuses what grid_tes.glsl:58-62 does per vertex):
+ *   displacement   float4 (-lambda*dx, s*height, -lambda*dz, J): the world-space
+ *                  offset the reference's tessellation shader adds, with the
+ *                  Jacobian J (foam where J is small or negative) in w;
+ *   render_normals float4 (nx, ny, nz, 1): the unit normal of the displaced
+ *                  surface P(u, v) = (u - lambda*dx, s*height, v - lambda*dz),
+ *                  n = dP/dv x dP/du (so ny = J / |n|), derivatives by periodic
+ *                  central differences with spacing L/N.  The reference shades
+ *                  with the undisplaced Sobel normal map instead. */
 typedef struct ocn_fields {
-    float* height;   /* m_dy         (main.cpp:1097) */
-    float* dx;       /* m_dx         (main.cpp:1096) */
-    float* dz;       /* m_dz         (main.cpp:1098) */
-    float* normals;  /* m_normal_map (main.cpp:1099) */
-    float* jacobian; /* no reference counterpart: foam/folding term */
+    float* height;         /* m_dy         (main.cpp:1097) */
+    float* dx;             /* m_dx         (main.cpp:1096) */
+    float* dz;             /* m_dz         (main.cpp:1098) */
+    float* normals;        /* m_normal_map (main.cpp:1099) */
+    float* jacobian;       /* no reference counterpart: foam/folding term */
+    float* displacement;   /* render export, float4 per texel */
+    float* render_normals; /* render export, float4 per texel */
 } ocn_fields;
 
 typedef struct ocn_handle ocn_handle;

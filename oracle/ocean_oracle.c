@@ -25,6 +25,10 @@
  *                                                  sinusoid gives the analytic Sobel response.
  *   orc_jacobian                                 : zero displacement -> 1; a linear-in-x
  *                                                  displacement field gives the closed form.
+ *   orc_render                                   : flat -> (0,1,0) and J = 1; a height-only
+ *                                                  sinusoid gives the analytic normal; the
+ *                                                  normal is orthogonal to both tangents and
+ *                                                  its y component times |n| is J.
  *   orc_fit_eval                                 : loss from an independent numpy inverse DFT;
  *                                                  dL/dh0 vs FP64 central differences on
  *                                                  sampled texels (both conj modes); the
@@ -417,6 +421,45 @@ void orc_jacobian(int N, double L, double lambda, const double* dx, const double
             double dzdz = (tap_nearest(N, dz, x, y + 1) - tap_nearest(N, dz, x, y - 1)) * inv2d;
             out[(size_t)y * N + x] =
                 (1.0 - lambda * dxdx) * (1.0 - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+        }
+}
+
+/* Render export (no reference counterpart; what grid_tes.glsl:58-62 applies per
+ * vertex, fused): disp = (-l Dx, s h, -l Dz, J) and the unit normal of the
+ * displaced surface P(u, v) = (u - l Dx, s h, v - l Dz), n = dP/dv x dP/du,
+ * with periodic central differences of spacing L/N (u along x, v along y);
+ * out = 4*N*N each, rnorm w = 1.  Either output may be NULL. */
+void orc_render(int N, double L, double lambda, double scale, const double* h, const double* dx, const double* dz,
+                double* disp, double* rnorm)
+{
+    const double inv2d = (double)N / (2.0 * L);
+    for (int y = 0; y < N; ++y)
+        for (int x = 0; x < N; ++x) {
+            const double dxdx = (tap_nearest(N, dx, x + 1, y) - tap_nearest(N, dx, x - 1, y)) * inv2d;
+            const double dxdz = (tap_nearest(N, dx, x, y + 1) - tap_nearest(N, dx, x, y - 1)) * inv2d;
+            const double dzdx = (tap_nearest(N, dz, x + 1, y) - tap_nearest(N, dz, x - 1, y)) * inv2d;
+            const double dzdz = (tap_nearest(N, dz, x, y + 1) - tap_nearest(N, dz, x, y - 1)) * inv2d;
+            const double dhdx = (tap_nearest(N, h, x + 1, y) - tap_nearest(N, h, x - 1, y)) * inv2d;
+            const double dhdz = (tap_nearest(N, h, x, y + 1) - tap_nearest(N, h, x, y - 1)) * inv2d;
+            /* tangents */
+            const double tu[3] = {1.0 - lambda * dxdx, scale * dhdx, -lambda * dzdx};
+            const double tv[3] = {-lambda * dxdz, scale * dhdz, 1.0 - lambda * dzdz};
+            const double n[3]  = {tv[1] * tu[2] - tv[2] * tu[1], tv[2] * tu[0] - tv[0] * tu[2],
+                                  tv[0] * tu[1] - tv[1] * tu[0]};
+            const size_t i = (size_t)y * N + x;
+            if (disp) {
+                disp[4 * i + 0] = -lambda * dx[i];
+                disp[4 * i + 1] = scale * h[i];
+                disp[4 * i + 2] = -lambda * dz[i];
+                disp[4 * i + 3] = (1.0 - lambda * dxdx) * (1.0 - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            }
+            if (rnorm) {
+                const double m = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+                rnorm[4 * i + 0] = n[0] / m;
+                rnorm[4 * i + 1] = n[1] / m;
+                rnorm[4 * i + 2] = n[2] / m;
+                rnorm[4 * i + 3] = 1.0;
+            }
         }
 }
 

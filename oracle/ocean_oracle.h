@@ -83,6 +83,11 @@ int orc_spectrum_t(const orc_params* p, const double* h0, double t,
  * by central differences on the grid spacing L/N, periodic. */
 void orc_jacobian(int N, double L, double lambda, const double* dx, const double* dz, double* out);
 
+/* Render export (no reference code; fuses grid_tes.glsl:58-62): disp = (-l Dx,
+ * s h, -l Dz, J) and the unit normal of the displaced surface, 4*N*N each. */
+void orc_render(int N, double L, double lambda, double scale, const double* h, const double* dx, const double* dz,
+                double* disp, double* rnorm);
+
 /* A10 extension (no reference code): the differentiable fit of the height.
  * L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2 over K frames at times[k],
  * targets K*N*N.  grad_h0 (N*N*4, may be NULL) receives dL/dh0 as

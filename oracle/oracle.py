@@ -66,6 +66,8 @@ def lib():
         L.orc_jacobian.restype = None
         L.orc_frame.argtypes = [ctypes.POINTER(OrcParams), dp, ctypes.c_double, dp, dp, dp, dp, dp,
                                 ctypes.c_int]
+        L.orc_render.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp, dp, dp, dp, dp]
+        L.orc_render.restype = None
         L.orc_fit_eval.argtypes = [ctypes.POINTER(OrcParams), dp, ctypes.c_int, dp, dp, dp, dp, ctypes.c_int]
         L.orc_fit_param_grad.argtypes = [ctypes.POINTER(OrcParams), dp, dp]
         _lib = L
@@ -224,3 +226,12 @@ def fit_param_grad(params, grad_h0):
     out = np.zeros(3, dtype=np.float64)
     assert lib().orc_fit_param_grad(ctypes.byref(params), _dp(g), _dp(out)) == 0
     return out
+
+
+def render(N, L, lam, scale, h, dx, dz):
+    """Render export: (displacement (N, N, 4), normals of the displaced surface (N, N, 4))."""
+    a, b, c = (np.ascontiguousarray(v, dtype=np.float64) for v in (h, dx, dz))
+    disp = np.zeros((N, N, 4), dtype=np.float64)
+    rn = np.zeros((N, N, 4), dtype=np.float64)
+    lib().orc_render(N, float(L), float(lam), float(scale), _dp(a), _dp(b), _dp(c), _dp(disp), _dp(rn))
+    return disp, rn

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     dyn = subprocess.run(["nm", "-D", "--defined-only", pyocean.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r"\bT (ocn_\w+)", dyn))
     assert set(names) <= exported
-    assert L.ocn_abi_version() == 1
+    assert L.ocn_abi_version() == 2
 
 
 def test_params_default_matches_reference_members():
@@ -43,19 +43,21 @@ def test_params_default_matches_reference_members():
         (256, 1000.0, 80.0, 2.0, 0.1, 0.75, 9.81)
     assert (p.wind_dir_x, p.wind_dir_y) == (1.0, 1.0)
     assert p.conj_mode == pyocean.OCN_CONJ_REFERENCE and p.normal_mode == pyocean.OCN_NORMALS_NEAREST
+    assert p.displacement_scale == 0.5  # m_displacement_scale (main.cpp:1632)
 
 
 def test_struct_layout_matches_header():
     # the ctypes mirror of ocn_params / ocn_fields must match the C layout
-    assert ctypes.sizeof(pyocean.OcnParams) == 6 * 4 + 8 + 8 * 8
+    assert ctypes.sizeof(pyocean.OcnParams) == 6 * 4 + 8 + 9 * 8
     assert pyocean.OcnParams.seed.offset == 24 and pyocean.OcnParams.L.offset == 32
-    assert ctypes.sizeof(pyocean.OcnFields) == 5 * 8
+    assert pyocean.OcnParams.displacement_scale.offset == 96
+    assert ctypes.sizeof(pyocean.OcnFields) == 7 * 8
 
 
 BAD = [
     ("N", 100), ("N", 32), ("N", 8192), ("N", 0), ("L", 0.0), ("L", float("nan")), ("wind_speed", -1.0),
     ("wind_speed", float("inf")), ("amplitude", -0.5), ("suppression", -1.0), ("choppiness", float("nan")),
-    ("gravity", 0.0), ("conj_mode", 2), ("normal_mode", 7),
+    ("gravity", 0.0), ("conj_mode", 2), ("normal_mode", 7), ("displacement_scale", float("inf")),
 ]
 
 

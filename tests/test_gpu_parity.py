@@ -106,6 +106,45 @@ def test_butterfly_fft_buffers_matches_oracle(hip, N):
     assert_field_close("ifft2+inversion", got, ref)
 
 
+@pytest.mark.parametrize("N", [256, 1024])
+def test_butterfly_fft_buffers_matches_rocfft(hip, N):
+    """An independent library pin of the inverse 2-D FFT: hipFFT/rocFFT's
+    unnormalised backward C2C transform of the same input on the GPU, then
+    (-1)^(x+y) Re(.) / N^2 (inversion_cs.glsl:35-41), against this library's
+    butterfly_fft_buffers.  Both are FP32; the tolerance is the oracle's."""
+    import ctypes
+    rt = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libhipfft.so"))
+    rt.hipfftPlan2d.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    rt.hipfftExecC2C.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    rt.hipfftDestroy.argtypes = [ctypes.c_void_p]
+    HIPFFT_C2C, HIPFFT_BACKWARD = 0x29, 1
+    rng = np.random.default_rng(1000 + N)
+    x = (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))).astype(np.complex64)
+    lp, _ = both_params(N=N)
+    _say("butterfly_fft vs hipFFT N=%d" % N)
+    with pyocean.Ocean(lp) as oc:
+        src = hip.malloc(x.nbytes)
+        lib_out = hip.malloc(N * N * 4)
+        fft_out = hip.malloc(x.nbytes)
+        plan = ctypes.c_void_p()
+        try:
+            hip.upload(src, x)
+            oc.butterfly_fft_buffers(src, lib_out)
+            assert rt.hipfftPlan2d(ctypes.byref(plan), N, N, HIPFFT_C2C) == 0
+            assert rt.hipfftExecC2C(plan, src, fft_out, HIPFFT_BACKWARD) == 0
+            hip.sync()
+            got = hip.download(lib_out, (N, N))
+            y = hip.download(fft_out, (N, N), np.complex64)
+        finally:
+            if plan.value:
+                rt.hipfftDestroy(plan)
+            for q in (src, lib_out, fft_out):
+                hip.free(q)
+    sign = np.where((np.add.outer(np.arange(N), np.arange(N)) & 1) == 1, -1.0, 1.0)
+    ref = sign * y.real.astype(np.float64) / (N * N)
+    assert_field_close("ifft2+inversion vs hipFFT", got, ref)
+
+
 # ---------------------------------------------------------------- A7 normals (+ Jacobian)
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -349,3 +388,53 @@ def test_cpp_host_api_batch_and_fit(hip):
     r = subprocess.run([exe, "--batch-fit"], capture_output=True, text=True, timeout=120)
     _say(r.stdout.strip())
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+# ---------------------------------------------------------------- render export (no reference counterpart)
+
+@pytest.mark.parametrize("rows_variant", [3, 1], ids=["fused_rows", "normal_map_kernel"])
+def test_render_export_matches_oracle(hip, rows_variant):
+    """displacement (-l Dx, s h, -l Dz, J) and the displaced-surface normals from
+    the fused row kernel and from the tiled normal-map kernel: against the
+    oracle on the GPU's own height/Dx/Dz, against the oracle's whole frame, and
+    the other outputs unchanged by the export (bit-identical)."""
+    N, F = 256, 3
+    kw = dict(c2_kwargs(), N=N, L=500.0, displacement_scale=0.7)
+    lp, op = both_params(**{k: v for k, v in kw.items() if k != "displacement_scale"})
+    lp.displacement_scale = 0.7
+    times = 0.5 + np.arange(F) / 60.0
+    nn = N * N
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4, "displacement": 16,
+             "render_normals": 16}
+    _say("render export rows_variant=%d" % rows_variant)
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.set_schedule(9, rows_variant)
+        bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
+        try:
+            def run(render):
+                fl = pyocean.OcnFields(*[bufs[k] for k in ("height", "dx", "dz", "normals", "jacobian")],
+                                       bufs["displacement"] if render else None,
+                                       bufs["render_normals"] if render else None)
+                oc.update_frames(times, fl)
+                hip.sync()
+                return {k: hip.download(v, (F, N, N, sizes[k] // 4) if sizes[k] == 16 else (F, N, N))
+                        for k, v in bufs.items()}
+            plain = run(False)
+            got = run(True)
+        finally:
+            for v in bufs.values():
+                hip.free(v)
+    for k in ("height", "dx", "dz", "normals", "jacobian"):
+        np.testing.assert_array_equal(got[k], plain[k])
+    h0 = oracle.h0(op)
+    for f, t in enumerate(times):
+        d_own, n_own = oracle.render(N, op.L, op.choppiness, 0.7, got["height"][f], got["dx"][f], got["dz"][f])
+        assert_field_close("displacement (own fields)", got["displacement"][f], d_own, rel_tol=1e-6,
+                           abs_rel_tol=1e-5)
+        assert_field_close("render normals (own fields)", got["render_normals"][f], n_own, rel_tol=1e-6,
+                           abs_rel_tol=1e-5)
+        ref = oracle.frame(op, h0, t, nthreads=NTHREADS, want=("height", "dx", "dz"))
+        d_ref, n_ref = oracle.render(N, op.L, op.choppiness, 0.7, ref["height"], ref["dx"], ref["dz"])
+        assert_field_close("displacement", got["displacement"][f], d_ref)
+        assert_field_close("render normals", got["render_normals"][f], n_ref)

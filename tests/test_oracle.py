@@ -332,3 +332,52 @@ def test_frame_real_part_equals_hermitian_fold():
     A = 0.5 * (hy + np.conj(mirror))
     assert np.abs(np.imag(np.fft.ifft2(A))).max() < 1e-15 * np.abs(A).max()
     assert np.allclose(np.real(np.fft.ifft2(A)), np.real(np.fft.ifft2(hy)), atol=1e-15)
+
+
+# ---------------------------------------------------------------- render export (no reference counterpart)
+
+def test_render_flat_surface():
+    N = 16
+    z = np.zeros((N, N))
+    disp, rn = oracle.render(N, 100.0, 0.8, 0.5, z, z, z)
+    assert np.all(disp[..., :3] == 0.0) and np.all(disp[..., 3] == 1.0)
+    assert np.allclose(rn, np.broadcast_to([0.0, 1.0, 0.0, 1.0], rn.shape))
+
+
+def test_render_height_sinusoid_closed_form():
+    """h = a cos(2 pi m x / N): the central difference is exactly
+    -a sin(2 pi m x/N) sin(2 pi m/N) N/L, and with lambda = 0 the normal is
+    (-s dh/dx, 1, 0)/|.|."""
+    N, L, a, m, sc = 32, 64.0, 1.5, 3, 0.7
+    x = np.arange(N)
+    h = np.tile(a * np.cos(2 * np.pi * m * x / N), (N, 1))
+    z = np.zeros((N, N))
+    disp, rn = oracle.render(N, L, 0.0, sc, h, z, z)
+    dhdx = -a * np.sin(2 * np.pi * m * x / N) * np.sin(2 * np.pi * m / N) * N / L
+    n = np.stack([-sc * dhdx, np.ones(N), np.zeros(N)], axis=-1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    assert np.allclose(rn[0, :, :3], n, atol=1e-14) and np.allclose(rn[5, :, :3], n, atol=1e-14)
+    assert np.allclose(disp[..., 1], sc * h) and np.all(disp[..., 3] == 1.0)
+
+
+def test_render_normal_is_orthogonal_to_the_displaced_tangents():
+    """For random fields the normal is orthogonal to dP/du and dP/dv built from
+    the same central differences, and |n| * n_y is the Jacobian (disp.w) up to
+    the normalisation: n_y / |(n_x, n_y, n_z)| * |n| = J."""
+    N, L, lam, sc = 16, 40.0, 1.3, 0.6
+    rng = np.random.default_rng(5)
+    h, dx, dz = (rng.standard_normal((N, N)) for _ in range(3))
+    disp, rn = oracle.render(N, L, lam, sc, h, dx, dz)
+    inv2d = N / (2 * L)
+    d_u = lambda f: (np.roll(f, -1, axis=1) - np.roll(f, 1, axis=1)) * inv2d
+    d_v = lambda f: (np.roll(f, -1, axis=0) - np.roll(f, 1, axis=0)) * inv2d
+    tu = np.stack([1 - lam * d_u(dx), sc * d_u(h), -lam * d_u(dz)], axis=-1)
+    tv = np.stack([-lam * d_v(dx), sc * d_v(h), 1 - lam * d_v(dz)], axis=-1)
+    n = rn[..., :3]
+    assert np.abs((n * tu).sum(-1)).max() < 1e-12 and np.abs((n * tv).sum(-1)).max() < 1e-12
+    cross = np.cross(tv, tu)
+    assert np.allclose(cross[..., 1], disp[..., 3])                 # n_y of dP/dv x dP/du is J
+    assert np.allclose(n, cross / np.linalg.norm(cross, axis=-1, keepdims=True))
+    assert np.allclose(disp[..., 0], -lam * dx) and np.allclose(disp[..., 2], -lam * dz)
+    # J agrees with the Jacobian function
+    assert np.allclose(disp[..., 3], oracle.jacobian(N, L, lam, dx, dz))
