@@ -10,8 +10,8 @@
 namespace ocn {
 
 // Block shapes shared by every N in a batch.
-constexpr int kBatchColsThreads = 384;   // 3 * C * (N/16) with C = kBatchColsWidth / N
-constexpr int kBatchColsWidth   = 2048;  // C * N: columns per column block times N
+constexpr int kBatchColsThreads = 768;   // 3 * C * (N/16) with C = kBatchColsWidth / N
+constexpr int kBatchColsWidth   = 4096;  // C * N: columns per column block times N (N = 64: C = N/2)
 constexpr int kBatchRowsThreads = 256;   // (256 / T) groups of T = N/8 threads
 constexpr int kBatchMinN        = 64;
 constexpr int kBatchMaxN        = 2048;
@@ -40,7 +40,11 @@ struct BatchMember {
 
 // Work items: int2 (member, strip) for the column pass, (member, first band)
 // for the row pass (each row block covers 256 / T consecutive bands).
-inline int batch_cols_items_per_member(int N) { return (N / 2) / (kBatchColsWidth / N) + 1; }
+__host__ __device__ constexpr int batch_cols_per_block(int N)
+{
+    return kBatchColsWidth / N < N / 2 ? kBatchColsWidth / N : N / 2;
+}
+inline int batch_cols_items_per_member(int N) { return (N / 2) / batch_cols_per_block(N) + 1; }
 inline int batch_rows_items_per_member(int N)
 {
     const int T = N / batch_rows_radix(N), gpb = kBatchRowsThreads / T, bands = N / batch_rows_band(N);

@@ -4,9 +4,10 @@
 //
 // A batch member is one ocean cascade (its own N, parameters, h0, intermediate
 // and outputs).  A frame is two launches over device tables of work items:
-//   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, 2048/N>, 384 threads
+//   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, C>, 768 threads,
+//                 C = 4096/N columns (N/2 at N = 64, whose strips use 384 threads)
 //   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8, R>,     256 threads
-// Both block shapes are the same for every N (C = 2048/N columns per column
+// Both block shapes are the same for every N (C = 4096/N columns per column
 // block, 256/T band groups per row block), so members of different N share a
 // launch; the kernels dispatch on the member's N.  Blocks are independent (no
 // inter-block waits).  Items are ordered member-major, and the XCD remap keeps
@@ -23,15 +24,15 @@ namespace ocn {
 
 namespace {
 
-constexpr int kColsLds = 3 * 2176;  // 3 * C * lds_len(N) with C*N = 2048
+constexpr int kColsLds = 3 * (kBatchColsWidth + kBatchColsWidth / 16);  // 3 * C * lds_len(N), C*N <= width
 constexpr int kRowsLds = 2176;      // (256 / T) * lds_len(N) with T = N/8
 
 template <int N>
 __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip, double t_rev, float2* lds)
 {
-    constexpr int C = kBatchColsWidth / N;
+    constexpr int C = batch_cols_per_block(N);
     using Sync = typename std::conditional<(N / 16 <= 64), WaveSync, BlockSync>::type;
-    cols_lds_strip<N, C, Sync>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
+    cols_lds_strip<N, C, Sync, kBatchColsThreads>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
 }
 
 template <int N>

@@ -377,6 +377,24 @@ int upload_vector(T** dst, const std::vector<T>& v, const char* what)
     return e == hipSuccess ? OCN_OK : hip_fail(e, what);
 }
 
+// Order of work items for a launch whose blocks are spread over the 8 XCDs in
+// contiguous ranges (xcd_logical): items sorted by decreasing cost and dealt
+// round-robin, so every XCD's range gets a like share of the work and runs its
+// largest items first; the last blocks of the launch are then the smallest.
+template <class Cost>
+std::vector<int2> balance_items(const std::vector<int2>& items, Cost cost)
+{
+    constexpr int kXcds = 8;
+    std::vector<int2> sorted(items);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int2 a, int2 b) { return cost(a) > cost(b); });
+    std::vector<std::vector<int2>> bins(kXcds);
+    for (size_t i = 0; i < sorted.size(); ++i) bins[i % kXcds].push_back(sorted[i]);
+    std::vector<int2> out;
+    out.reserve(items.size());
+    for (const auto& bin : bins) out.insert(out.end(), bin.begin(), bin.end());
+    return out;
+}
+
 int check_batch(const ocn_batch* b)
 {
     if (!b) return fail(OCN_EINVAL, "batch is NULL");
@@ -913,6 +931,11 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
             const int gpb = ocn::batch_rows_groups_per_block(p.N);
             for (int k = 0; k < ocn::batch_rows_items_per_member(p.N); ++k) rows.push_back(make_int2(i, k * gpb));
         }
+        rows = balance_items(rows, [&](int2 it) {
+            const int N = b->params[it.x].N, R = ocn::batch_rows_band(N);
+            const int bands = std::min(ocn::batch_rows_groups_per_block(N), N / R - it.y);
+            return (double)bands * (R + 2) * N;  // rows transformed by the item
+        });
         b->n_cols = (int)cols.size();
         b->n_rows = (int)rows.size();
         if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) &&

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fft_core.hpp"
 #include "ocean_kernels.h"
 
@@ -203,17 +205,22 @@ __host__ __device__ constexpr int strip_group() { return C >= 16 ? 1 : 16 / C; }
 // WaveSync when a group lies in one wave, else BlockSync) and leaves its
 // result in natural order in the same buffer.  Phase C: every thread stores
 // Z1 = G_h + i G_dx (columns tx and N-tx) and Z2 = G_dz(2p) + i G_dz(2p+1).
-template <int N, int C, class Sync>
+template <int N, int C, class Sync, int NB = 3 * C * (N / 16)>
 __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
                                                float2* __restrict__ z2, const float2* __restrict__ tw,
                                                const SpecConst& sc, double t_rev, int strip, float2* lds)
 {
+    // NB = threads of the block; threads from NT on (a larger block shared with
+    // other strip shapes) only take part in the barriers
     constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
     constexpr int ITERS = (C * N + NT - 1) / NT;
+    static_assert(NB >= NT, "block too small for the strip");
+    static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
+    const bool active = NB == NT || (int)threadIdx.x < NT;
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NT;
-        if (k < C * N) {
+        if (active && k < C * N) {
             const int c   = k % C, ty = k / C;
             const int txc = min(strip * C + c, N / 2);
             const int mxc = (N - txc) & (N - 1);
@@ -227,7 +234,7 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
         }
     }
     __syncthreads();
-    {
+    if (active) {
         const int g = threadIdx.x / T, j = threadIdx.x % T;
         float2* buf = lds + g * LEN;
         float2  v[E];
@@ -242,7 +249,7 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NT;
-        if (k < C * N) {
+        if (active && k < C * N) {
             const int c = k % C, y = k / C;
             const int tx = strip * C + c;
             if (tx <= N / 2) {
