@@ -55,8 +55,11 @@ __host__ __device__ constexpr int twiddle_offset_rec(int N, int E, int ns_cur, i
 }
 
 // LDS length (float2) of one transform's exchange buffer: one pad slot per 16
-// entries spreads the strided Stockham writes over the 64 LDS banks.
-__host__ __device__ constexpr int lds_len(int N) { return N + N / 16; }
+// entries spreads the strided Stockham writes over the LDS banks, and the odd
+// length makes neighbouring buffers of a block start in different banks (with
+// a length of a multiple of 32 float2, the same element of every buffer sits
+// in one bank, and threads that walk buffers side by side conflict).
+__host__ __device__ constexpr int lds_len(int N) { return N + N / 16 + 1; }
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
 // ------------------------------------------------------------------ complex helpers

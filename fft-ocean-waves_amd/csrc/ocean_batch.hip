@@ -24,8 +24,27 @@ namespace ocn {
 
 namespace {
 
-constexpr int kColsLds = 3 * (kBatchColsWidth + kBatchColsWidth / 16);  // 3 * C * lds_len(N), C*N <= width
-constexpr int kRowsLds = 2176;      // (256 / T) * lds_len(N) with T = N/8
+// LDS of one block: the largest need over the batched sizes N = 64..2048
+constexpr int cols_lds_need()
+{
+    int m = 0;
+    for (int N = kBatchMinN; N <= kBatchMaxN; N *= 2) {
+        const int need = 3 * batch_cols_per_block(N) * lds_len(N);
+        m = m > need ? m : need;
+    }
+    return m;
+}
+constexpr int rows_lds_need()
+{
+    int m = 0;
+    for (int N = kBatchMinN; N <= kBatchMaxN; N *= 2) {
+        const int need = kBatchRowsThreads / (N / batch_rows_radix(N)) * lds_len(N);
+        m = m > need ? m : need;
+    }
+    return m;
+}
+constexpr int kColsLds = cols_lds_need();  // 3 * C * lds_len(N)
+constexpr int kRowsLds = rows_lds_need();  // (256 / T) * lds_len(N), T = N/8
 
 template <int N>
 __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip, double t_rev, float2* lds)

@@ -37,8 +37,27 @@ namespace ocn {
 
 namespace {
 
-constexpr int kColsLds = 4 * (kFitColsWidth + kFitColsWidth / 16);  // 2 pairs * 2C * lds_len(N)
-constexpr int kRowsLds = 2176;                                       // (256 / T) * lds_len(N), T = N/8
+// LDS of one block: the largest need over the fitted sizes N = 64..1024
+constexpr int cols_lds_need()
+{
+    int m = 0;
+    for (int N = kFitMinN; N <= kFitMaxN; N *= 2) {
+        const int need = 4 * fit_cols_per_block(N) * lds_len(N);  // 2 pairs x 2C transforms
+        m = m > need ? m : need;
+    }
+    return m;
+}
+constexpr int rows_lds_need()
+{
+    int m = 0;
+    for (int N = kFitMinN; N <= kFitMaxN; N *= 2) {
+        const int need = kFitRowsThreads / (N / 8) * lds_len(N);
+        m = m > need ? m : need;
+    }
+    return m;
+}
+constexpr int kColsLds = cols_lds_need();
+constexpr int kRowsLds = rows_lds_need();
 
 // Spectrum constants of the member at its current variables.
 __device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars& v)
@@ -72,15 +91,30 @@ __device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, flo
 
 // dL/dh0 += the pull-back of the frame spectrum gradient G through
 // H = h0k e^{i phi} + C(h0mk) e^{-i phi}: ga += e^{-i phi} G, gb += e^{+i phi} G
-// (identity C) or e^{-i phi} conj(G) (conjugation).  w = omega_fp64 of the texel.
-__device__ __forceinline__ void pull_back(const SpecConst& sc, float2 G, double w, double t_rev, float2& ga, float2& gb)
+// (identity C) or e^{-i phi} conj(G) (conjugation), w = omega_fp64 of the texel;
+// done for a texel u (ga, gb) and its mirror -u (gam, gbm), which share the
+// phase and whose gradients are conjugate: G(-u) = conj G(u).
+__device__ __forceinline__ void pull_back_pair(const SpecConst& sc, float2 G, double w, double t_rev, float2& ga,
+                                               float2& gb, float2& gam, float2& gbm)
 {
     float s, c;
     sincos_turns(turns(w, t_rev), s, c);
-    ga.x += c * G.x + s * G.y;
-    ga.y += c * G.y - s * G.x;
-    gb.x += c * G.x - s * G.y;
-    gb.y += sc.conj_mode ? -(s * G.x + c * G.y) : (s * G.x + c * G.y);
+    const float cx = c * G.x, cy = c * G.y, sx = s * G.x, sy = s * G.y;
+    ga.x += cx + sy;  // e^{-i phi} G
+    ga.y += cy - sx;
+    gam.x += cx - sy;  // e^{-i phi} conj G
+    gam.y -= cy + sx;
+    if (sc.conj_mode) {
+        gb.x += cx - sy;  // e^{-i phi} conj G
+        gb.y -= sx + cy;
+        gbm.x += cx + sy;  // e^{-i phi} G
+        gbm.y += cy - sx;
+    } else {
+        gb.x += cx - sy;  // e^{+i phi} G
+        gb.y += sx + cy;
+        gbm.x += cx + sy;  // e^{+i phi} conj G
+        gbm.y += sx - cy;
+    }
 }
 
 template <class T>
@@ -229,30 +263,57 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
 
 // ---------------------------------------------------------------- adjoint columns + parameter gradient
 
+// One texel's share of the parameter gradient: dL/ds * ds/d(V, theta, A)
+// with s = sqrt(P/2) unclamped; nothing where the clamp binds or s = 0.
+// s = c |d| with d = k_hat . w and c free of d, so ds/dtheta = c sgn(d) dd/dtheta
+// needs no division by d (a d recomputed apart from s could round to 0 where s
+// does not).
+__device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* __restrict__ xi_tex, int N, int tx,
+                                            int ty, float2 ga, float2 gb, float dv_scale, float da_scale, float& dV,
+                                            float& dT, float& dA)
+{
+    const float kx = sc.kscale * (float)(tx - N / 2), ky = sc.kscale * (float)(ty - N / 2);
+    const float kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
+    const float k2   = kmag * kmag;
+    const float d    = (kx * sc.wx + ky * sc.wy) / kmag;
+    const float dp   = (ky * sc.wx - kx * sc.wy) / kmag;
+    const float c    = sqrtf(sc.A * expf(-1.0f / (k2 * sc.Lp2)) * expf(-k2 * sc.l2) / (k2 * k2)) * 0.70710678118654752f;
+    const float s    = c * fabsf(d);
+    if (!(s > 0.0f) || s > 4000.0f) return;
+    const float4 xi   = xi_tex[(size_t)ty * N + tx];
+    const float  dLds = ga.x * xi.x + ga.y * xi.y + gb.x * xi.z + gb.y * xi.w;
+    const float  w    = dLds * s;
+    dV = fmaf(w, dv_scale / k2, dV);
+    dT = fmaf(dLds * c, d < 0.0f ? -dp : dp, dT);  // c sgn(d) dd/dtheta
+    dA = fmaf(w, da_scale, dA);
+}
+
+// Each thread owns texel pairs (u, -u): u in strip column tx = strip*C + c
+// (tx <= N/2) at row ty, -u in column N - tx at row N - ty (mod N).  The two
+// share |k| and so the phase, and G_k(-u) = conj G_k(u) (h_k is real), so
+// one phase and one sincos per frame serve both.  Self-mirror columns (tx = 0,
+// N/2) map -u into the same column, where it is a u of another row: there
+// only u is accumulated.
 template <int N>
 __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, float2* lds, float* red)
 {
     constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
-    constexpr int IT = 2 * C * N / NT;  // texels per thread
     constexpr int S  = 2 * C;           // transforms per frame pair: the strip's columns and their mirrors
-    static_assert(2 * S * T == NT && (2 * C * N) % NT == 0 && T <= 64, "column block shape");
+    constexpr int IT = C * N / NT;      // texel pairs per thread
+    static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
     const FitVars   vars = *m.vars;
     const SpecConst sc = fit_spec(m, vars);
     const size_t    nn = (size_t)N * N;
 
-    // texel it of this thread: slot kk % 2C (C strip columns, then their mirrors), row kk / 2C
-    auto column = [&](int slot) {
-        const int txc = strip * C + slot % C;
-        return slot < C ? txc : (N - txc) & (N - 1);
-    };
-    float2 ga[IT], gb[IT];
+    float2 ga[IT], gb[IT], gam[IT], gbm[IT];  // u, then -u
     double w[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        ga[it] = gb[it] = make_float2(0.0f, 0.0f);
-        const int kk = threadIdx.x + it * NT;
-        const int slot = kk % S, ty = kk / S;
-        const int nx = column(slot) - N / 2, ny = ty - N / 2;
+        ga[it] = gb[it] = gam[it] = gbm[it] = make_float2(0.0f, 0.0f);
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, ty = k / C;
+        const int tx = min(strip * C + c, N / 2);
+        const int nx = tx - N / 2, ny = ty - N / 2;
         float kf;
         w[it] = omega_fp64(sc, nx * nx + ny * ny, kf);
     }
@@ -263,12 +324,15 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         for (int q = 0; q < 2; ++q) {
             if (q >= np) break;
             const float2* yp = m.y + (size_t)(p0 + q) * nn;
+            // load the strip's columns (slots 0..C-1) and their mirrors (slots C..2C-1), conjugated
+            constexpr int ITL = 2 * C * N / NT;
 #pragma unroll
-            for (int it = 0; it < IT; ++it) {
+            for (int it = 0; it < ITL; ++it) {
                 const int kk = threadIdx.x + it * NT;
                 const int slot = kk % S, ty = kk / S;
                 const int txc = strip * C + slot % C;
-                const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + zpos<N>(column(slot))] : make_float2(0.0f, 0.0f);
+                const int col = slot < C ? txc : (N - txc) & (N - 1);
+                const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + zpos<N>(col)] : make_float2(0.0f, 0.0f);
                 lds[(q * S + slot) * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
             }
         }
@@ -293,16 +357,15 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
             const double tr1 = m.t_rev[k1], tr2 = m.t_rev[two ? k2 : k1];
 #pragma unroll
             for (int it = 0; it < IT; ++it) {
-                const int kk = threadIdx.x + it * NT;
-                const int slot = kk % S, ty = kk / S;
-                const int mslot = slot < C ? slot + C : slot - C, my = (N - ty) & (N - 1);
+                const int k = threadIdx.x + it * NT;
+                const int c = k % C, ty = k / C, my = (N - ty) & (N - 1);
                 // F = conj of the stored transform; G1 = (F(u) + conj F(-u))/2, G2 = (F(u) - conj F(-u))/(2i)
-                const float2 fu = lds[(q * S + slot) * LEN + lds_pad(ty)];
-                const float2 fm = lds[(q * S + mslot) * LEN + lds_pad(my)];
+                const float2 fu = lds[(q * S + c) * LEN + lds_pad(ty)];
+                const float2 fm = lds[(q * S + C + c) * LEN + lds_pad(my)];
                 const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
                 const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
-                pull_back(sc, G1, w[it], tr1, ga[it], gb[it]);
-                if (two) pull_back(sc, G2, w[it], tr2, ga[it], gb[it]);
+                pull_back_pair(sc, G1, w[it], tr1, ga[it], gb[it], gam[it], gbm[it]);
+                if (two) pull_back_pair(sc, G2, w[it], tr2, ga[it], gb[it], gam[it], gbm[it]);
             }
         }
         __syncthreads();
@@ -314,25 +377,16 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
     float dV = 0.0f, dT = 0.0f, dA = 0.0f;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-        const int kk = threadIdx.x + it * NT;
-        const int slot = kk % S, ty = kk / S;
-        const int txc = strip * C + slot % C;
-        if (txc > N / 2 || (slot >= C && (txc == 0 || txc == N / 2))) continue;  // not this block's texel
-        const int    tx = column(slot);
-        const size_t u  = (size_t)ty * N + tx;
-        m.grad_h0[u] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
-        const float s = phillips_amp_raw(sc, tx, ty);
-        if (!(s > 0.0f) || s > 4000.0f) continue;  // no amplitude, or the clamp binds
-        const float4 xi   = m.xi[u];
-        const float  dLds = ga[it].x * xi.x + ga[it].y * xi.y + gb[it].x * xi.z + gb[it].y * xi.w;
-        const float  kx = sc.kscale * (float)(tx - N / 2), ky = sc.kscale * (float)(ty - N / 2);
-        const float  kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
-        const float  d    = (kx * sc.wx + ky * sc.wy) / kmag;
-        const float  dp   = (ky * sc.wx - kx * sc.wy) / kmag;
-        const float  wgt  = dLds * s;
-        dV = fmaf(wgt, dv_scale / (kmag * kmag), dV);
-        dT = fmaf(wgt, dp / d, dT);
-        dA = fmaf(wgt, da_scale, dA);
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, ty = k / C;
+        const int tx = strip * C + c;
+        if (tx > N / 2) continue;  // past the half plane (the last strip)
+        m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
+        param_terms(sc, m.xi, N, tx, ty, ga[it], gb[it], dv_scale, da_scale, dV, dT, dA);
+        if (tx == 0 || tx == N / 2) continue;  // -u lies in the same column: a u of its own
+        const int mx = N - tx, my = (N - ty) & (N - 1);
+        m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
+        param_terms(sc, m.xi, N, mx, my, gam[it], gbm[it], dv_scale, da_scale, dV, dT, dA);
     }
     dV = block_sum(dV, red);
     dT = block_sum(dT, red);
