@@ -267,7 +267,7 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
 // with s = sqrt(P/2) unclamped; nothing where the clamp binds or s = 0.
 // s = c |d| with d = k_hat . w and c free of d, so ds/dtheta = c sgn(d) dd/dtheta
 // needs no division by d (a d recomputed apart from s could round to 0 where s
-// does not).
+// does not).  At |d| < 1e-6 the angle term is dropped, as in the oracle.
 __device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* __restrict__ xi_tex, int N, int tx,
                                             int ty, float2 ga, float2 gb, float dv_scale, float da_scale, float& dV,
                                             float& dT, float& dA)
@@ -284,7 +284,9 @@ __device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* _
     const float  dLds = ga.x * xi.x + ga.y * xi.y + gb.x * xi.z + gb.y * xi.w;
     const float  w    = dLds * s;
     dV = fmaf(w, dv_scale / k2, dV);
-    dT = fmaf(dLds * c, d < 0.0f ? -dp : dp, dT);  // c sgn(d) dd/dtheta
+    // c sgn(d) dd/dtheta; |d| < 1e-6 (k perpendicular to the wind, up to rounding)
+    // is the kink of |d|, where the symmetric derivative has no term
+    if (fabsf(d) >= 1e-6f) dT = fmaf(dLds * c, d < 0.0f ? -dp : dp, dT);
     dA = fmaf(w, da_scale, dA);
 }
 

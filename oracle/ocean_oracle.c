@@ -554,7 +554,9 @@ static void texel_xi(const orc_params* p, int tx, int ty, double xi[4])
  * h0 = s(k; V, theta, A) xi (orc_h0), wind direction (cos theta, sin theta)
  * with theta = atan2(wind_dir_y, wind_dir_x).  With s = sqrt(P/2) unclamped:
  *   ds/dA = s/(2A),  ds/dV = s 2 g^2/(|k|^2 V^5),  ds/dtheta = s (k_hat.w_perp)/(k_hat.w),
- * w_perp = (-sin theta, cos theta); zero where the clamp at 4000 binds or s = 0.
+ * w_perp = (-sin theta, cos theta); zero where the clamp at 4000 binds or s = 0,
+ * and ds/dtheta zero where |k_hat.w| < 1e-6 (the symmetric derivative at the
+ * kink of |k_hat.w|).
  * dL/ds = Re(conj(dL/dh0k) s xi01 / s) + Re(conj(dL/dh0mk) xi23). */
 int orc_fit_param_grad(const orc_params* p, const double* grad_h0, double grad[3])
 {
@@ -581,7 +583,11 @@ int orc_fit_param_grad(const orc_params* p, const double* grad_h0, double grad[3
             const double dLds = gg[0] * xi[0] + gg[1] * xi[1] + gg[2] * xi[2] + gg[3] * xi[3];
             const double dperp = (-kx * wy + ky * wx) / kmag;
             dV += dLds * s * 2.0 * g * g / (k2 * V * V * V * V * V);
-            dT += dLds * s * dperp / d;
+            /* s is proportional to |d|: where k is perpendicular to the wind
+             * (|d| below 1e-6, rounding residue included) the loss has a kink in
+             * theta and the symmetric derivative, the one central differences
+             * measure, has no term from this texel. */
+            if (fabs(d) >= 1e-6) dT += dLds * s * dperp / d;
             dA += dLds * s / (2.0 * A);
         }
     grad[0] = dV;

@@ -128,3 +128,25 @@ def test_amplitude_gradient_closed_form():
     K, N = len(times), p.N
     ref = ((h - tg) * h).sum() / (K * N * N) / p.amplitude
     assert abs(grad[2] - ref) <= 1e-10 * abs(ref)
+
+
+@pytest.mark.parametrize("theta_deg", [45.0, 90.0, 135.0])
+def test_angle_gradient_at_lattice_kinks_matches_central_differences(theta_deg):
+    """At 45/90/135 degrees whole lattice lines are perpendicular to the wind,
+    where s = c |k_hat . w| has a kink in theta.  Central differences measure
+    the symmetric derivative there, which the adjoint must reproduce (those
+    texels contribute no angle term)."""
+    th = math.radians(theta_deg)
+    times = [0.5, 1.0, 1.5]
+    tg = _targets(_params(N=32, V=14.0, theta=math.radians(20.0), A=3.0), times)
+
+    def loss(t_):
+        q = _params(N=32, V=12.0, theta=t_, A=2.5)
+        return oracle.fit_eval(q, oracle.h0(q), times, tg, want_grad=False)
+
+    p = _params(N=32, V=12.0, theta=th, A=2.5)
+    _, gh = oracle.fit_eval(p, oracle.h0(p), times, tg)
+    g = oracle.fit_param_grad(p, gh)[1]
+    h = 1e-6
+    fd = (loss(th + h) - loss(th - h)) / (2 * h)
+    assert abs(g - fd) <= 1e-5 * abs(fd), (g, fd)
