@@ -246,53 +246,6 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
         for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
     }
     __syncthreads();
-    if constexpr (C % 2 == 0) {
-        // Column pairs (tx, tx+1), tx even: one 16-byte store per pair at tx and
-        // one at the pair's mirror position N/2 + tx (mirror order follows tx).
-        constexpr int CP = C / 2, ITP = (CP * N + NT - 1) / NT;
-#pragma unroll
-        for (int it = 0; it < ITP; ++it) {
-            const int k = threadIdx.x + it * NT;
-            if (!(active && k < CP * N)) continue;
-            const int    cp = k % CP, y = k / CP;
-            const int    c0 = 2 * cp, tx = strip * C + c0;
-            const size_t row = (size_t)y * N;
-            if (tx > N / 2) continue;
-            const float2 gy0 = lds[(0 * C + c0) * LEN + lds_pad(y)], gx0 = lds[(1 * C + c0) * LEN + lds_pad(y)];
-            const float2 za = make_float2(gy0.x - gx0.y, gy0.y + gx0.x);   // Z1 at tx
-            const float2 zam = make_float2(gy0.x + gx0.y, gx0.x - gy0.y);  // Z1 at N - tx
-            float2 lo0, hi0;
-            if (y < N / 2) {  // Z2 row y pairs G_dz rows 2y and 2y+1
-                lo0 = lds[(2 * C + c0) * LEN + lds_pad(2 * y)];
-                hi0 = lds[(2 * C + c0) * LEN + lds_pad(2 * y + 1)];
-            }
-            if (tx == N / 2) {  // the Nyquist column alone, its own mirror
-                z1[row + tx] = za;
-                if (y < N / 2) z2[row + tx] = make_float2(lo0.x - hi0.y, lo0.y + hi0.x);
-                continue;
-            }
-            const float2 gy1 = lds[(0 * C + c0 + 1) * LEN + lds_pad(y)], gx1 = lds[(1 * C + c0 + 1) * LEN + lds_pad(y)];
-            const float2 zb = make_float2(gy1.x - gx1.y, gy1.y + gx1.x);
-            const float2 zbm = make_float2(gy1.x + gx1.y, gx1.x - gy1.y);
-            *reinterpret_cast<float4*>(z1 + row + tx) = make_float4(za.x, za.y, zb.x, zb.y);
-            if (tx != 0)
-                *reinterpret_cast<float4*>(z1 + row + N / 2 + tx) = make_float4(zam.x, zam.y, zbm.x, zbm.y);
-            else
-                z1[row + N / 2 + 1] = zbm;  // column 0 is its own mirror
-            if (y < N / 2) {
-                const float2 lo1 = lds[(2 * C + c0 + 1) * LEN + lds_pad(2 * y)];
-                const float2 hi1 = lds[(2 * C + c0 + 1) * LEN + lds_pad(2 * y + 1)];
-                *reinterpret_cast<float4*>(z2 + row + tx) =
-                    make_float4(lo0.x - hi0.y, lo0.y + hi0.x, lo1.x - hi1.y, lo1.y + hi1.x);
-                if (tx != 0)
-                    *reinterpret_cast<float4*>(z2 + row + N / 2 + tx) =
-                        make_float4(lo0.x + hi0.y, hi0.x - lo0.y, lo1.x + hi1.y, hi1.x - lo1.y);
-                else
-                    z2[row + N / 2 + 1] = make_float2(lo1.x + hi1.y, hi1.x - lo1.y);
-            }
-        }
-        return;
-    }
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NT;
