@@ -57,6 +57,7 @@ LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
 MEASURED_TRAFFIC = {
     ("k_rows_nrm", 9, 5): 13.0 + 32.0,
     ("k_cols_lds", 9, 5): 1.7 + 15.2,
+    ("k_rows_nrm", 10, 5): 13.0 + 32.0,
 }
 
 
@@ -389,7 +390,8 @@ def main():
     reps = 40
     sched = oc.schedule(lf)
     fused = sched["rows_band"] > 0  # normal map + Jacobian inside the row kernel
-    stages = [("k_cols_lds" if sched["cols_variant"] >= 7 else "k_cols", pyocean.OCN_STAGE_COLUMNS),
+    cols_name = "k_cols" if sched["cols_variant"] < 7 else "k_cols_lds" if sched["cols_variant"] < 10 else "k_cols_lds2"
+    stages = [(cols_name, pyocean.OCN_STAGE_COLUMNS),
               ("k_rows_nrm" if fused else "k_rows", pyocean.OCN_STAGE_ROWS)]
     if not fused:
         stages.append(("k_normals", pyocean.OCN_STAGE_NORMALS))
@@ -402,7 +404,7 @@ def main():
         hip.record(e1, stream)
         stage_ms[name] = hip.elapsed_ms(e0, e1) / reps
     dominant = max(stage_ms, key=stage_ms.get)
-    dom_bytes = CANON_STAGE[dominant.replace("k_cols_lds", "k_cols")] * nn * lf
+    dom_bytes = CANON_STAGE["k_cols" if dominant.startswith("k_cols") else dominant] * nn * lf
     achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
     tb = MEASURED_TRAFFIC.get((dominant, sched["cols_variant"], sched["rows_variant"]))

@@ -127,7 +127,8 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 // MI355X per N for one frame per launch and for batches of frames; the
 // autotuner may replace them.  Column variants: 0 (16,1) 1 (16,2) 2 (16,4)
 // 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block) of k_cols,
-// 7..9 k_cols_lds with 1, 2, 4 columns per block.  Row variants: 0 radix 16,
+// 7..9 k_cols_lds with 1, 2, 4 columns per block, 10..12 k_cols_lds2 with 2, 4,
+// 8 columns per block.  Row variants: 0 radix 16,
 // 1 radix 8 (each followed by the normal-map kernel), 2..5 the fused
 // row + normals kernel with bands of 4, 8, 16, 32 rows.
 std::pair<int, int> default_schedule(int N, int nframes)
@@ -138,8 +139,8 @@ std::pair<int, int> default_schedule(int N, int nframes)
     case 64: cv = batch ? 5 : 9; rv = 1; break;
     case 128: cv = batch ? 9 : 8; rv = 1; break;
     case 256: cv = batch ? 9 : 7; rv = 1; break;
-    case 512: cv = batch ? 5 : 8; rv = batch ? 2 : 1; break;
-    case 1024: cv = 9; rv = batch ? 5 : 1; break;
+    case 512: cv = batch ? 9 : 8; rv = batch ? 2 : 1; break;
+    case 1024: cv = batch ? 10 : 9; rv = batch ? 5 : 1; break;
     case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 2; break;
     default: cv = 4; rv = batch ? 5 : 0; break;  // 4096: radix-8 rows would need 1536 threads
     }

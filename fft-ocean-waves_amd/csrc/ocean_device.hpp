@@ -270,6 +270,91 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     }
 }
 
+// The same strip in two rounds through half the LDS: block = 2*C*T threads,
+// lds holds 2*C*lds_len(N).  Phase A folds all three spectra; height and Dx go
+// to LDS, each thread keeps its Dz values in registers.  Round 1 transforms the
+// 2C height/Dx columns and stores Z1; round 2 moves Dz into LDS, transforms the
+// C Dz columns (half the block) and stores Z2.  With half the LDS of
+// cols_lds_strip, two blocks fit a CU and one block's transforms overlap the
+// other's loads and stores.
+template <int N, int C>
+__device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
+                                                float2* __restrict__ z2, const float2* __restrict__ tw,
+                                                const SpecConst& sc, double t_rev, int strip, float2* lds)
+{
+    constexpr int E = 16, T = N / E, NT = 2 * C * T, LEN = lds_len(N);
+    constexpr int ITERS = C * N / NT;  // = N / (2T) = 8
+    static_assert(T <= 64 && 64 % T == 0 && (C * N) % NT == 0, "each transform group inside one wave");
+    float2 dzr[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k   = threadIdx.x + it * NT;
+        const int c   = k % C, ty = k / C;
+        const int txc = min(strip * C + c, N / 2);
+        const int mxc = (N - txc) & (N - 1);
+        const int my  = (N - ty) & (N - 1);
+        float2 ay, ax;
+        pair_fold(sc, h0[(size_t)ty * N + txc], h0[(size_t)my * N + mxc], txc - N / 2, ty - N / 2, mxc - N / 2,
+                  my - N / 2, t_rev, ay, ax, dzr[it]);
+        lds[c * LEN + lds_pad(ty)]       = ay;
+        lds[(C + c) * LEN + lds_pad(ty)] = ax;
+    }
+    auto transform = [&](int groups) {
+        if ((int)threadIdx.x < groups * T) {  // whole waves
+            const int g = threadIdx.x / T, j = threadIdx.x % T;
+            float2*   buf = lds + g * LEN;
+            float2    v[E];
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+            fft_inverse<N, E>(v, j, buf, tw, WaveSync{});
+            WaveSync{}();
+#pragma unroll
+            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+        }
+    };
+    __syncthreads();
+    transform(2 * C);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, y = k / C;
+        const int tx = strip * C + c;
+        if (tx <= N / 2) {
+            const int    mxc = (N - tx) & (N - 1);
+            const float2 gy  = lds[c * LEN + lds_pad(y)];
+            const float2 gx  = lds[(C + c) * LEN + lds_pad(y)];
+            const size_t row = (size_t)y * N;
+            z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
+            if (mxc != tx) z1[row + zpos<N>(mxc)] = make_float2(gy.x + gx.y, gx.x - gy.y);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        lds[(k % C) * LEN + lds_pad(k / C)] = dzr[it];
+    }
+    __syncthreads();
+    transform(C);
+    __syncthreads();
+    // Z2 row y pairs G_dz rows 2y and 2y+1, y < N/2
+#pragma unroll
+    for (int it = 0; it < ITERS / 2; ++it) {
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, y = k / C;
+        const int tx = strip * C + c;
+        if (tx <= N / 2) {
+            const int    mxc = (N - tx) & (N - 1);
+            const float2 lo  = lds[c * LEN + lds_pad(2 * y)];
+            const float2 hi  = lds[c * LEN + lds_pad(2 * y + 1)];
+            const size_t row = (size_t)y * N;
+            z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
+            if (mxc != tx) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ row pass + normals + Jacobian: one band
 
 // Row transforms fused with the normal map (nearest taps) and the Jacobian

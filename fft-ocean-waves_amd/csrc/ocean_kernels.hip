@@ -179,6 +179,25 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __r
                                    fb.t_rev[f], strip, lds);
 }
 
+// Two-round variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
+// LDS, same block order.
+template <int N, int C>
+__global__ __launch_bounds__(2 * C * (N / 16)) void k_cols_lds2(const float4* __restrict__ h0,
+                                                                float2* __restrict__ z1, float2* __restrict__ z2,
+                                                                const float2* __restrict__ tw, SpecConst sc,
+                                                                FrameBatch fb)
+{
+    constexpr int G = strip_group<C>();
+    __shared__ float2 lds[2 * C * lds_len(N)];
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int rest  = L / G;
+    const int f     = rest % fb.nframes;
+    const int strip = (rest / fb.nframes) * G + L % G;
+    if (strip > (N / 2) / C) return;  // padding of the last group (whole block)
+    cols_lds2_strip<N, C>(h0, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, tw, sc, fb.t_rev[f],
+                          strip, lds);
+}
+
 // ------------------------------------------------------------------ fused frame, pass 2
 
 // Row pass.  Grid = (N/2, frames).  Block p handles rows 2p and 2p+1: group 0
@@ -398,12 +417,14 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // Column-pass schedules (radix E, columns per block C); the autotuner picks one
 // per N.  Row pass: radix E, three transforms per block.
 // kind 0: k_cols (register-resident, block-synchronised transforms);
-// kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only).
+// kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only);
+// kind 2: k_cols_lds2 (the same in two rounds through half the LDS).
 struct ColsVariant {
     int E, C, kind;
 };
 constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
-                                                         {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1}};
+                                                         {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1},
+                                                         {16, 2, 2}, {16, 4, 2}, {16, 8, 2}};
 // Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
 // band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused.
 struct RowsVariant {
@@ -416,9 +437,12 @@ constexpr bool cols_fits()
 {
     if constexpr (K == 0)
         return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
-    else
+    else if constexpr (K == 1)
         return E == 16 && N / 16 <= 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024 &&
                C <= N / 2;
+    else
+        return E == 16 && N >= 64 && N / 16 <= 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024 &&
+               C <= N / 2 && (C * N) % (2 * C * (N / 16)) == 0;
 }
 template <int N, int E, int B>
 constexpr bool rows_fits()
@@ -442,6 +466,13 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
         constexpr int strips = (N / 2) / C + 1;
         constexpr int groups = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols_lds<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
+                           a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        return hipGetLastError();
+    } else if constexpr (K == 2) {
+        constexpr int G      = strip_group<C>();
+        constexpr int strips = (N / 2) / C + 1;
+        constexpr int groups = (strips + G - 1) / G;
+        hipLaunchKernelGGL((k_cols_lds2<N, C>), dim3(groups * G * a.nframes), dim3(2 * C * (N / 16)), 0, s, a.h0,
                            a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else {
@@ -495,6 +526,9 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 7: e = launch_cols_v<N, 7>(a, s); break;
         case 8: e = launch_cols_v<N, 8>(a, s); break;
         case 9: e = launch_cols_v<N, 9>(a, s); break;
+        case 10: e = launch_cols_v<N, 10>(a, s); break;
+        case 11: e = launch_cols_v<N, 11>(a, s); break;
+        case 12: e = launch_cols_v<N, 12>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -531,7 +565,8 @@ static bool variant_ok_n(int cols_v, int rows_v)
     constexpr bool c[kNumColsVariants] = {
 #define OCN_FITS(v) cols_fits<N, kColsVariants[v].E, kColsVariants[v].C, kColsVariants[v].kind>()
         OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
-        OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9)};
+        OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9),
+        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12)};
 #undef OCN_FITS
 #define OCN_RFITS(v) rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band>()
     constexpr bool r[kNumRowsVariants] = {OCN_RFITS(0), OCN_RFITS(1), OCN_RFITS(2),

@@ -301,7 +301,7 @@ def test_every_schedule_variant_matches_oracle(hip, N):
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
         tried = 0
-        for cv in range(10):
+        for cv in range(13):
             for rv in range(6):
                 try:
                     oc.set_schedule(cv, rv)
