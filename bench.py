@@ -58,6 +58,7 @@ MEASURED_TRAFFIC = {
     ("k_rows_nrm", 9, 5): 13.0 + 32.0,
     ("k_cols_lds", 9, 5): 1.7 + 15.2,
     ("k_rows_nrm", 10, 5): 13.0 + 32.0,
+    ("k_cols_lds2", 10, 5): 1.4 + 14.3,
 }
 
 
