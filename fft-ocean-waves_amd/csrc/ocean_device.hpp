@@ -270,11 +270,11 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     }
 }
 
-// The same strip in two rounds through half the LDS: block = 2*C*T threads,
+// The same strip in two passes through half the LDS: block = 2*C*T threads,
 // lds holds 2*C*lds_len(N).  Phase A folds all three spectra; height and Dx go
-// to LDS, each thread keeps its Dz values in registers.  Round 1 transforms the
-// 2C height/Dx columns and stores Z1; round 2 moves Dz into LDS, transforms the
-// C Dz columns (half the block) and stores Z2.  With half the LDS of
+// to LDS, each thread keeps its Dz values in registers.  The first pass
+// transforms the 2C height/Dx columns and stores Z1; the second moves Dz into
+// LDS, transforms the C Dz columns (half the block) and stores Z2.  With half the LDS of
 // cols_lds_strip, two blocks fit a CU and one block's transforms overlap the
 // other's loads and stores.
 template <int N, int C>

@@ -446,8 +446,8 @@ __global__ __launch_bounds__(kFitRowsThreads) void k_fit_rows(const FitMember* _
 #undef OCN_CALL
 }
 
-// Capped at three waves per SIMD (a few registers spill): measured on one box,
-// 0.80 -> 0.68 ms per batched-fit step against the uncapped two waves.
+// Capped at three waves per SIMD (a few registers spill): 0.80 -> 0.68 ms per
+// step of a 256-ocean N=256 fit against the uncapped two waves.
 __global__ __launch_bounds__(kFitColsThreads, 3) void k_fit_cols_bwd(const FitMember* __restrict__ members,
                                                                   const int2* __restrict__ items)
 {
