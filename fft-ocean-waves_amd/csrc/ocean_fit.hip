@@ -133,7 +133,7 @@ __device__ __forceinline__ T block_sum(T x, T* red)
 
 // ---------------------------------------------------------------- forward columns
 
-// Column transforms run two frame pairs per round: 2 pairs x 2C columns of T
+// Column transforms run two frame pairs at a time: 2 pairs x 2C columns of T
 // threads fill the 256-thread block.
 template <int N>
 __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)

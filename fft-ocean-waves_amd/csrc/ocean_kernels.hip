@@ -179,7 +179,7 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __r
                                    fb.t_rev[f], strip, lds);
 }
 
-// Two-round variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
+// Two-pass variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
 // LDS, same block order.
 template <int N, int C>
 __global__ __launch_bounds__(2 * C * (N / 16)) void k_cols_lds2(const float4* __restrict__ h0,
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // per N.  Row pass: radix E, three transforms per block.
 // kind 0: k_cols (register-resident, block-synchronised transforms);
 // kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only);
-// kind 2: k_cols_lds2 (the same in two rounds through half the LDS).
+// kind 2: k_cols_lds2 (the same in two passes through half the LDS).
 struct ColsVariant {
     int E, C, kind;
 };
