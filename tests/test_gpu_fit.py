@@ -65,6 +65,8 @@ CASES = [
     ("n128_k1_textbook", 128, 250.0, [1.5], 1, START),
     ("n256_k4_wind45", 256, 500.0, [0.5, 1.0, 1.5, 2.0], 0, (12.0, math.pi / 4, 3.0)),
     ("n512_k8_wind135_textbook", 512, 1000.0, TIMES8, 1, (25.0, 3 * math.pi / 4, 4.0)),
+    ("n64_k16_max_frames", 64, 100.0, [0.25 * k for k in range(1, 17)], 0, START),
+    ("n128_k15_odd", 128, 250.0, [0.3 * k for k in range(1, 16)], 1, START),
 ]
 
 

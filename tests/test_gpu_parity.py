@@ -242,6 +242,50 @@ def test_update_frames_equals_per_frame_update(hip):
                 np.testing.assert_array_equal(batch[k][f], one[k])
 
 
+def test_updates_captured_in_a_graph_replay_like_eager_launches(hip):
+    """ocn_update_into of three frames recorded by stream capture into one HIP
+    graph (the launches hold kernel nodes only), replayed three times with the
+    program's own eager launches in between; every replay equals the eager
+    frames bit for bit."""
+    kw = dict(c1_kwargs(), N=256)
+    lp, _ = both_params(**kw)
+    N, nn = 256, 256 * 256
+    times = [0.3, 1.7, 4.2]
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    stream = hip.stream()
+    with pyocean.Ocean(lp, stream=stream) as oc:
+        oc.tilde_h0_k()
+        bufs = [{k: hip.malloc(nn * b) for k, b in sizes.items()} for _ in times]
+        fl = [pyocean.OcnFields(*[b[k] for k in sizes]) for b in bufs]
+        down = lambda i: {k: hip.download(v, (N, N, 4) if k == "normals" else (N, N)) for k, v in bufs[i].items()}
+        try:
+            for i, t in enumerate(times):
+                oc.update_into(t, fl[i])
+            hip.stream_sync(stream)
+            eager = [down(i) for i in range(len(times))]
+            for i in range(len(times)):
+                for v in bufs[i].values():
+                    hip.memset(v, 0, nn * 4)
+            g, ex = hip.capture(stream, lambda: [oc.update_into(t, fl[i]) for i, t in enumerate(times)])
+            try:
+                for rep in range(3):
+                    hip.graph_launch(ex, stream)
+                    hip.stream_sync(stream)
+                    for i in range(len(times)):
+                        got = down(i)
+                        for k in sizes:
+                            np.testing.assert_array_equal(got[k], eager[i][k])
+                    oc.update(9.0)  # eager work between replays
+                    hip.stream_sync(stream)
+            finally:
+                hip.graph_destroy(g, ex)
+        finally:
+            for b in bufs:
+                for v in b.values():
+                    hip.free(v)
+    hip.stream_destroy(stream)
+
+
 # ---------------------------------------------------------------- full sizes: properties
 
 def _direct_points(op, h0, t, pts):
