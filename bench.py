@@ -55,10 +55,8 @@ LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
 # profiles/k_rows_nrm_k_cols_lds_n1024_batched_pmc_traffic.md.  Reported only
 # when the schedule in use matches, otherwise null.
 MEASURED_TRAFFIC = {
-    ("k_rows_nrm", 9, 5): 13.0 + 32.0,
-    ("k_cols_lds", 9, 5): 1.7 + 15.2,
-    ("k_rows_nrm", 10, 5): 13.0 + 32.0,
-    ("k_cols_lds2", 10, 5): 1.4 + 14.3,
+    ("k_rows_nrm", 10, 5): 15.1 + 32.3,
+    ("k_cols_lds2", 10, 5): 1.3 + 16.8,
 }
 
 

@@ -60,11 +60,17 @@ __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_
     constexpr int E = batch_rows_radix(N), T = N / E, R = batch_rows_band(N);
     static_assert(T <= kBatchRowsThreads && kBatchRowsThreads % T == 0, "whole groups per row block");
     using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
-    const int g = threadIdx.x / T, j = threadIdx.x % T;
+    // a group of T >= 64 threads is made of whole waves: its index is
+    // wave-uniform, and reading it from the first lane keeps the band scalar
+    const int gl = (int)threadIdx.x / T, j = (int)threadIdx.x % T;
+    const int g  = T >= 64 ? __builtin_amdgcn_readfirstlane(gl) : gl;
     const int band = first_band + g;
     if (band >= N / R) return;  // whole group (wave-aligned when GPB > 1)
-    rows_nrm_band<N, E, R, Sync>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac, E == 8 ? m.tw8 : m.tw16,
-                                 lds + g * lds_len(N), j, band, m.lambda, m.inv2d);
+    // Z2 rows paired too: the walk stays within 256 registers a lane, so two
+    // 256-thread blocks still share a CU
+    rows_nrm_band<N, E, R, Sync, false, true>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
+                                              E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, m.lambda,
+                                              m.inv2d);
 }
 
 }  // namespace

@@ -174,6 +174,29 @@ __device__ __forceinline__ int zpos_elem(int j, int r)
     return j == 0 ? N / 2 : N - j;
 }
 
+// Z1 and Z2 are stored in tiles of two rows: element (row y, position p) sits
+// at float2 index (y/2 * N + p) * 2 + y%2, the two rows of a tile interleaved.
+// A column strip of C columns then writes runs of 2C values (one 16-byte store
+// per column and tile) instead of runs of C per row, which halves the write
+// requests of the column pass, and the row pass reads both rows of a tile with
+// one 16-byte load per position.  Tiles start 16-byte aligned when the buffer
+// does.
+template <int N>
+__device__ __forceinline__ size_t zidx(int y, int p)
+{
+    return ((size_t)(y >> 1) * N + p) * 2 + (y & 1);
+}
+template <int N>
+__device__ __forceinline__ float4* ztile(float2* z, int m)
+{
+    return reinterpret_cast<float4*>(z) + (size_t)m * N;
+}
+template <int N>
+__device__ __forceinline__ const float4* ztile(const float2* z, int m)
+{
+    return reinterpret_cast<const float4*>(z) + (size_t)m * N;
+}
+
 // ------------------------------------------------------------------ XCD-aware block order
 
 // MI355X spreads the workgroups of a launch round-robin over its 8 XCDs
@@ -189,7 +212,8 @@ __device__ __forceinline__ int xcd_logical(int b, int n)
     return x * q + (x < r ? x : r) + b / X;
 }
 
-// Column strips per block group: G strips of C columns fill 128 B of a Z row.
+// Column strips per block group: G strips of C columns fill whole cache lines
+// of a Z tile (16 columns, 256 B of a two-row tile).
 template <int C>
 __host__ __device__ constexpr int strip_group() { return C >= 16 ? 1 : 16 / C; }
 
@@ -204,7 +228,8 @@ __host__ __device__ constexpr int strip_group() { return C >= 16 ? 1 : 16 / C; }
 // Phase B: each of the 3C transforms runs inside one T-thread group (Sync:
 // WaveSync when a group lies in one wave, else BlockSync) and leaves its
 // result in natural order in the same buffer.  Phase C: every thread stores
-// Z1 = G_h + i G_dx (columns tx and N-tx) and Z2 = G_dz(2p) + i G_dz(2p+1).
+// Z1 = G_h + i G_dx (columns tx and N-tx) and Z2 = G_dz(2p) + i G_dz(2p+1),
+// both in two-row tiles (zidx).
 template <int N, int C, class Sync, int NB = 3 * C * (N / 16)>
 __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
                                                float2* __restrict__ z2, const float2* __restrict__ tw,
@@ -246,24 +271,32 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
         for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
     }
     __syncthreads();
+    // thread k stores Z1 tile m = k / C (rows 2m, 2m+1) and, for m < N/4, Z2
+    // tile m (Z2 rows 2m, 2m+1, i.e. G_dz rows 4m..4m+3) of column c = k % C
+    constexpr int ITZ = (C * N / 2 + NT - 1) / NT;
 #pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
+    for (int it = 0; it < ITZ; ++it) {
         const int k = threadIdx.x + it * NT;
-        if (active && k < C * N) {
-            const int c = k % C, y = k / C;
+        if (active && k < C * N / 2) {
+            const int c = k % C, m = k / C;
             const int tx = strip * C + c;
             if (tx <= N / 2) {
                 const int    mxc = (N - tx) & (N - 1);
-                const float2 gy  = lds[(0 * C + c) * LEN + lds_pad(y)];
-                const float2 gx  = lds[(1 * C + c) * LEN + lds_pad(y)];
-                const size_t row = (size_t)y * N;
-                z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
-                if (mxc != tx) z1[row + zpos<N>(mxc)] = make_float2(gy.x + gx.y, gx.x - gy.y);
-                if (y < N / 2) {  // Z2 row y pairs G_dz rows 2y and 2y+1
-                    const float2 lo = lds[(2 * C + c) * LEN + lds_pad(2 * y)];
-                    const float2 hi = lds[(2 * C + c) * LEN + lds_pad(2 * y + 1)];
-                    z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
-                    if (mxc != tx) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
+                const float2 y0  = lds[(0 * C + c) * LEN + lds_pad(2 * m)];
+                const float2 x0  = lds[(1 * C + c) * LEN + lds_pad(2 * m)];
+                const float2 y1  = lds[(0 * C + c) * LEN + lds_pad(2 * m + 1)];
+                const float2 x1  = lds[(1 * C + c) * LEN + lds_pad(2 * m + 1)];
+                float4*      t1  = ztile<N>(z1, m);
+                t1[tx] = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
+                if (mxc != tx) t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
+                if (m < N / 4) {  // Z2 row q pairs G_dz rows 2q and 2q+1
+                    const float2 a = lds[(2 * C + c) * LEN + lds_pad(4 * m)];
+                    const float2 b = lds[(2 * C + c) * LEN + lds_pad(4 * m + 1)];
+                    const float2 d = lds[(2 * C + c) * LEN + lds_pad(4 * m + 2)];
+                    const float2 e = lds[(2 * C + c) * LEN + lds_pad(4 * m + 3)];
+                    float4*      t2 = ztile<N>(z2, m);
+                    t2[tx] = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
+                    if (mxc != tx) t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
                 }
             }
         }
@@ -315,18 +348,21 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
     __syncthreads();
     transform(2 * C);
     __syncthreads();
+    // Z1 tile m = k / C (rows 2m, 2m+1) of column c = k % C
 #pragma unroll
-    for (int it = 0; it < ITERS; ++it) {
+    for (int it = 0; it < ITERS / 2; ++it) {
         const int k = threadIdx.x + it * NT;
-        const int c = k % C, y = k / C;
+        const int c = k % C, m = k / C;
         const int tx = strip * C + c;
         if (tx <= N / 2) {
             const int    mxc = (N - tx) & (N - 1);
-            const float2 gy  = lds[c * LEN + lds_pad(y)];
-            const float2 gx  = lds[(C + c) * LEN + lds_pad(y)];
-            const size_t row = (size_t)y * N;
-            z1[row + tx] = make_float2(gy.x - gx.y, gy.y + gx.x);
-            if (mxc != tx) z1[row + zpos<N>(mxc)] = make_float2(gy.x + gx.y, gx.x - gy.y);
+            const float2 y0  = lds[c * LEN + lds_pad(2 * m)];
+            const float2 x0  = lds[(C + c) * LEN + lds_pad(2 * m)];
+            const float2 y1  = lds[c * LEN + lds_pad(2 * m + 1)];
+            const float2 x1  = lds[(C + c) * LEN + lds_pad(2 * m + 1)];
+            float4*      t1  = ztile<N>(z1, m);
+            t1[tx] = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
+            if (mxc != tx) t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
         }
     }
     __syncthreads();
@@ -338,19 +374,21 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
     __syncthreads();
     transform(C);
     __syncthreads();
-    // Z2 row y pairs G_dz rows 2y and 2y+1, y < N/2
+    // Z2 tile m = k / C, m < N/4: Z2 rows 2m, 2m+1, Z2 row q pairing G_dz rows 2q and 2q+1
 #pragma unroll
-    for (int it = 0; it < ITERS / 2; ++it) {
+    for (int it = 0; it < ITERS / 4; ++it) {
         const int k = threadIdx.x + it * NT;
-        const int c = k % C, y = k / C;
+        const int c = k % C, m = k / C;
         const int tx = strip * C + c;
         if (tx <= N / 2) {
             const int    mxc = (N - tx) & (N - 1);
-            const float2 lo  = lds[c * LEN + lds_pad(2 * y)];
-            const float2 hi  = lds[c * LEN + lds_pad(2 * y + 1)];
-            const size_t row = (size_t)y * N;
-            z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
-            if (mxc != tx) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
+            const float2 a   = lds[c * LEN + lds_pad(4 * m)];
+            const float2 b   = lds[c * LEN + lds_pad(4 * m + 1)];
+            const float2 d   = lds[c * LEN + lds_pad(4 * m + 2)];
+            const float2 e   = lds[c * LEN + lds_pad(4 * m + 3)];
+            float4*      t2  = ztile<N>(z2, m);
+            t2[tx] = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
+            if (mxc != tx) t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
         }
     }
 }
@@ -371,7 +409,7 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
 // normals or jac may be null.  RENDER adds the render export (disp, rnorm,
 // either may be null): (-l Dx, s h, -l Dz, J) and the unit normal of the
 // displaced surface, dP/dv x dP/du.
-template <int N, int E, int R, class Sync, bool RENDER = false>
+template <int N, int E, int R, class Sync, bool RENDER = false, bool PAIR2 = false>
 __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, const float2* __restrict__ z2,
                                               float* __restrict__ height, float* __restrict__ dx,
                                               float* __restrict__ dz, float4* __restrict__ normals,
@@ -385,37 +423,61 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
     constexpr float scale = 1.0f / ((float)N * (float)N);
     const int y0 = band * R;
 
-    // inverse transform of one Z row into v (natural order, thread j holds x = j + r*T)
-    auto fft_row = [&](const float2* src, float2 (&v)[E]) {
+    // Rows are read from the two-row tiles of Z1 / Z2: load_row takes one row
+    // (8 bytes of every 16), load_pair both rows of the tile holding row y
+    // (y even) -- row y into v, row y+1 into w -- with one 16-byte load per
+    // position.  Thread j holds x = j + r*T (natural order after the transform).
+    auto load_row = [&](const float2* z, int y, float2 (&v)[E]) {
 #pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
-        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+        for (int r = 0; r < E; ++r) v[r] = z[zidx<N>(y, zpos_elem<N, E>(j, r))];
     };
+    auto load_pair = [&](const float2* z, int y, float2 (&v)[E], float2 (&w)[E]) {
+        const float4* t = ztile<N>(z, y >> 1);
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const float4 q = t[zpos_elem<N, E>(j, r)];
+            v[r] = make_float2(q.x, q.y);
+            w[r] = make_float2(q.z, q.w);
+        }
+    };
+    auto fft = [&](float2 (&v)[E]) { fft_inverse<N, E>(v, j, buf, tw, Sync{}); };
     // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y
     auto sgn = [&](int y) { return ((j + y) & 1) ? -scale : scale; };
 
+    // The band reads Z1 rows y0-1 .. y0+R and Z2 rows (y0-1)/2 .. (y0+R)/2
+    // (row N wraps to 0), counted unwrapped: an even Z1 row y >= y0 other
+    // than the last is loaded together with row y+1, which waits in p1 for
+    // the next step.  With PAIR2 the Z2 rows b2 .. e2 are paired the same way
+    // (pending row in p2); without it they are read one at a time, which saves
+    // the registers of p2.
+    const int b2 = y0 >> 1, e2 = b2 + R / 2;
     float hA[E], xA[E], zA[E], hB[E], xB[E], zB[E], zN[E];
-    float2 v[E];
+    float2 v[E], p1[E], p2[E];
     {
-        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the imaginary part of pair yA >> 1
-        fft_row(z2 + (size_t)(yA >> 1) * N, v);
+        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the imaginary part of Z2 row yA >> 1
+        load_row(z2, yA >> 1, v);
+        fft(v);
         const float sa = sgn(yA);
 #pragma unroll
         for (int r = 0; r < E; ++r) zA[r] = sa * v[r].y;
-        fft_row(z1 + (size_t)yA * N, v);
+        load_row(z1, yA, v);
+        fft(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             hA[r] = sa * v[r].x;
             xA[r] = sa * v[r].y;
         }
         const float sb = sgn(y0);
-        fft_row(z2 + (size_t)(y0 >> 1) * N, v);
+        if (PAIR2 && (b2 & 1) == 0) load_pair(z2, b2, v, p2);
+        else load_row(z2, b2, v);
+        fft(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             zB[r] = sb * v[r].x;
             zN[r] = -sb * v[r].y;  // Dz of row y0 + 1 (opposite sign)
         }
-        fft_row(z1 + (size_t)y0 * N, v);
+        load_pair(z1, y0, v, p1);  // R >= 2: row y0 + 1 is in the band
+        fft(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             hB[r] = sb * v[r].x;
@@ -426,11 +488,21 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 #pragma unroll 1
     for (int k = 0; k < R; ++k) {
         const int   y  = y0 + k;
-        const int   yC = (y + 1) & (N - 1);
+        const int   yu = y + 1;              // unwrapped, <= N
+        const int   yC = yu & (N - 1);
         const float sc = sgn(yC);
         float zC[E];
         if ((yC & 1) == 0) {
-            fft_row(z2 + (size_t)(yC >> 1) * N, v);
+            const int u = yu >> 1;  // Z2 row, unwrapped
+            if (PAIR2 && (u & 1)) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) v[r] = p2[r];
+            } else if (PAIR2 && u < e2) {
+                load_pair(z2, u, v, p2);
+            } else {
+                load_row(z2, yC >> 1, v);
+            }
+            fft(v);
 #pragma unroll
             for (int r = 0; r < E; ++r) {
                 zC[r] = sc * v[r].x;
@@ -440,7 +512,15 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 #pragma unroll
             for (int r = 0; r < E; ++r) zC[r] = zN[r];
         }
-        fft_row(z1 + (size_t)yC * N, v);  // v = height + i Dx of row yC (unsigned)
+        if (yu & 1) {
+#pragma unroll
+            for (int r = 0; r < E; ++r) v[r] = p1[r];
+        } else if (yu < y0 + R) {
+            load_pair(z1, yC, v, p1);
+        } else {
+            load_row(z1, yC, v);
+        }
+        fft(v);  // v = height + i Dx of row yC (unsigned)
 
         const size_t row = (size_t)y * N;
         if (normals) {

@@ -140,19 +140,20 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
 
     if (!valid) return;
     const bool mirror = mxc != tx;
+    // two-row tiles (zidx): threads j and j+1 of a column write the two rows
+    // of a tile, so the C columns of a strip fill runs of 2C values
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        const size_t row = (size_t)(j + r * T) * N;
-        z1[row + tx] = make_float2(ay[r].x - ax[r].y, ay[r].y + ax[r].x);
-        if (mirror) z1[row + zpos<N>(mxc)] = make_float2(ay[r].x + ax[r].y, ax[r].x - ay[r].y);
+        const int y = j + r * T;
+        z1[zidx<N>(y, tx)] = make_float2(ay[r].x - ax[r].y, ay[r].y + ax[r].x);
+        if (mirror) z1[zidx<N>(y, zpos<N>(mxc))] = make_float2(ay[r].x + ax[r].y, ax[r].x - ay[r].y);
     }
 #pragma unroll
     for (int r = 0; r < E / 2; ++r) {
-        const int    p   = j + r * T;
-        const size_t row = (size_t)p * N;
+        const int    p  = j + r * T;
         const float2 lo = buf[lds_pad(2 * p)], hi = buf[lds_pad(2 * p + 1)];  // rows 2p and 2p+1
-        z2[row + tx] = make_float2(lo.x - hi.y, lo.y + hi.x);
-        if (mirror) z2[row + zpos<N>(mxc)] = make_float2(lo.x + hi.y, hi.x - lo.y);
+        z2[zidx<N>(p, tx)] = make_float2(lo.x - hi.y, lo.y + hi.x);
+        if (mirror) z2[zidx<N>(p, zpos<N>(mxc))] = make_float2(lo.x + hi.y, hi.x - lo.y);
     }
 }
 
@@ -222,10 +223,12 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
     const int g = threadIdx.x / T;
     const int j = threadIdx.x % T;
     const int p = blockIdx.x;
-    const float2* src = g == 0 ? z1 + (size_t)(2 * p) * N : g == 1 ? z1 + (size_t)(2 * p + 1) * N : z2 + (size_t)p * N;
+    // Z1 rows 2p, 2p+1 form one two-row tile (zidx); Z2 row p is read alone
+    const float2* src = g == 2 ? z2 : z1;
+    const int     y   = g == 2 ? p : 2 * p + g;
     float2 v[E];
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
+    for (int r = 0; r < E; ++r) v[r] = src[zidx<N>(y, zpos_elem<N, E>(j, r))];
     fft_inverse<N, E>(v, j, lds + g * lds_len(N), tw, BlockSync{});
     constexpr float scale = 1.0f / ((float)N * (float)N);
     const size_t row0 = (size_t)(2 * p) * N, row1 = row0 + N;
@@ -253,8 +256,11 @@ __global__ __launch_bounds__(3 * (N / E)) void k_rows(const float2* __restrict__
 // lie in one wave when T <= 64 (wave-level ordering), otherwise a block is one
 // group.  1-D grid of ceil(bands*frames / groups per block) blocks; after the
 // XCD remap, neighbouring bands (which share halo rows) run on the same XCD.
+// At least two waves per SIMD: the band walk holds about 260 registers a lane
+// when left unbounded, and one wave per SIMD cannot hide its load latency (the
+// bound trades that for a few spilled registers; N=1024 frames 18.7 -> 17.6 us).
 template <int N, int E, int R, bool RENDER>
-__global__ __launch_bounds__(N / E >= 64 ? N / E : 64) void k_rows_nrm(
+__global__ __launch_bounds__(N / E >= 64 ? N / E : 64, 2) void k_rows_nrm(
     const float2* __restrict__ z1, const float2* __restrict__ z2, float* __restrict__ height,
     float* __restrict__ dx, float* __restrict__ dz, float4* __restrict__ normals, float* __restrict__ jac,
     const float2* __restrict__ tw, FrameBatch fb, float lambda, float inv2d, float4* __restrict__ disp,
@@ -265,17 +271,21 @@ __global__ __launch_bounds__(N / E >= 64 ? N / E : 64) void k_rows_nrm(
     constexpr int NB  = N / R;                  // bands per frame
     using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
     __shared__ float2 lds[GPB * lds_len(N)];
-    const int g    = threadIdx.x / T;
-    const int j    = threadIdx.x % T;
+    // With one group per block (block = T threads) g is 0: stated at compile
+    // time so that the frame and its buffer offsets stay scalar.
+    const int g    = GPB == 1 ? 0 : (int)threadIdx.x / T;
+    const int j    = GPB == 1 ? (int)threadIdx.x : (int)threadIdx.x % T;
     const int L    = xcd_logical(blockIdx.x, gridDim.x) * GPB + g;
     const int band = L % NB;
     const int f    = L / NB;
     if (f >= fb.nframes) return;  // whole group (a wave part when T <= 64): padding of the last block
     const size_t fo = (size_t)f * fb.out_stride;
-    rows_nrm_band<N, E, R, Sync, RENDER>(z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo,
-                                         dx + fo, dz + fo, normals ? normals + fo : nullptr,
-                                         jac ? jac + fo : nullptr, tw, lds + g * lds_len(N), j, band, lambda, inv2d,
-                                         disp ? disp + fo : nullptr, rnorm ? rnorm + fo : nullptr, dscale);
+    // Z2 rows paired up to N=1024 (N=512 frames -4%, N=1024 -1%); at N=2048 the
+    // extra registers spill and cost more than the second Z2 fetch
+    rows_nrm_band<N, E, R, Sync, RENDER, (N <= 1024)>(
+        z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo, dx + fo, dz + fo,
+        normals ? normals + fo : nullptr, jac ? jac + fo : nullptr, tw, lds + g * lds_len(N), j, band, lambda, inv2d,
+        disp ? disp + fo : nullptr, rnorm ? rnorm + fo : nullptr, dscale);
 }
 
 // ------------------------------------------------------------------ A7 normals (+ Jacobian)
