@@ -334,9 +334,53 @@ def test_full_size_point_samples_and_parseval(hip, N):
     assert abs(e_got - e_ref) <= 1e-5 * e_ref, (e_got, e_ref)
 
 
+@pytest.mark.parametrize("N", [2048, 4096])
+def test_full_size_multiframe_launch(hip, N):
+    """Two frames in one launch take the multi-frame schedule, at these N the
+    fused row + normal-map + Jacobian pass with bands of 32 rows that reads Z2
+    rows one at a time from their two-row tiles.  Height, Dx and Dz of both
+    frames against direct-DFT point samples; normals and Jacobian against the
+    oracle's stencils applied to the GPU's own height, Dx and Dz over the
+    whole field, which covers every band seam (the halo rows of each band)."""
+    kw = dict(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(0.5), math.sin(0.5)), amplitude=4.0,
+              choppiness=1.3)
+    lp, op = both_params(**kw)
+    times = np.array([7.25, 7.25 + 1.0 / 60.0])
+    F, nn = len(times), N * N
+    names = ("height", "dx", "dz", "normals", "jacobian")
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    bufs = {k: hip.malloc(F * nn * sizes[k]) for k in names}
+    _say("full size multi-frame N=%d" % N)
+    try:
+        with pyocean.Ocean(lp) as oc:
+            oc.tilde_h0_k()
+            sch = oc.schedule(F)
+            assert sch["rows_band"] == 32, sch
+            oc.update_frames(times, pyocean.OcnFields(*[bufs[k] for k in names]))
+            hip.sync()
+            got = [{k: hip.download(bufs[k] + f * nn * sizes[k], (N, N, 4) if k == "normals" else (N, N))
+                    for k in names} for f in range(F)]
+    finally:
+        for b in bufs.values():
+            hip.free(b)
+    h0 = oracle.h0(op)
+    rng = np.random.default_rng(N + 1)
+    pts = [(int(a), int(b)) for a, b in rng.integers(0, N, size=(8, 2))] + [(0, 0), (N - 1, 31), (5, 32)]
+    for f, t in enumerate(times):
+        g = got[f]
+        ref = _direct_points(op, h0, t, pts)
+        for i, (x, y) in enumerate(pts):
+            for k, fld in enumerate(("height", "dx", "dz")):
+                scale = np.abs(g[fld]).max()
+                assert abs(g[fld][y, x] - ref[i, k]) <= 1e-4 * scale, (f, fld, x, y, g[fld][y, x], ref[i, k])
+        assert_field_close("normals f%d" % f, g["normals"], oracle.normal_map(g["height"]))
+        assert_field_close("jacobian f%d" % f, g["jacobian"],
+                           oracle.jacobian(N, kw["L"], kw["choppiness"], g["dx"], g["dz"]))
+
+
 # ---------------------------------------------------------------- schedules
 
-@pytest.mark.parametrize("N", [64, 256, 1024])
+@pytest.mark.parametrize("N", [64, 256, 1024, 2048])
 def test_every_schedule_variant_matches_oracle(hip, N):
     kw = dict(c2_kwargs(), N=N, L=N * 2.0)
     lp, op = both_params(**kw)
