@@ -472,6 +472,12 @@ int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, void* stream)
 
 }  // namespace
 
+// One imported memory object of a graphics API (ocn_external_import).
+struct ocn_external {
+    int                 device = 0;
+    hipExternalMemory_t mem    = nullptr;
+};
+
 extern "C" {
 
 int ocn_abi_version(void) { return OCN_ABI_VERSION; }
@@ -1300,6 +1306,65 @@ int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* 
         for (int i = 0; i < 3; ++i) grad[i] = v.grad[i];
     if (loss) *loss = v.loss;
     return OCN_OK;
+}
+
+// ---------------------------------------------------------------- graphics interop
+
+// Stands in for the GL texture storage of main.cpp:1096-1099: the memory a
+// renderer exports is mapped, and the frames are written into it.
+int ocn_external_import(int device, int fd, uint64_t size, uint64_t offset, uint64_t bytes, ocn_external** out,
+                        void** dptr)
+{
+    if (!out || !dptr) return fail(OCN_EINVAL, "out/dptr is NULL");
+    *out  = nullptr;
+    *dptr = nullptr;
+    if (fd < 0) return fail(OCN_EINVAL, "fd %d is not a file descriptor", fd);
+    if (size == 0 || bytes == 0) return fail(OCN_EINVAL, "size and bytes must be > 0");
+    if (offset > size || bytes > size - offset)
+        return fail(OCN_EINVAL, "[%llu, %llu + %llu) exceeds the %llu-byte object", (unsigned long long)offset,
+                    (unsigned long long)offset, (unsigned long long)bytes, (unsigned long long)size);
+    if (offset % 256) return fail(OCN_EINVAL, "offset %llu is not a multiple of 256", (unsigned long long)offset);
+    int        count = 0;
+    hipError_t e     = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= count) return fail(OCN_EINVAL, "device %d not in [0, %d)", device, count);
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipExternalMemoryHandleDesc hd{};
+    hd.type      = hipExternalMemoryHandleTypeOpaqueFd;
+    hd.handle.fd = fd;
+    hd.size      = size;
+    hipExternalMemory_t mem = nullptr;
+    if ((e = hipImportExternalMemory(&mem, &hd)) != hipSuccess) return hip_fail(e, "hipImportExternalMemory");
+    hipExternalMemoryBufferDesc bd{};
+    bd.offset = offset;
+    bd.size   = bytes;
+    void* p   = nullptr;
+    if ((e = hipExternalMemoryGetMappedBuffer(&p, mem, &bd)) != hipSuccess || !p) {
+        (void)hipDestroyExternalMemory(mem);
+        return e != hipSuccess ? hip_fail(e, "hipExternalMemoryGetMappedBuffer")
+                               : fail(OCN_EHIP, "hipExternalMemoryGetMappedBuffer returned NULL");
+    }
+    auto* x = new (std::nothrow) ocn_external;
+    if (!x) {
+        (void)hipDestroyExternalMemory(mem);
+        return fail(OCN_ENOMEM, "host allocation of the import record");
+    }
+    x->device = device;
+    x->mem    = mem;
+    *out      = x;
+    *dptr     = p;
+    return OCN_OK;
+}
+
+// The mapped address lies inside the imported object, whose release
+// unmaps it; it is not freed on its own.
+int ocn_external_release(ocn_external* x)
+{
+    if (!x) return OCN_OK;
+    hipError_t e = hipSetDevice(x->device);
+    if (e == hipSuccess) e = hipDestroyExternalMemory(x->mem);
+    delete x;
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipDestroyExternalMemory");
 }
 
 }  // extern "C"

@@ -36,6 +36,7 @@ EXPORTS = [
     "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
+    "ocn_external_import", "ocn_external_release",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
 
@@ -138,6 +139,9 @@ def lib():
             "ocn_fit_step": [vp, ctypes.POINTER(OcnAdam), vp],
             "ocn_fit_get": [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl), vp],
             "ocn_fit_grad_h0": [vp, i32, ctypes.POINTER(vp)],
+            "ocn_external_import": [i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(vp),
+                                    ctypes.POINTER(vp)],
+            "ocn_external_release": [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -431,6 +435,47 @@ class Ocean:
 
 
 # ---------------------------------------------------------------- batches of oceans / cascades
+
+class External:
+    """Memory a graphics API exported as a POSIX file descriptor, mapped on the
+    GPU (ocn_external_import); `ptr` is its device address.  The frames are
+    written there by pointing OcnFields into it.  fd is not closed."""
+
+    def __init__(self, fd, size, offset=0, nbytes=None, device=0):
+        self.h = ctypes.c_void_p()
+        p = ctypes.c_void_p()
+        nbytes = size - offset if nbytes is None else nbytes
+        check(lib().ocn_external_import(int(device), int(fd), int(size), int(offset), int(nbytes),
+                                        ctypes.byref(self.h), ctypes.byref(p)))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def close(self):
+        if self.h:
+            check(lib().ocn_external_release(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def texture_fields(base, N, frames=1, align=256):
+    """OcnFields for the reference's five output textures laid out one after
+    another from device address `base` (height, dx, dz as R32F, normals as
+    RGBA32F, Jacobian R32F, each `frames` slices deep as ocn_update_frames
+    writes them; each start rounded up to `align` bytes), and the bytes the
+    block spans."""
+    sizes = (4, 4, 4, 16, 4)
+    ptrs, off = [], 0
+    for b in sizes:
+        off = (off + align - 1) // align * align
+        ptrs.append(base + off)
+        off += frames * N * N * b
+    return OcnFields(*ptrs), off
+
 
 def cascade_params(n_oceans=64, cascades=((2000.0, 256), (250.0, 512), (30.0, 1024)), seed=42,
                    wind_speed=(5.0, 30.0), choppiness=(0.8, 1.5), amplitude=4.0):

@@ -245,4 +245,26 @@ private:
     std::vector<size_t> target_sizes_;
 };
 
+// Memory a renderer exported as a POSIX file descriptor (a Vulkan
+// VkDeviceMemory from vkGetMemoryFdKHR, or an OpenGL memory object), mapped on
+// the GPU for the lifetime of this object; point ocn_fields into data() and
+// the frames land in the renderer's textures (main.cpp:1096-1099 in the
+// reference).  The descriptor is not closed.
+class ExternalMemory {
+public:
+    ExternalMemory(int fd, uint64_t size, uint64_t offset = 0, uint64_t bytes = 0, int device = 0)
+    {
+        check(ocn_external_import(device, fd, size, offset, bytes ? bytes : size - offset, &e_, &ptr_));
+    }
+    ~ExternalMemory() { ocn_external_release(e_); }
+    ExternalMemory(const ExternalMemory&)            = delete;
+    ExternalMemory& operator=(const ExternalMemory&) = delete;
+
+    void* data() const { return ptr_; }
+
+private:
+    ocn_external* e_   = nullptr;
+    void*         ptr_ = nullptr;
+};
+
 }  // namespace ocean

@@ -338,6 +338,32 @@ int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* 
 /* Device pointer of one member's dL/dh0 (float4[N*N]) from the last backward. */
 int ocn_fit_grad_h0(const ocn_fit* f, int member, void** grad_h0);
 
+/* ---------------------------------------------------------------- graphics interop
+ *
+ * The reference's outputs are GL textures that its renderer samples
+ * (main.cpp:1096-1099: R32F m_dx/m_dy/m_dz, RGBA32F m_normal_map; REPEAT wrap,
+ * main.cpp:1142-1144).  Here the renderer allocates that memory itself and
+ * exports it as a POSIX file descriptor -- a Vulkan VkDeviceMemory through
+ * vkGetMemoryFdKHR with VK_EXTERNAL_MEMORY_HANDLE_TYPE_OPAQUE_FD_BIT, or an
+ * OpenGL memory object exported the same way -- and the library maps it on
+ * the GPU.  ocn_fields pointing into the mapping make ocn_update_into and
+ * ocn_update_frames write the frames straight into the renderer's buffers or
+ * linear-tiled images (row pitch exactly N * texel size; the REPEAT wrap is
+ * the renderer's sampler state).  Nothing is copied.  Ordering stays with the
+ * caller: the renderer may read a frame once the update's stream has passed
+ * it (hipStreamSynchronize or an event), since HIP offers no external
+ * semaphores on Linux.  No reference counterpart. */
+typedef struct ocn_external ocn_external;
+
+/* Imports the memory object behind `fd` (`size` bytes, as exported) on HIP
+ * device `device` and maps [offset, offset + bytes) of it at *dptr.  offset
+ * must be a multiple of 256.  The call does not close fd. */
+int ocn_external_import(int device, int fd, uint64_t size, uint64_t offset, uint64_t bytes, ocn_external** out,
+                        void** dptr);
+/* Releases an import (NULL is a no-op).  Work that writes the mapping must
+ * have finished. */
+int ocn_external_release(ocn_external* e);
+
 /* ---------------------------------------------------------------- host utilities */
 
 /* generate_bit_reversed_indices (main.cpp:733-744): out[i] = log2N-bit reversal of i. */

@@ -169,6 +169,29 @@ def test_fit_null_pointers_rejected():
     assert L.ocn_fit_destroy(None) == pyocean.OCN_OK
 
 
+@pytest.mark.parametrize("case", ["null_out", "bad_fd", "zero_size", "zero_bytes", "past_end", "misaligned"])
+def test_external_import_rejects_bad_arguments_before_hip(case):
+    L = pyocean.lib()
+    h, p = ctypes.c_void_p(), ctypes.c_void_p()
+    args = dict(device=0, fd=3, size=1 << 20, offset=0, nbytes=1 << 20)
+    if case == "bad_fd":
+        args["fd"] = -1
+    elif case == "zero_size":
+        args["size"] = 0
+    elif case == "zero_bytes":
+        args["nbytes"] = 0
+    elif case == "past_end":
+        args.update(offset=4096, nbytes=1 << 20)
+    elif case == "misaligned":
+        args.update(offset=100, nbytes=1000)
+    out = None if case == "null_out" else ctypes.byref(h)
+    rc = L.ocn_external_import(args["device"], args["fd"], args["size"], args["offset"], args["nbytes"], out,
+                               ctypes.byref(p))
+    assert rc == pyocean.OCN_EINVAL, (case, rc, L.ocn_last_error())
+    assert not p.value
+    assert L.ocn_external_release(None) == pyocean.OCN_OK
+
+
 def test_cascade_params_match_the_batched_workload():
     # 64 oceans x 3 cascades (L, N) = (2000, 256), (250, 512), (30, 1024); per-ocean draws shared by its cascades
     ms = pyocean.cascade_params()
