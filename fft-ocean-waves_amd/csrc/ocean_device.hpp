@@ -77,15 +77,25 @@ __device__ __forceinline__ float phillips_amp(const SpecConst& sc, int tx, int t
 // where FP32 rounding of w alone would cost ~1e-5 rad, so w is refined in FP64:
 // an FP32 estimate w0 plus one Newton step on w^4 = g^2 |k|^2 (|k|^2 is exact
 // from the integer wave numbers), then the product with t / (2 pi) is reduced
-// to one turn before the FP32 sincos.  m = nx^2 + ny^2.
-__device__ __forceinline__ double omega_fp64(const SpecConst& sc, int m, float& kf)
+// to one turn before the FP32 sincos.  m = nx^2 + ny^2.  kinv receives
+// kscale / |k|_clamped, the factor of the choppy spectra (-i k/|k| = -i n kinv).
+// Two reciprocal square roots give the FP32 parts: 1/sqrt(m) is kinv itself,
+// and s = 1/sqrt(g |k|) gives w0 = g |k| s and the Newton step's 1/w0^3 = s^3.
+__device__ __forceinline__ double omega_fp64(const SpecConst& sc, int m, float& kinv)
 {
-    kf = fmaxf(sc.kscale * __builtin_amdgcn_sqrtf((float)m), 0.00001f);
-    const float  w0 = __builtin_amdgcn_sqrtf(sc.g * kf);
+    const float mf = (float)m;
+    const float r  = __builtin_amdgcn_rsqf(mf);  // +inf at m = 0
+    const float kr = sc.kscale * (mf * r);       // |k|; NaN at m = 0
+    const bool  lo = !(kr >= 0.00001f);          // |k| below the 1e-5 clamp (or DC)
+    const float kf = lo ? 0.00001f : kr;
+    kinv = lo ? sc.kscale * 100000.0f : r;
+    const float  gk = sc.g * kf;
+    const float  s  = __builtin_amdgcn_rsqf(gk);
+    const float  w0 = gk * s;
     const double Q  = fmax(sc.k2_d * (double)m, 1e-10) * sc.g2_d;  // g^2 |k|_clamped^2
     const double w  = (double)w0;
     const double w2 = w * w;
-    return w - (w2 * w2 - Q) * (double)(0.25f * __builtin_amdgcn_rcpf(w0 * w0 * w0));
+    return w - (w2 * w2 - Q) * (double)(0.25f * s * s * s);
 }
 
 // w t / (2 pi) reduced to [-1/2, 1/2] turns, w from omega_fp64.
@@ -95,9 +105,9 @@ __device__ __forceinline__ float turns(double w, double t_rev)
     return (float)(rev - rint(rev));
 }
 
-__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kf)
+__device__ __forceinline__ float phase_turns(const SpecConst& sc, int m, double t_rev, float& kinv)
 {
-    return turns(omega_fp64(sc, m, kf), t_rev);
+    return turns(omega_fp64(sc, m, kinv), t_rev);
 }
 
 // sin and cos of 2 pi fr for fr in [-1/2, 1/2]: reduction by quarter turns
@@ -114,9 +124,13 @@ __device__ __forceinline__ void sincos_turns(float fr, float& s, float& c)
     const float cp = fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, -2.75573192239858907e-7f, 2.48015873015873016e-5f),
                                                        -1.38888888888888889e-3f), 4.16666666666666667e-2f), -0.5f),
                           1.0f);
-    const int qi = (int)q & 3;
-    s = qi == 0 ? sp : qi == 1 ? cp : qi == 2 ? -sp : -cp;
-    c = qi == 0 ? cp : qi == 1 ? -sp : qi == 2 ? -cp : sp;
+    // quarter turn qi: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp),
+    // as a select and two sign-bit flips (no divergent branches)
+    const int   qi = (int)q & 3;
+    const float a  = (qi & 1) ? cp : sp;
+    const float b  = (qi & 1) ? sp : cp;
+    s = __int_as_float(__float_as_int(a) ^ ((qi & 2) << 30));
+    c = __int_as_float(__float_as_int(b) ^ (((qi + 1) & 2) << 30));
 }
 
 // a e^{i th} + C(b) e^{-i th} for h = (a, b) = (h0k, h0mk) (tilde_h0_t_cs.glsl:94-110).
@@ -136,9 +150,8 @@ __device__ __forceinline__ float2 choppy(float m, float2 h) { return make_float2
 __device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
                                           double t_rev, float2& ay, float2& ax, float2& az)
 {
-    float kf, s, c;
-    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kf), s, c);
-    const float  kinv = sc.kscale * __builtin_amdgcn_rcpf(kf);
+    float kinv, s, c;
+    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kinv), s, c);
     const float2 H = evolve_amp(sc, p, s, c);
     const float2 G = evolve_amp(sc, q, s, c);
     const float mx = (float)nx * kinv, mz = (float)ny * kinv;

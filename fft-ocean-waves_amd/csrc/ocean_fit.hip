@@ -161,8 +161,8 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         const int nx = txc - N / 2, ny = ty - N / 2;
-        float kf;
-        w[it] = omega_fp64(sc, nx * nx + ny * ny, kf);  // |k| is shared by t and t'
+        float kinv;
+        w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
     }
     for (int p0 = 0; p0 < m.P; p0 += 2) {
         const int np = min(2, m.P - p0);
@@ -316,8 +316,8 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         const int c = k % C, ty = k / C;
         const int tx = min(strip * C + c, N / 2);
         const int nx = tx - N / 2, ny = ty - N / 2;
-        float kf;
-        w[it] = omega_fp64(sc, nx * nx + ny * ny, kf);
+        float kinv;
+        w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);
     }
 
     for (int p0 = 0; p0 < m.P; p0 += 2) {

@@ -74,10 +74,9 @@ __global__ __launch_bounds__(256) void k_tilde_h0_t(const float4* __restrict__ h
     if (tx >= sc.N) return;
     const size_t i  = (size_t)ty * sc.N + tx;
     const int    nx = tx - sc.N / 2, ny = ty - sc.N / 2;
-    float kf, s, c;
-    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kf), s, c);
-    const float  kinv = sc.kscale * __builtin_amdgcn_rcpf(kf);
-    const float2 h    = evolve_amp(sc, h0[i], s, c);
+    float kinv, s, c;
+    sincos_turns(phase_turns(sc, nx * nx + ny * ny, t_rev, kinv), s, c);
+    const float2 h = evolve_amp(sc, h0[i], s, c);
     hy[i] = h;
     hx[i] = choppy((float)nx * kinv, h);
     hz[i] = choppy((float)ny * kinv, h);
