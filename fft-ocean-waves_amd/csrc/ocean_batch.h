@@ -36,6 +36,9 @@ struct BatchMember {
     SpecConst     sc;
     float         lambda;
     float         inv2d;
+    float4*       disp;    // render export (k_batch_rows_render only)
+    float4*       rnorm;
+    float         dscale;  // displacement_scale
 };
 
 // Work items: int2 (member, strip) for the column pass, (member, first band)
@@ -52,9 +55,10 @@ inline int batch_rows_items_per_member(int N)
 }
 inline int batch_rows_groups_per_block(int N) { return kBatchRowsThreads / (N / batch_rows_radix(N)); }
 
+// render: the row pass also writes each member's render export (disp, rnorm).
 hipError_t  launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
-                               int n_rows, double t_rev, hipStream_t s);
+                               int n_rows, double t_rev, bool render, hipStream_t s);
 const void* batch_cols_kernel();
-const void* batch_rows_kernel();
+const void* batch_rows_kernel(bool render);
 
 }  // namespace ocn

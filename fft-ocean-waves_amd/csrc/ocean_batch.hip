@@ -7,6 +7,7 @@
 //   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, C>, 768 threads,
 //                 C = 4096/N columns (N/2 at N = 64, whose strips use 384 threads)
 //   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8, R>,     256 threads
+//                 (k_batch_rows_render: the same with each member's render export)
 // Both block shapes are the same for every N (C = 4096/N columns per column
 // block, 256/T band groups per row block), so members of different N share a
 // launch; the kernels dispatch on the member's N.  Blocks are independent (no
@@ -54,7 +55,7 @@ __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip,
     cols_lds_strip<N, C, Sync, kBatchColsThreads>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
 }
 
-template <int N>
+template <int N, bool RENDER>
 __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_band, float2* lds)
 {
     constexpr int E = batch_rows_radix(N), T = N / E, R = batch_rows_band(N);
@@ -67,10 +68,28 @@ __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_
     const int band = first_band + g;
     if (band >= N / R) return;  // whole group (wave-aligned when GPB > 1)
     // Z2 rows paired too: the walk stays within 256 registers a lane, so two
-    // 256-thread blocks still share a CU
-    rows_nrm_band<N, E, R, Sync, false, true>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
-                                              E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, m.lambda,
-                                              m.inv2d);
+    // 256-thread blocks still share a CU (the render variant reads them alone
+    // to keep its extra outputs within the same budget)
+    rows_nrm_band<N, E, R, Sync, RENDER, !RENDER>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
+                                                  E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, m.lambda,
+                                                  m.inv2d, m.disp, m.rnorm, m.dscale);
+}
+
+template <bool RENDER>
+__device__ __forceinline__ void batch_rows(const BatchMember* __restrict__ members, const int2* __restrict__ items,
+                                           float2* lds)
+{
+    const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchMember m  = members[it.x];
+    switch (m.sc.N) {
+    case 64: batch_rows_item<64, RENDER>(m, it.y, lds); break;
+    case 128: batch_rows_item<128, RENDER>(m, it.y, lds); break;
+    case 256: batch_rows_item<256, RENDER>(m, it.y, lds); break;
+    case 512: batch_rows_item<512, RENDER>(m, it.y, lds); break;
+    case 1024: batch_rows_item<1024, RENDER>(m, it.y, lds); break;
+    case 2048: batch_rows_item<2048, RENDER>(m, it.y, lds); break;
+    default: break;
+    }
 }
 
 }  // namespace
@@ -96,29 +115,33 @@ __global__ __launch_bounds__(kBatchRowsThreads) void k_batch_rows(const BatchMem
                                                                   const int2* __restrict__ items)
 {
     __shared__ float2 lds[kRowsLds];
-    const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
-    const BatchMember m  = members[it.x];
-    switch (m.sc.N) {
-    case 64: batch_rows_item<64>(m, it.y, lds); break;
-    case 128: batch_rows_item<128>(m, it.y, lds); break;
-    case 256: batch_rows_item<256>(m, it.y, lds); break;
-    case 512: batch_rows_item<512>(m, it.y, lds); break;
-    case 1024: batch_rows_item<1024>(m, it.y, lds); break;
-    case 2048: batch_rows_item<2048>(m, it.y, lds); break;
-    default: break;
-    }
+    batch_rows<false>(members, items, lds);
+}
+
+// The same row pass with the render export of every member (disp, rnorm).
+__global__ __launch_bounds__(kBatchRowsThreads) void k_batch_rows_render(const BatchMember* __restrict__ members,
+                                                                         const int2* __restrict__ items)
+{
+    __shared__ float2 lds[kRowsLds];
+    batch_rows<true>(members, items, lds);
 }
 
 hipError_t launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
-                              int n_rows, double t_rev, hipStream_t s)
+                              int n_rows, double t_rev, bool render, hipStream_t s)
 {
     if (n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_cols, dim3(n_cols), dim3(kBatchColsThreads), 0, s, members, cols_items, t_rev);
-    hipLaunchKernelGGL(k_batch_rows, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+    if (render)
+        hipLaunchKernelGGL(k_batch_rows_render, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+    else
+        hipLaunchKernelGGL(k_batch_rows, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
     return hipGetLastError();
 }
 
 const void* batch_cols_kernel() { return reinterpret_cast<const void*>(&k_batch_cols); }
-const void* batch_rows_kernel() { return reinterpret_cast<const void*>(&k_batch_rows); }
+const void* batch_rows_kernel(bool render)
+{
+    return render ? reinterpret_cast<const void*>(&k_batch_rows_render) : reinterpret_cast<const void*>(&k_batch_rows);
+}
 
 }  // namespace ocn

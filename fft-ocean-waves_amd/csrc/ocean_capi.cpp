@@ -339,6 +339,8 @@ struct ocn_batch {
     int                                    n_cols    = 0;
     int                                    n_rows    = 0;
     char*                                  pool      = nullptr;  // every member's buffers
+    char*                                  render_pool = nullptr;    // render exports (on request)
+    bool                                   render      = false;
     std::map<int, std::array<float2*, 2>>  tw;  // per N: radix-8 and radix-16 Stockham twiddles
     bool                                   h0_ready  = false;
     hipGraph_t                             graph     = nullptr;
@@ -353,7 +355,7 @@ void batch_free(ocn_batch* b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->d_members, b->d_cols, b->d_rows, b->pool};
+    void* ptrs[] = {b->d_members, b->d_cols, b->d_rows, b->pool, b->render_pool};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& kv : b->tw)
@@ -933,6 +935,7 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
             m.sc      = spec_const(p);
             m.lambda  = (float)p.choppiness;
             m.inv2d   = (float)((double)p.N / (2.0 * p.L));
+            m.dscale  = (float)p.displacement_scale;
             b->members.push_back(m);
             for (int k = 0; k < ocn::batch_cols_items_per_member(p.N); ++k) cols.push_back(make_int2(i, k));
             const int gpb = ocn::batch_rows_groups_per_block(p.N);
@@ -1008,7 +1011,7 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream)
     int rc = batch_ready(b, t);
     if (rc) return rc;
     hipError_t e = ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows,
-                                           t / (2.0 * kPi), (hipStream_t)stream);
+                                           t / (2.0 * kPi), b->render, (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "batch update launch");
 }
 
@@ -1031,7 +1034,7 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream)
     hipError_t e;
     if (!b->exec) {
         hipKernelNodeParams kr = kc;
-        kr.func                = const_cast<void*>(ocn::batch_rows_kernel());
+        kr.func                = const_cast<void*>(ocn::batch_rows_kernel(b->render));
         kr.gridDim             = dim3(b->n_rows);
         kr.blockDim            = dim3(ocn::kBatchRowsThreads);
         kr.kernelParams        = rows_args;
@@ -1061,7 +1064,44 @@ int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out)
     if (member < 0 || member >= (int)b->members.size())
         return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
     const ocn::BatchMember& m = b->members[member];
-    *out = ocn_fields{m.height, m.dx, m.dz, reinterpret_cast<float*>(m.normals), m.jac};
+    *out = ocn_fields{m.height, m.dx, m.dz, reinterpret_cast<float*>(m.normals), m.jac,
+                      reinterpret_cast<float*>(m.disp), reinterpret_cast<float*>(m.rnorm)};
+    return OCN_OK;
+}
+
+// grid_tes.glsl:58-62 / main.cpp:1632-1633 for every member: the render export
+// of the single-ocean fields (ocn_fields.displacement / render_normals).
+int ocn_batch_set_render_export(ocn_batch* b, int enable)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (enable != 0 && enable != 1) return fail(OCN_EINVAL, "enable must be 0 or 1");
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();  // no launch may read the member table while it changes
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    if (enable && !b->render_pool) {
+        size_t total = 0;
+        for (const ocn_params& p : b->params) total += 2 * al((size_t)p.N * p.N * 16);
+        if ((rc = dalloc(&b->render_pool, total, "batch render pool"))) return rc;
+    }
+    char* q = b->render_pool;
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        const size_t bytes = al((size_t)b->params[i].N * b->params[i].N * 16);
+        b->members[i].disp  = enable ? reinterpret_cast<float4*>(q) : nullptr;
+        b->members[i].rnorm = enable ? reinterpret_cast<float4*>(q + bytes) : nullptr;
+        if (q) q += 2 * bytes;
+    }
+    e = hipMemcpy(b->d_members, b->members.data(), b->members.size() * sizeof(ocn::BatchMember),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(batch members)");
+    if (b->exec && b->render != (enable != 0)) {  // the graph's row node changes kernel: rebuilt on next use
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec  = nullptr;
+        b->graph = nullptr;
+    }
+    b->render = enable != 0;
     return OCN_OK;
 }
 

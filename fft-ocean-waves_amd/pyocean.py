@@ -33,7 +33,8 @@ EXPORTS = [
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
-    "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_fit_upload_targets",
+    "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_batch_set_render_export",
+    "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_external_import", "ocn_external_release",
@@ -129,6 +130,7 @@ def lib():
             "ocn_batch_get_fields": [vp, i32, F],
             "ocn_batch_get_h0": [vp, i32, ctypes.POINTER(vp)],
             "ocn_batch_copy_fields_to_host": [vp, i32] + [vp] * 6,
+            "ocn_batch_set_render_export": [vp, i32],
             "ocn_fit_upload_targets": [vp, i32, vp, vp],
             "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
             "ocn_fit_destroy": [vp],
@@ -572,6 +574,18 @@ class Batch:
         self.hip.sync()
         N = self.members[member].N
         return self.hip.download(self.h0_ptr(member), (N, N, 4))
+
+    def set_render_export(self, enable=True):
+        check(lib().ocn_batch_set_render_export(self.h, 1 if enable else 0))
+
+    def download_render(self, member):
+        """The member's render export (displacement, render_normals), N x N x 4 each."""
+        self.hip.sync()
+        f, N = self.fields(member), self.members[member].N
+        if not f.displacement:
+            raise OceanError("OCN_ESTATE", "the render export is off")
+        return {"displacement": self.hip.download(f.displacement, (N, N, 4)),
+                "render_normals": self.hip.download(f.render_normals, (N, N, 4))}
 
 
 # ---------------------------------------------------------------- differentiable fits

@@ -152,6 +152,8 @@ public:
     void       tilde_h0_k() { check(ocn_batch_tilde_h0_k(b_, stream_)); }
     void       update(double t) { check(ocn_batch_update(b_, t, stream_)); }
     void       update_graph(double t) { check(ocn_batch_update_graph(b_, t, stream_)); }
+    // Every member's render export (fields().displacement / render_normals) on or off.
+    void       set_render_export(bool enable) { check(ocn_batch_set_render_export(b_, enable ? 1 : 0)); }
     ocn_fields fields(int member) const
     {
         ocn_fields f;

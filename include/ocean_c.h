@@ -16,7 +16,9 @@
  *   - Device pointers are HIP device memory on the handle's device; `stream`
  *     is a hipStream_t passed as void* (NULL = the null stream).  Work is
  *     stream-ordered; no call synchronises the host except ocn_create,
- *     ocn_set_params (buffer setup) and ocn_destroy.
+ *     ocn_set_params (buffer setup), ocn_destroy, the copies to host memory
+ *     and ocn_fit_get (which wait for their stream) and
+ *     ocn_batch_set_render_export.
  *   - Layouts match the reference textures: texel (tx, ty) is element
  *     ty*N + tx; tx is the first texture coordinate (world x), ty the second
  *     (world z).  R32F fields are float[N*N]; RGBA32F fields are float[N*N*4].
@@ -283,9 +285,17 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream);
  * not overlap in time with each other or with ocn_batch_update. */
 int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
 
-/* Device pointers of one member's outputs (valid until ocn_batch_destroy)
+/* Device pointers of one member's outputs (valid until ocn_batch_destroy;
+ * displacement and render_normals are NULL unless the render export is on)
  * and its h0 spectrum (float4[N*N]). */
 int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out);
+/* Render export for every member on (1) or off (0): the row pass then also
+ * writes each member's displacement and render_normals as ocn_update_into
+ * does for one ocean given those fields (grid_tes.glsl:58-62; each member's
+ * displacement_scale), into buffers the batch allocates on the first enable
+ * (32 bytes per texel).  Waits for the device to be idle; call it between
+ * frames. */
+int ocn_batch_set_render_export(ocn_batch* b, int enable);
 int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0);
 /* Host copies of one member's outputs after the work on `stream` (synchronises
  * it); any pointer may be NULL; sizes as ocn_copy_fields_to_host. */

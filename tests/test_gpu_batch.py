@@ -156,3 +156,45 @@ def test_cascade_workload_members_match_single_ocean(hip):
         ref = oracle.frame(op, oracle.h0(op), t, nthreads=NTHREADS)
         for k in FIELDS:
             assert_field_close("cascade member %d %s" % (i, k), got[i][k], ref[k])
+
+
+def test_batch_render_export_matches_oracle_stencils(hip):
+    """With the render export on, every member also writes its displacement
+    and displaced-surface normals (each with its own displacement_scale):
+    against the oracle's render stencils on the member's own height, Dx and
+    Dz over the whole field (every band seam), with the other five outputs
+    bit-identical to the batch without the export.  Graph replays follow the
+    switch both ways."""
+    scales = [0.5, 0.8, 1.0, 0.3, 0.6, 0.9, 0.4]
+    members = [pyocean.make_params(**kw, displacement_scale=s) for kw, s in zip(MIXED, scales)]
+    t = 2.2
+    _say("batch render export N=%s" % [m.N for m in members])
+    with pyocean.Batch(members) as b:
+        b.tilde_h0_k()
+        b.update(t)
+        plain = [b.download_fields(i) for i in range(len(members))]
+        assert not b.fields(0).displacement
+        b.set_render_export(True)
+        b.update(t)
+        got = [b.download_fields(i) for i in range(len(members))]
+        rend = [b.download_render(i) for i in range(len(members))]
+        b.update_graph(t + 1.0)
+        b.update_graph(t)
+        rend_graph = [b.download_render(i) for i in range(len(members))]
+        b.set_render_export(False)
+        b.update_graph(t)
+        off = [b.download_fields(i) for i in range(len(members))]
+        assert not b.fields(0).displacement
+    for i, m in enumerate(members):
+        for k in FIELDS:
+            np.testing.assert_array_equal(got[i][k], plain[i][k], err_msg="member %d %s" % (i, k))
+            np.testing.assert_array_equal(off[i][k], plain[i][k], err_msg="member %d %s (export off)" % (i, k))
+        for k in ("displacement", "render_normals"):
+            np.testing.assert_array_equal(rend_graph[i][k], rend[i][k], err_msg="member %d %s graph" % (i, k))
+        g = got[i]
+        d_own, n_own = oracle.render(m.N, m.L, m.choppiness, m.displacement_scale, g["height"], g["dx"], g["dz"])
+        _say("  member %d N=%d" % (i, m.N))
+        assert_field_close("member %d displacement" % i, rend[i]["displacement"], d_own, rel_tol=1e-6,
+                           abs_rel_tol=1e-5)
+        assert_field_close("member %d render normals" % i, rend[i]["render_normals"], n_own, rel_tol=1e-6,
+                           abs_rel_tol=1e-5)
