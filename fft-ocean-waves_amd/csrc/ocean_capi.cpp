@@ -9,6 +9,7 @@
 #include <array>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -1067,6 +1068,38 @@ int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out)
     *out = ocn_fields{m.height, m.dx, m.dz, reinterpret_cast<float*>(m.normals), m.jac,
                       reinterpret_cast<float*>(m.disp), reinterpret_cast<float*>(m.rnorm)};
     return OCN_OK;
+}
+
+// Caller-provided outputs of one member (the batch counterpart of
+// ocn_update_into): the row pass writes that member's textures at the given
+// device addresses from the next frame on.
+int ocn_batch_set_member_fields(ocn_batch* b, int member, const ocn_fields* f)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!f) return fail(OCN_EINVAL, "fields is NULL");
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    if (!f->height || !f->dx || !f->dz) return fail(OCN_EINVAL, "height, dx and dz must be non-NULL");
+    if (((uintptr_t)f->normals & 15) || ((uintptr_t)f->displacement & 15) || ((uintptr_t)f->render_normals & 15))
+        return fail(OCN_EINVAL, "float4 fields must be 16-byte aligned");
+    if ((f->displacement || f->render_normals) && !b->render)
+        return fail(OCN_ESTATE, "the render export is off (ocn_batch_set_render_export)");
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();  // no launch may read the member table while it changes
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    ocn::BatchMember& m = b->members[member];
+    m.height = f->height;
+    m.dx     = f->dx;
+    m.dz     = f->dz;
+    m.normals = reinterpret_cast<float4*>(f->normals);
+    m.jac     = f->jacobian;
+    if (b->render) {
+        if (f->displacement) m.disp = reinterpret_cast<float4*>(f->displacement);
+        if (f->render_normals) m.rnorm = reinterpret_cast<float4*>(f->render_normals);
+    }
+    e = hipMemcpy(b->d_members + member, &m, sizeof(m), hipMemcpyHostToDevice);
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpy(batch member)");
 }
 
 // grid_tes.glsl:58-62 / main.cpp:1632-1633 for every member: the render export

@@ -34,7 +34,7 @@ EXPORTS = [
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
     "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_batch_set_render_export",
-    "ocn_fit_upload_targets",
+    "ocn_batch_set_member_fields", "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_external_import", "ocn_external_release",
@@ -131,6 +131,7 @@ def lib():
             "ocn_batch_get_h0": [vp, i32, ctypes.POINTER(vp)],
             "ocn_batch_copy_fields_to_host": [vp, i32] + [vp] * 6,
             "ocn_batch_set_render_export": [vp, i32],
+            "ocn_batch_set_member_fields": [vp, i32, F],
             "ocn_fit_upload_targets": [vp, i32, vp, vp],
             "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
             "ocn_fit_destroy": [vp],
@@ -577,6 +578,10 @@ class Batch:
 
     def set_render_export(self, enable=True):
         check(lib().ocn_batch_set_render_export(self.h, 1 if enable else 0))
+
+    def set_member_fields(self, member, fields):
+        """Outputs of one member written at caller-provided device addresses."""
+        check(lib().ocn_batch_set_member_fields(self.h, int(member), ctypes.byref(fields)))
 
     def download_render(self, member):
         """The member's render export (displacement, render_normals), N x N x 4 each."""

@@ -154,6 +154,8 @@ public:
     void       update_graph(double t) { check(ocn_batch_update_graph(b_, t, stream_)); }
     // Every member's render export (fields().displacement / render_normals) on or off.
     void       set_render_export(bool enable) { check(ocn_batch_set_render_export(b_, enable ? 1 : 0)); }
+    // One member's outputs at caller-provided device addresses (e.g. ExternalMemory).
+    void set_member_fields(int member, const ocn_fields& f) { check(ocn_batch_set_member_fields(b_, member, &f)); }
     ocn_fields fields(int member) const
     {
         ocn_fields f;

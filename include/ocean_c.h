@@ -293,9 +293,19 @@ int ocn_batch_get_fields(const ocn_batch* b, int member, ocn_fields* out);
  * writes each member's displacement and render_normals as ocn_update_into
  * does for one ocean given those fields (grid_tes.glsl:58-62; each member's
  * displacement_scale), into buffers the batch allocates on the first enable
- * (32 bytes per texel).  Waits for the device to be idle; call it between
- * frames. */
+ * (32 bytes per texel; every call points the members' export back at them).
+ * Waits for the device to be idle; call it between frames. */
 int ocn_batch_set_render_export(ocn_batch* b, int enable);
+/* Caller-provided outputs of one member, the batch counterpart of
+ * ocn_update_into (for example texture memory a renderer exported, mapped by
+ * ocn_external_import): from the next frame on the row pass writes height,
+ * dx, dz (required), normals and jacobian (NULL skips them) and, while the
+ * render export is on, displacement / render_normals (NULL keeps the batch's
+ * own) at these device addresses, N*N texels each.  float4 fields must be
+ * 16-byte aligned.  The batch's own buffers stay allocated: passing the
+ * fields ocn_batch_get_fields returned before restores them.  Waits for the
+ * device to be idle; call it between frames. */
+int ocn_batch_set_member_fields(ocn_batch* b, int member, const ocn_fields* f);
 int ocn_batch_get_h0(const ocn_batch* b, int member, void** h0);
 /* Host copies of one member's outputs after the work on `stream` (synchronises
  * it); any pointer may be NULL; sizes as ocn_copy_fields_to_host. */

@@ -1,6 +1,7 @@
-"""Graphics interop (ocn_external_import / ocn_external_release): frames
-written straight into memory that a renderer exported, where the reference's
-renderer samples its GL textures (main.cpp:1096-1099).
+"""Graphics interop (ocn_external_import / ocn_external_release, and
+ocn_batch_set_member_fields for cascades): frames written straight into
+memory that a renderer exported, where the reference's renderer samples its
+GL textures (main.cpp:1096-1099).
 
 There is no Vulkan or OpenGL stack on these machines, so the exported memory
 is made with HIP's virtual memory API (tests/cpp/vmm_export.cpp): a device
@@ -151,3 +152,50 @@ def test_multiframe_launch_into_imported_memory_at_an_offset(hip, vmm):
                     np.testing.assert_array_equal(got[f][k], ref[f][k], err_msg="%s frame %d" % (k, f))
     finally:
         ex.close()
+
+
+def test_batch_member_written_into_imported_memory(hip, vmm):
+    """One member of a batch (render export on) redirected into imported
+    memory writes there exactly what it wrote into the batch's own buffers;
+    the other members are untouched, and the batch's buffers come back when
+    their addresses are passed again."""
+    members = [pyocean.make_params(N=64, L=100.0, wind_speed=12.0, ocean_id=0),
+               pyocean.make_params(N=256, L=600.0, wind_speed=25.0, wind_dir=(0.3, 1.0), ocean_id=1,
+                                   displacement_scale=0.8),
+               pyocean.make_params(N=512, L=900.0, wind_speed=30.0, ocean_id=2)]
+    N, t = 256, 1.3
+    with pyocean.Batch(members) as b:
+        b.tilde_h0_k()
+        b.set_render_export(True)
+        b.update(t)
+        ref = [b.download_fields(i) for i in range(3)]
+        ref_r = b.download_render(1)
+        own = b.fields(1)
+        _, nbytes = pyocean.texture_fields(0, N)
+        roff = (nbytes + 255) // 256 * 256
+        ex = Exported(vmm, roff + 2 * N * N * 16)
+        try:
+            with pyocean.External(ex.fd, ex.size) as ext:
+                f, _ = pyocean.texture_fields(ext.ptr, N)
+                f.displacement, f.render_normals = ext.ptr + roff, ext.ptr + roff + N * N * 16
+                view, _ = pyocean.texture_fields(ex.va, N)
+                b.set_member_fields(1, f)
+                b.update(t)
+                hip.sync()
+                got = _download(hip, view, N)
+                got_d = hip.download(ex.va + roff, (N, N, 4))
+                got_n = hip.download(ex.va + roff + N * N * 16, (N, N, 4))
+                others = [b.download_fields(i) for i in (0, 2)]
+                b.set_member_fields(1, own)  # before the import goes away
+            b.update(t + 0.5)
+            b.update(t)
+            back = b.download_fields(1)
+        finally:
+            ex.close()
+    for k in NAMES:
+        np.testing.assert_array_equal(got[k], ref[1][k], err_msg=k)
+        np.testing.assert_array_equal(back[k], ref[1][k], err_msg=k + " (own buffers again)")
+        np.testing.assert_array_equal(others[0][k], ref[0][k], err_msg=k + " member 0")
+        np.testing.assert_array_equal(others[1][k], ref[2][k], err_msg=k + " member 2")
+    np.testing.assert_array_equal(got_d, ref_r["displacement"])
+    np.testing.assert_array_equal(got_n, ref_r["render_normals"])
