@@ -14,7 +14,11 @@ constexpr int kBatchColsThreads = 768;   // 3 * C * (N/16) with C = kBatchColsWi
 constexpr int kBatchColsWidth   = 4096;  // C * N: columns per column block times N (N = 64: C = N/2)
 constexpr int kBatchRowsThreads = 256;   // (256 / T) groups of T = N/8 threads
 constexpr int kBatchMinN        = 64;
-constexpr int kBatchMaxN        = 2048;
+constexpr int kBatchMaxN        = 4096;
+// Members of this N walk their row bands in a launch of their own: a radix-8
+// row group of N = 4096 is 512 threads, wider than the shared row block.
+constexpr int kBatchWideN           = 4096;
+constexpr int kBatchWideRowsThreads = kBatchWideN / 8;
 
 // Row-pass radix and band height per N: radix 8 keeps the fused row kernel
 // at two waves per SIMD; short bands at small N keep a row block busy.
@@ -48,17 +52,24 @@ __host__ __device__ constexpr int batch_cols_per_block(int N)
     return kBatchColsWidth / N < N / 2 ? kBatchColsWidth / N : N / 2;
 }
 inline int batch_cols_items_per_member(int N) { return (N / 2) / batch_cols_per_block(N) + 1; }
+// groups per row block: one band per block in the wide launch
+inline int batch_rows_groups_per_block(int N)
+{
+    return N == kBatchWideN ? 1 : kBatchRowsThreads / (N / batch_rows_radix(N));
+}
 inline int batch_rows_items_per_member(int N)
 {
-    const int T = N / batch_rows_radix(N), gpb = kBatchRowsThreads / T, bands = N / batch_rows_band(N);
+    const int gpb = batch_rows_groups_per_block(N), bands = N / batch_rows_band(N);
     return (bands + gpb - 1) / gpb;
 }
-inline int batch_rows_groups_per_block(int N) { return kBatchRowsThreads / (N / batch_rows_radix(N)); }
 
 // render: the row pass also writes each member's render export (disp, rnorm).
+// wide_items: (member, band) of the N = 4096 members (n_wide may be 0).
 hipError_t  launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
-                               int n_rows, double t_rev, bool render, hipStream_t s);
+                               int n_rows, const int2* wide_items, int n_wide, double t_rev, bool render,
+                               hipStream_t s);
 const void* batch_cols_kernel();
 const void* batch_rows_kernel(bool render);
+const void* batch_wide_rows_kernel(bool render);
 
 }  // namespace ocn

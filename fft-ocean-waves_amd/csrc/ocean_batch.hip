@@ -7,7 +7,8 @@
 //   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, C>, 768 threads,
 //                 C = 4096/N columns (N/2 at N = 64, whose strips use 384 threads)
 //   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8, R>,     256 threads
-//                 (k_batch_rows_render: the same with each member's render export)
+//                 (k_batch_rows_render: the same with each member's render export;
+//                 members of N = 4096 walk their bands in k_batch_rows_wide, 512 threads)
 // Both block shapes are the same for every N (C = 4096/N columns per column
 // block, 256/T band groups per row block), so members of different N share a
 // launch; the kernels dispatch on the member's N.  Blocks are independent (no
@@ -45,7 +46,7 @@ constexpr int rows_lds_need()
     return m;
 }
 constexpr int kColsLds = cols_lds_need();  // 3 * C * lds_len(N)
-constexpr int kRowsLds = rows_lds_need();  // (256 / T) * lds_len(N), T = N/8
+constexpr int kRowsLds = rows_lds_need();  // (256 / T) * lds_len(N), T = N/8 (N = 4096: the wide launch)
 
 template <int N>
 __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip, double t_rev, float2* lds)
@@ -107,6 +108,7 @@ __global__ __launch_bounds__(kBatchColsThreads) void k_batch_cols(const BatchMem
     case 512: batch_cols_item<512>(m, it.y, t_rev, lds); break;
     case 1024: batch_cols_item<1024>(m, it.y, t_rev, lds); break;
     case 2048: batch_cols_item<2048>(m, it.y, t_rev, lds); break;
+    case 4096: batch_cols_item<4096>(m, it.y, t_rev, lds); break;
     default: break;
     }
 }
@@ -126,15 +128,56 @@ __global__ __launch_bounds__(kBatchRowsThreads) void k_batch_rows_render(const B
     batch_rows<true>(members, items, lds);
 }
 
-hipError_t launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
-                              int n_rows, double t_rev, bool render, hipStream_t s)
+// Row bands of the N = 4096 members: one band of 32 rows per 512-thread block
+// (the radix-8 group of rows_nrm_band), Z2 rows read alone as in the
+// single-ocean row kernel at this N.  Two waves per SIMD.
+template <bool RENDER>
+__device__ __forceinline__ void batch_wide_rows(const BatchMember* __restrict__ members,
+                                                const int2* __restrict__ items, float2* lds)
 {
-    if (n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
+    constexpr int N = kBatchWideN, E = 8, R = batch_rows_band(N);
+    static_assert(N / E == kBatchWideRowsThreads, "one radix-8 group per block");
+    const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const BatchMember m  = members[it.x];
+    rows_nrm_band<N, E, R, BlockSync, RENDER, false>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac, m.tw8,
+                                                     lds, (int)threadIdx.x, it.y, m.lambda, m.inv2d, m.disp, m.rnorm,
+                                                     m.dscale);
+}
+
+__global__ __launch_bounds__(kBatchWideRowsThreads, 2) void k_batch_rows_wide(
+    const BatchMember* __restrict__ members, const int2* __restrict__ items)
+{
+    __shared__ float2 lds[lds_len(kBatchWideN)];
+    batch_wide_rows<false>(members, items, lds);
+}
+
+__global__ __launch_bounds__(kBatchWideRowsThreads, 2) void k_batch_rows_wide_render(
+    const BatchMember* __restrict__ members, const int2* __restrict__ items)
+{
+    __shared__ float2 lds[lds_len(kBatchWideN)];
+    batch_wide_rows<true>(members, items, lds);
+}
+
+hipError_t launch_batch_frame(const BatchMember* members, const int2* cols_items, int n_cols, const int2* rows_items,
+                              int n_rows, const int2* wide_items, int n_wide, double t_rev, bool render,
+                              hipStream_t s)
+{
+    if (n_cols < 1 || n_rows + n_wide < 1 || n_rows < 0 || n_wide < 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_cols, dim3(n_cols), dim3(kBatchColsThreads), 0, s, members, cols_items, t_rev);
-    if (render)
-        hipLaunchKernelGGL(k_batch_rows_render, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
-    else
-        hipLaunchKernelGGL(k_batch_rows, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+    if (n_rows > 0) {
+        if (render)
+            hipLaunchKernelGGL(k_batch_rows_render, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+        else
+            hipLaunchKernelGGL(k_batch_rows, dim3(n_rows), dim3(kBatchRowsThreads), 0, s, members, rows_items);
+    }
+    if (n_wide > 0) {
+        if (render)
+            hipLaunchKernelGGL(k_batch_rows_wide_render, dim3(n_wide), dim3(kBatchWideRowsThreads), 0, s, members,
+                               wide_items);
+        else
+            hipLaunchKernelGGL(k_batch_rows_wide, dim3(n_wide), dim3(kBatchWideRowsThreads), 0, s, members,
+                               wide_items);
+    }
     return hipGetLastError();
 }
 
@@ -142,6 +185,11 @@ const void* batch_cols_kernel() { return reinterpret_cast<const void*>(&k_batch_
 const void* batch_rows_kernel(bool render)
 {
     return render ? reinterpret_cast<const void*>(&k_batch_rows_render) : reinterpret_cast<const void*>(&k_batch_rows);
+}
+const void* batch_wide_rows_kernel(bool render)
+{
+    return render ? reinterpret_cast<const void*>(&k_batch_rows_wide_render)
+                  : reinterpret_cast<const void*>(&k_batch_rows_wide);
 }
 
 }  // namespace ocn

@@ -337,8 +337,10 @@ struct ocn_batch {
     ocn::BatchMember*                      d_members = nullptr;
     int2*                                  d_cols    = nullptr;
     int2*                                  d_rows    = nullptr;
+    int2*                                  d_wide    = nullptr;  // row items of the N = 4096 members
     int                                    n_cols    = 0;
     int                                    n_rows    = 0;
+    int                                    n_wide    = 0;
     char*                                  pool      = nullptr;  // every member's buffers
     char*                                  render_pool = nullptr;    // render exports (on request)
     bool                                   render      = false;
@@ -348,6 +350,7 @@ struct ocn_batch {
     hipGraphExec_t                         exec      = nullptr;
     hipGraphNode_t                         node_cols = nullptr;
     hipGraphNode_t                         node_rows = nullptr;
+    hipGraphNode_t                         node_wide = nullptr;
 };
 
 namespace {
@@ -356,7 +359,7 @@ void batch_free(ocn_batch* b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->d_members, b->d_cols, b->d_rows, b->pool, b->render_pool};
+    void* ptrs[] = {b->d_members, b->d_cols, b->d_rows, b->d_wide, b->pool, b->render_pool};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& kv : b->tw)
@@ -918,7 +921,7 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
     }
     if (!rc) {
         char*             q = b->pool;
-        std::vector<int2> cols, rows;
+        std::vector<int2> cols, rows, wide;
         for (int i = 0; i < count; ++i) {
             const ocn_params& p  = b->params[i];
             const size_t      nn = (size_t)p.N * p.N;
@@ -940,7 +943,8 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
             b->members.push_back(m);
             for (int k = 0; k < ocn::batch_cols_items_per_member(p.N); ++k) cols.push_back(make_int2(i, k));
             const int gpb = ocn::batch_rows_groups_per_block(p.N);
-            for (int k = 0; k < ocn::batch_rows_items_per_member(p.N); ++k) rows.push_back(make_int2(i, k * gpb));
+            for (int k = 0; k < ocn::batch_rows_items_per_member(p.N); ++k)
+                (p.N == ocn::kBatchWideN ? wide : rows).push_back(make_int2(i, k * gpb));
         }
         rows = balance_items(rows, [&](int2 it) {
             const int N = b->params[it.x].N, R = ocn::batch_rows_band(N);
@@ -949,9 +953,11 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
         });
         b->n_cols = (int)cols.size();
         b->n_rows = (int)rows.size();
+        b->n_wide = (int)wide.size();
         if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) &&
-            !(rc = upload_vector(&b->d_cols, cols, "batch column items")))
+            !(rc = upload_vector(&b->d_cols, cols, "batch column items")) && !rows.empty())
             rc = upload_vector(&b->d_rows, rows, "batch row items");
+        if (!rc && !wide.empty()) rc = upload_vector(&b->d_wide, wide, "batch wide row items");
     }
     if (rc) {
         batch_free(b);
@@ -1011,8 +1017,8 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream)
 {
     int rc = batch_ready(b, t);
     if (rc) return rc;
-    hipError_t e = ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows,
-                                           t / (2.0 * kPi), b->render, (hipStream_t)stream);
+    hipError_t e = ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows, b->d_wide,
+                                           b->n_wide, t / (2.0 * kPi), b->render, (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "batch update launch");
 }
 
@@ -1020,11 +1026,13 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream)
 {
     int rc = batch_ready(b, t);
     if (rc) return rc;
-    // The graph holds two kernel nodes (columns -> rows) and nothing else; t
-    // reaches the column node through its parameters on every replay.
+    // The graph holds kernel nodes only: columns, then the row pass (and the
+    // row pass of the N = 4096 members, beside it); t reaches the column node
+    // through its parameters on every replay.
     double              t_rev       = t / (2.0 * kPi);
     void*               cols_args[] = {&b->d_members, &b->d_cols, &t_rev};
     void*               rows_args[] = {&b->d_members, &b->d_rows};
+    void*               wide_args[] = {&b->d_members, &b->d_wide};
     hipKernelNodeParams kc{};
     kc.func           = const_cast<void*>(ocn::batch_cols_kernel());
     kc.gridDim        = dim3(b->n_cols);
@@ -1034,21 +1042,28 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream)
     kc.extra          = nullptr;
     hipError_t e;
     if (!b->exec) {
-        hipKernelNodeParams kr = kc;
+        hipKernelNodeParams kr = kc, kw = kc;
         kr.func                = const_cast<void*>(ocn::batch_rows_kernel(b->render));
         kr.gridDim             = dim3(b->n_rows);
         kr.blockDim            = dim3(ocn::kBatchRowsThreads);
         kr.kernelParams        = rows_args;
+        kw.func                = const_cast<void*>(ocn::batch_wide_rows_kernel(b->render));
+        kw.gridDim             = dim3(b->n_wide);
+        kw.blockDim            = dim3(ocn::kBatchWideRowsThreads);
+        kw.kernelParams        = wide_args;
         if ((e = hipGraphCreate(&b->graph, 0)) != hipSuccess) {
             b->graph = nullptr;
             return hip_fail(e, "hipGraphCreate");
         }
-        if ((e = hipGraphAddKernelNode(&b->node_cols, b->graph, nullptr, 0, &kc)) != hipSuccess ||
-            (e = hipGraphAddKernelNode(&b->node_rows, b->graph, &b->node_cols, 1, &kr)) != hipSuccess)
-            return hip_fail(e, "hipGraphAddKernelNode");
-        if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) {
-            b->exec = nullptr;
-            return hip_fail(e, "hipGraphInstantiate");
+        e = hipGraphAddKernelNode(&b->node_cols, b->graph, nullptr, 0, &kc);
+        if (e == hipSuccess && b->n_rows > 0) e = hipGraphAddKernelNode(&b->node_rows, b->graph, &b->node_cols, 1, &kr);
+        if (e == hipSuccess && b->n_wide > 0) e = hipGraphAddKernelNode(&b->node_wide, b->graph, &b->node_cols, 1, &kw);
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+            b->exec  = nullptr;
+            return hip_fail(e, "batch graph construction");
         }
     } else if ((e = hipGraphExecKernelNodeSetParams(b->exec, b->node_cols, &kc)) != hipSuccess) {
         return hip_fail(e, "hipGraphExecKernelNodeSetParams");

@@ -259,10 +259,11 @@ int ocn_copy_fields_to_host(ocn_handle* h, float* height, float* dx, float* dz, 
  *
  * No reference counterpart: the reference simulates one N=256 ocean
  * (main.cpp:17).  A batch holds many oceans / cascades, each with its own
- * ocn_params (N a power of two in [64, 2048], different N may mix; nearest-tap
+ * ocn_params (N a power of two in [64, 4096], different N may mix; nearest-tap
  * normals only), and computes one frame of every member in two launches: a
  * column pass over all members' strips, then a row pass that also writes the
- * normal map and Jacobian.  Each member's outputs equal what ocn_update_into
+ * normal map and Jacobian (members of N = 4096 walk their rows in a third
+ * launch beside it).  Each member's outputs equal what ocn_update_into
  * of a single ocean with the same parameters writes, to float rounding.  For
  * callers that animate many oceans or several cascades per ocean. */
 
@@ -276,10 +277,11 @@ int ocn_batch_get_params(const ocn_batch* b, int member, ocn_params* p);
 /* tilde_h0_k of every member (Philox noise, as ocn_tilde_h0_k). */
 int ocn_batch_tilde_h0_k(ocn_batch* b, void* stream);
 
-/* One frame at time t of every member: two kernel launches on `stream`. */
+/* One frame at time t of every member: two kernel launches on `stream`
+ * (three with members of N = 4096). */
 int ocn_batch_update(ocn_batch* b, double t, void* stream);
 
-/* The same frame through a HIP graph of the two kernels, built on the first
+/* The same frame through a HIP graph of those kernels, built on the first
  * call and replayed afterwards with t set through the graph's node
  * parameters (the graph holds kernel nodes only).  Calls on one batch must
  * not overlap in time with each other or with ocn_batch_update. */

@@ -102,14 +102,14 @@ def _batch_create(members):
     return rc, h.value
 
 
-@pytest.mark.parametrize("case", ["empty", "N4096", "N32", "bilinear", "bad_member"])
+@pytest.mark.parametrize("case", ["empty", "N8192", "N32", "bilinear", "bad_member"])
 def test_batch_create_rejects_bad_members_before_hip(case):
     good = pyocean.make_params(N=256)
     members = [good, pyocean.make_params(N=512)]
     if case == "empty":
         members = []
-    elif case == "N4096":
-        members[1] = pyocean.make_params(N=4096)
+    elif case == "N8192":
+        members[1] = pyocean.make_params(N=8192)
     elif case == "N32":
         members[1] = pyocean.make_params(N=32)
     elif case == "bilinear":

@@ -198,3 +198,39 @@ def test_batch_render_export_matches_oracle_stencils(hip):
                            abs_rel_tol=1e-5)
         assert_field_close("member %d render normals" % i, rend[i]["render_normals"], n_own, rel_tol=1e-6,
                            abs_rel_tol=1e-5)
+
+
+def test_batch_with_n4096_members(hip):
+    """Members of N = 4096, whose rows run in a launch of their own, next to a
+    small one: every member against the single-ocean path (at N = 4096 with
+    other kernels, so at the oracle tolerance; that path is pinned at this N
+    by direct-DFT samples and Parseval), the graph (three kernel nodes) equal
+    to eager launches, and the render export of an N = 4096 member against the
+    oracle's stencils on its own fields."""
+    members = [pyocean.make_params(N=4096, L=4000.0, wind_speed=30.0, wind_dir=(0.8, 0.6), choppiness=1.2,
+                                   ocean_id=9, displacement_scale=0.7),
+               pyocean.make_params(N=256, L=500.0, wind_speed=15.0, ocean_id=10),
+               pyocean.make_params(N=4096, L=2000.0, wind_speed=20.0, wind_dir=(-0.3, 1.0), ocean_id=11,
+                                   conj_mode=1)]
+    t = 4.1
+    _say("batch with N=4096 members")
+    with pyocean.Batch(members) as b:
+        b.tilde_h0_k()
+        b.update(t)
+        got = [b.download_fields(i) for i in range(len(members))]
+        b.update_graph(t + 0.7)
+        b.update_graph(t)
+        graph = [b.download_fields(i) for i in range(len(members))]
+        b.set_render_export(True)
+        b.update(t)
+        rend = b.download_render(0)
+    for i, m in enumerate(members):
+        for k in FIELDS:
+            np.testing.assert_array_equal(graph[i][k], got[i][k], err_msg="member %d %s graph" % (i, k))
+        _, one = _single(m, t)
+        _say("  member %d N=%d vs single" % (i, m.N))
+        _assert_same_as_single(i, m, got[i], one)
+    m, g = members[0], got[0]
+    d_own, n_own = oracle.render(m.N, m.L, m.choppiness, m.displacement_scale, g["height"], g["dx"], g["dz"])
+    assert_field_close("N=4096 displacement", rend["displacement"], d_own, rel_tol=1e-6, abs_rel_tol=1e-5)
+    assert_field_close("N=4096 render normals", rend["render_normals"], n_own, rel_tol=1e-6, abs_rel_tol=1e-5)
