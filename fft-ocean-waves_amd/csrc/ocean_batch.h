@@ -20,8 +20,10 @@ constexpr int kBatchMaxN        = 4096;
 constexpr int kBatchWideN           = 4096;
 constexpr int kBatchWideRowsThreads = kBatchWideN / 8;
 
-// Row-pass radix and band height per N: radix 8 keeps the fused row kernel
-// at two waves per SIMD; short bands at small N keep a row block busy.
+// Row-pass radix and default band height per N: radix 8 keeps the fused row
+// kernel at two waves per SIMD; short bands at small N keep a row block busy.
+// The band height of each member can be changed (ocn_batch_set_schedule,
+// ocn_batch_autotune) among the heights batch_band_valid accepts.
 __host__ __device__ constexpr int batch_rows_radix(int) { return 8; }
 __host__ __device__ constexpr int batch_rows_band(int N) { return N <= 64 ? 2 : N <= 256 ? 8 : N <= 512 ? 16 : 32; }
 
@@ -43,6 +45,7 @@ struct BatchMember {
     float4*       disp;    // render export (k_batch_rows_render only)
     float4*       rnorm;
     float         dscale;  // displacement_scale
+    int           band;    // rows per band of the row pass
 };
 
 // Work items: int2 (member, strip) for the column pass, (member, first band)
@@ -57,10 +60,19 @@ inline int batch_rows_groups_per_block(int N)
 {
     return N == kBatchWideN ? 1 : kBatchRowsThreads / (N / batch_rows_radix(N));
 }
-inline int batch_rows_items_per_member(int N)
+inline int batch_rows_items_per_member(int N, int band)
 {
-    const int gpb = batch_rows_groups_per_block(N), bands = N / batch_rows_band(N);
+    const int gpb = batch_rows_groups_per_block(N), bands = N / band;
     return (bands + gpb - 1) / gpb;
+}
+// A band height R of the row pass at N: even, at most 32 (the walk is
+// sequential within a band), dividing N, and where groups are wider than a
+// wave (block barriers) a whole number of bands for every row block.
+inline bool batch_band_valid(int N, int R)
+{
+    if (R < 2 || R > 32 || (R & (R - 1)) || N % R) return false;
+    const int T = N / batch_rows_radix(N);
+    return T <= 64 || (N / R) % batch_rows_groups_per_block(N) == 0;
 }
 
 // render: the row pass also writes each member's render export (disp, rnorm).

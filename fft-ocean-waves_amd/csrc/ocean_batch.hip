@@ -6,7 +6,8 @@
 // and outputs).  A frame is two launches over device tables of work items:
 //   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, C>, 768 threads,
 //                 C = 4096/N columns (N/2 at N = 64, whose strips use 384 threads)
-//   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8, R>,     256 threads
+//   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8>,        256 threads
+//                 (bands of the member's R rows)
 //                 (k_batch_rows_render: the same with each member's render export;
 //                 members of N = 4096 walk their bands in k_batch_rows_wide, 512 threads)
 // Both block shapes are the same for every N (C = 4096/N columns per column
@@ -59,8 +60,12 @@ __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip,
 template <int N, bool RENDER>
 __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_band, float2* lds)
 {
-    constexpr int E = batch_rows_radix(N), T = N / E, R = batch_rows_band(N);
+    constexpr int E = batch_rows_radix(N), T = N / E;
     static_assert(T <= kBatchRowsThreads && kBatchRowsThreads % T == 0, "whole groups per row block");
+    // m.band: even, dividing N, and with T > 64 (block barriers) a multiple
+    // of the groups per block in bands, so that every group walks one
+    // (batch_band_valid, checked on the host)
+    const int R = m.band;
     using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
     // a group of T >= 64 threads is made of whole waves: its index is
     // wave-uniform, and reading it from the first lane keeps the band scalar
@@ -71,9 +76,9 @@ __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_
     // Z2 rows paired too: the walk stays within 256 registers a lane, so two
     // 256-thread blocks still share a CU (the render variant reads them alone
     // to keep its extra outputs within the same budget)
-    rows_nrm_band<N, E, R, Sync, RENDER, !RENDER>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
-                                                  E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, m.lambda,
-                                                  m.inv2d, m.disp, m.rnorm, m.dscale);
+    rows_nrm_band<N, E, Sync, RENDER, !RENDER>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
+                                               E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, R, m.lambda,
+                                               m.inv2d, m.disp, m.rnorm, m.dscale);
 }
 
 template <bool RENDER>
@@ -135,13 +140,13 @@ template <bool RENDER>
 __device__ __forceinline__ void batch_wide_rows(const BatchMember* __restrict__ members,
                                                 const int2* __restrict__ items, float2* lds)
 {
-    constexpr int N = kBatchWideN, E = 8, R = batch_rows_band(N);
+    constexpr int N = kBatchWideN, E = 8;
     static_assert(N / E == kBatchWideRowsThreads, "one radix-8 group per block");
     const int2        it = items[xcd_logical(blockIdx.x, gridDim.x)];
     const BatchMember m  = members[it.x];
-    rows_nrm_band<N, E, R, BlockSync, RENDER, false>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac, m.tw8,
-                                                     lds, (int)threadIdx.x, it.y, m.lambda, m.inv2d, m.disp, m.rnorm,
-                                                     m.dscale);
+    rows_nrm_band<N, E, BlockSync, RENDER, false>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac, m.tw8, lds,
+                                                  (int)threadIdx.x, it.y, m.band, m.lambda, m.inv2d, m.disp, m.rnorm,
+                                                  m.dscale);
 }
 
 __global__ __launch_bounds__(kBatchWideRowsThreads, 2) void k_batch_rows_wide(

@@ -402,6 +402,42 @@ std::vector<int2> balance_items(const std::vector<int2>& items, Cost cost)
     return out;
 }
 
+// Row work items from every member's band height: (member, first band) of
+// the shared row launch, balanced over the XCDs, and (member, band) of the
+// wide launch.  Replaces the device tables and drops a built graph (its row
+// nodes change shape); no launch may be reading the old tables.
+int batch_build_rows(ocn_batch* b)
+{
+    std::vector<int2> rows, wide;
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        const int N = b->params[i].N, R = b->members[i].band, gpb = ocn::batch_rows_groups_per_block(N);
+        for (int k = 0; k < ocn::batch_rows_items_per_member(N, R); ++k)
+            (N == ocn::kBatchWideN ? wide : rows).push_back(make_int2((int)i, k * gpb));
+    }
+    rows = balance_items(rows, [&](int2 it) {
+        const int N = b->params[it.x].N, R = b->members[it.x].band;
+        const int bands = std::min(ocn::batch_rows_groups_per_block(N), N / R - it.y);
+        return (double)bands * (R + 2) * N;  // rows transformed by the item
+    });
+    for (int2** d : {&b->d_rows, &b->d_wide})
+        if (*d) {
+            (void)hipFree(*d);
+            *d = nullptr;
+        }
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec  = nullptr;
+        b->graph = nullptr;
+    }
+    b->n_rows = (int)rows.size();
+    b->n_wide = (int)wide.size();
+    int rc    = OCN_OK;
+    if (!rows.empty()) rc = upload_vector(&b->d_rows, rows, "batch row items");
+    if (!rc && !wide.empty()) rc = upload_vector(&b->d_wide, wide, "batch wide row items");
+    return rc;
+}
+
 int check_batch(const ocn_batch* b)
 {
     if (!b) return fail(OCN_EINVAL, "batch is NULL");
@@ -921,7 +957,7 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
     }
     if (!rc) {
         char*             q = b->pool;
-        std::vector<int2> cols, rows, wide;
+        std::vector<int2> cols;
         for (int i = 0; i < count; ++i) {
             const ocn_params& p  = b->params[i];
             const size_t      nn = (size_t)p.N * p.N;
@@ -940,24 +976,14 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
             m.lambda  = (float)p.choppiness;
             m.inv2d   = (float)((double)p.N / (2.0 * p.L));
             m.dscale  = (float)p.displacement_scale;
+            m.band    = ocn::batch_rows_band(p.N);
             b->members.push_back(m);
             for (int k = 0; k < ocn::batch_cols_items_per_member(p.N); ++k) cols.push_back(make_int2(i, k));
-            const int gpb = ocn::batch_rows_groups_per_block(p.N);
-            for (int k = 0; k < ocn::batch_rows_items_per_member(p.N); ++k)
-                (p.N == ocn::kBatchWideN ? wide : rows).push_back(make_int2(i, k * gpb));
         }
-        rows = balance_items(rows, [&](int2 it) {
-            const int N = b->params[it.x].N, R = ocn::batch_rows_band(N);
-            const int bands = std::min(ocn::batch_rows_groups_per_block(N), N / R - it.y);
-            return (double)bands * (R + 2) * N;  // rows transformed by the item
-        });
         b->n_cols = (int)cols.size();
-        b->n_rows = (int)rows.size();
-        b->n_wide = (int)wide.size();
         if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) &&
-            !(rc = upload_vector(&b->d_cols, cols, "batch column items")) && !rows.empty())
-            rc = upload_vector(&b->d_rows, rows, "batch row items");
-        if (!rc && !wide.empty()) rc = upload_vector(&b->d_wide, wide, "batch wide row items");
+            !(rc = upload_vector(&b->d_cols, cols, "batch column items")))
+            rc = batch_build_rows(b);
     }
     if (rc) {
         batch_free(b);
@@ -1115,6 +1141,85 @@ int ocn_batch_set_member_fields(ocn_batch* b, int member, const ocn_fields* f)
     }
     e = hipMemcpy(b->d_members + member, &m, sizeof(m), hipMemcpyHostToDevice);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpy(batch member)");
+}
+
+// Band height of the row pass for every member of size N (the batch's
+// schedule at that N; the outputs do not depend on it).
+int ocn_batch_set_schedule(ocn_batch* b, int N, int rows_band)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    bool present = false;
+    for (const ocn_params& p : b->params) present |= p.N == N;
+    if (!present) return fail(OCN_EINVAL, "no member has N=%d", N);
+    if (!ocn::batch_band_valid(N, rows_band))
+        return fail(OCN_EINVAL, "rows_band %d is not a valid band height at N=%d", rows_band, N);
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();  // no launch may read the tables while they change
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (b->params[i].N == N) b->members[i].band = rows_band;
+    e = hipMemcpy(b->d_members, b->members.data(), b->members.size() * sizeof(ocn::BatchMember),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(batch members)");
+    return batch_build_rows(b);
+}
+
+int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!rows_band) return fail(OCN_EINVAL, "rows_band is NULL");
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    *rows_band = b->members[member].band;
+    return OCN_OK;
+}
+
+// Per N present, every valid band height timed over `reps` whole frames of
+// the batch (HIP events on `stream`, one warm-up frame first); the fastest
+// is kept, N by N.
+int ocn_batch_autotune(ocn_batch* b, int reps, void* stream)
+{
+    int rc = batch_ready(b, 0.0);
+    if (rc) return rc;
+    if (reps < 1 || reps > 1000) return fail(OCN_EINVAL, "reps must be in [1, 1000]");
+    std::vector<int> sizes;
+    for (const ocn_params& p : b->params)
+        if (std::find(sizes.begin(), sizes.end(), p.N) == sizes.end()) sizes.push_back(p.N);
+    hipStream_t s  = (hipStream_t)stream;
+    hipEvent_t  e0 = nullptr, e1 = nullptr;
+    hipError_t  e  = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    for (int N : sizes) {
+        int   best_band = 0;
+        float best_ms   = 0.0f;
+        for (int R = 2; R <= 32 && e == hipSuccess && !rc; R *= 2) {
+            if (!ocn::batch_band_valid(N, R)) continue;
+            if ((rc = ocn_batch_set_schedule(b, N, R))) break;
+            auto frame = [&](int i) {
+                return ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows, b->d_wide,
+                                               b->n_wide, 0.25 * i, b->render, s);
+            };
+            e = frame(0);
+            if (e == hipSuccess) e = hipEventRecord(e0, s);
+            for (int i = 1; i <= reps && e == hipSuccess; ++i) e = frame(i);
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess && (best_band == 0 || ms < best_ms)) {
+                best_band = R;
+                best_ms   = ms;
+            }
+        }
+        if (e != hipSuccess || rc) break;
+        if ((rc = ocn_batch_set_schedule(b, N, best_band))) break;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return e == hipSuccess ? OCN_OK : hip_fail(e, "batch autotune");
 }
 
 // grid_tes.glsl:58-62 / main.cpp:1632-1633 for every member: the render export

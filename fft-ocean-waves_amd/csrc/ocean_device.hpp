@@ -409,7 +409,8 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
 // ------------------------------------------------------------------ row pass + normals + Jacobian: one band
 
 // Row transforms fused with the normal map (nearest taps) and the Jacobian
-// for one band of R rows [y0, y0 + R), y0 = band*R even.  A group of T = N/E
+// for one band of R rows [y0, y0 + R), y0 = band*R even (R even, dividing N;
+// the caller checks).  A group of T = N/E
 // threads (thread j holds x = j + r*T) walks the band with a three-row window
 // in registers: for each new row it transforms Z1 (height + i Dx) and, on
 // even rows, the Z2 row holding Dz of that row and the next, then applies
@@ -422,17 +423,16 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
 // normals or jac may be null.  RENDER adds the render export (disp, rnorm,
 // either may be null): (-l Dx, s h, -l Dz, J) and the unit normal of the
 // displaced surface, dP/dv x dP/du.
-template <int N, int E, int R, class Sync, bool RENDER = false, bool PAIR2 = false>
+template <int N, int E, class Sync, bool RENDER = false, bool PAIR2 = false>
 __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, const float2* __restrict__ z2,
                                               float* __restrict__ height, float* __restrict__ dx,
                                               float* __restrict__ dz, float4* __restrict__ normals,
                                               float* __restrict__ jac, const float2* __restrict__ tw, float2* buf,
-                                              int j, int band, float lambda, float inv2d,
+                                              int j, int band, int R, float lambda, float inv2d,
                                               float4* __restrict__ disp = nullptr, float4* __restrict__ rnorm = nullptr,
                                               float dscale = 0.0f)
 {
     constexpr int T = N / E;
-    static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
     constexpr float scale = 1.0f / ((float)N * (float)N);
     const int y0 = band * R;
 

@@ -34,7 +34,8 @@ EXPORTS = [
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
     "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_batch_set_render_export",
-    "ocn_batch_set_member_fields", "ocn_fit_upload_targets",
+    "ocn_batch_set_member_fields", "ocn_batch_set_schedule", "ocn_batch_get_schedule", "ocn_batch_autotune",
+    "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_external_import", "ocn_external_release",
@@ -132,6 +133,9 @@ def lib():
             "ocn_batch_copy_fields_to_host": [vp, i32] + [vp] * 6,
             "ocn_batch_set_render_export": [vp, i32],
             "ocn_batch_set_member_fields": [vp, i32, F],
+            "ocn_batch_set_schedule": [vp, i32, i32],
+            "ocn_batch_get_schedule": [vp, i32, ctypes.POINTER(i32)],
+            "ocn_batch_autotune": [vp, i32, vp],
             "ocn_fit_upload_targets": [vp, i32, vp, vp],
             "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
             "ocn_fit_destroy": [vp],
@@ -578,6 +582,17 @@ class Batch:
 
     def set_render_export(self, enable=True):
         check(lib().ocn_batch_set_render_export(self.h, 1 if enable else 0))
+
+    def set_schedule(self, N, rows_band):
+        check(lib().ocn_batch_set_schedule(self.h, int(N), int(rows_band)))
+
+    def schedule(self, member):
+        r = ctypes.c_int()
+        check(lib().ocn_batch_get_schedule(self.h, int(member), ctypes.byref(r)))
+        return r.value
+
+    def autotune(self, reps=10):
+        check(lib().ocn_batch_autotune(self.h, int(reps), self.stream))
 
     def set_member_fields(self, member, fields):
         """Outputs of one member written at caller-provided device addresses."""

@@ -154,6 +154,15 @@ public:
     void       update_graph(double t) { check(ocn_batch_update_graph(b_, t, stream_)); }
     // Every member's render export (fields().displacement / render_normals) on or off.
     void       set_render_export(bool enable) { check(ocn_batch_set_render_export(b_, enable ? 1 : 0)); }
+    // Rows per band of the row pass for the members of size N, and the autotuner over it.
+    void set_schedule(int N, int rows_band) { check(ocn_batch_set_schedule(b_, N, rows_band)); }
+    int  schedule(int member) const
+    {
+        int r = 0;
+        check(ocn_batch_get_schedule(b_, member, &r));
+        return r;
+    }
+    void autotune(int reps = 10) { check(ocn_batch_autotune(b_, reps, stream_)); }
     // One member's outputs at caller-provided device addresses (e.g. ExternalMemory).
     void set_member_fields(int member, const ocn_fields& f) { check(ocn_batch_set_member_fields(b_, member, &f)); }
     ocn_fields fields(int member) const

@@ -287,6 +287,21 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream);
  * not overlap in time with each other or with ocn_batch_update. */
 int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
 
+/* The batch's schedule: rows per band of the row pass for the members of
+ * size N (default 2 at N = 64, 8 up to 256, 16 at 512, 32 above).  A height
+ * is valid when it is a power of two in [2, 32] dividing N and, at N = 1024
+ * and 2048 (row groups wider than a wave), leaves every row block whole
+ * bands.  Short bands give more row blocks for batches of few members, long
+ * ones fewer halo rows.  The outputs do not depend on it.  set_schedule waits
+ * for the device to be idle. */
+int ocn_batch_set_schedule(ocn_batch* b, int N, int rows_band);
+int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band);
+/* For each N in the batch, times every valid band height over `reps` whole
+ * frames on `stream` (after one warm-up frame) and keeps the fastest.  The
+ * outputs afterwards hold a frame of the tuning runs.  Needs the spectra
+ * (ocn_batch_tilde_h0_k). */
+int ocn_batch_autotune(ocn_batch* b, int reps, void* stream);
+
 /* Device pointers of one member's outputs (valid until ocn_batch_destroy;
  * displacement and render_normals are NULL unless the render export is on)
  * and its h0 spectrum (float4[N*N]). */

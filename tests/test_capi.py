@@ -133,6 +133,9 @@ def test_batch_null_pointers_rejected():
     assert L.ocn_batch_size(None, None) == pyocean.OCN_EINVAL
     assert L.ocn_batch_set_render_export(None, 1) == pyocean.OCN_EINVAL
     assert L.ocn_batch_set_member_fields(None, 0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_set_schedule(None, 256, 8) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_get_schedule(None, 0, None) == pyocean.OCN_EINVAL
+    assert L.ocn_batch_autotune(None, 5, None) == pyocean.OCN_EINVAL
     assert L.ocn_batch_destroy(None) == pyocean.OCN_OK
 
 

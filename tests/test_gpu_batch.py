@@ -234,3 +234,44 @@ def test_batch_with_n4096_members(hip):
     d_own, n_own = oracle.render(m.N, m.L, m.choppiness, m.displacement_scale, g["height"], g["dx"], g["dz"])
     assert_field_close("N=4096 displacement", rend["displacement"], d_own, rel_tol=1e-6, abs_rel_tol=1e-5)
     assert_field_close("N=4096 render normals", rend["render_normals"], n_own, rel_tol=1e-6, abs_rel_tol=1e-5)
+
+
+def test_batch_band_heights_give_identical_frames_and_autotune(hip):
+    """The band height of the row pass only regroups the rows: every height
+    at every N gives bit-identical frames, invalid heights are refused, and
+    the autotuner leaves a valid height per N with the graph (rebuilt for the
+    new row items) reproducing the frame."""
+    members = [pyocean.make_params(**kw) for kw in MIXED]
+    sizes = sorted({m.N for m in members})
+    t = 1.9
+    with pyocean.Batch(members) as b:
+        b.tilde_h0_k()
+        defaults = {m.N: b.schedule(i) for i, m in enumerate(members)}
+        b.update(t)
+        ref = [b.download_fields(i) for i in range(len(members))]
+        for N in sizes:
+            for R in (2, 4, 8, 16, 32):
+                b.set_schedule(N, R)
+                b.update(t)
+                _say("  N=%d band %d" % (N, R))
+                for i, m in enumerate(members):
+                    assert b.schedule(i) == (R if m.N == N else defaults[m.N])
+                    if m.N != N:
+                        continue
+                    got = b.download_fields(i)
+                    for k in FIELDS:
+                        np.testing.assert_array_equal(got[k], ref[i][k], err_msg="N=%d R=%d %s" % (N, R, k))
+            b.set_schedule(N, defaults[N])
+        for N, R in ((256, 3), (256, 64), (256, 1), (100, 8)):
+            with pytest.raises(pyocean.OceanError):
+                b.set_schedule(N, R)
+        b.autotune(reps=3)
+        tuned = [b.schedule(i) for i in range(len(members))]
+        _say("  autotuned bands %s" % dict(zip([m.N for m in members], tuned)))
+        b.update_graph(t + 1.0)
+        b.update_graph(t)
+        after = [b.download_fields(i) for i in range(len(members))]
+    for i, m in enumerate(members):
+        assert tuned[i] in (2, 4, 8, 16, 32) and m.N % tuned[i] == 0
+        for k in FIELDS:
+            np.testing.assert_array_equal(after[i][k], ref[i][k], err_msg="member %d %s after autotune" % (i, k))
