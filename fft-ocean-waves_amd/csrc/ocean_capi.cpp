@@ -140,8 +140,8 @@ std::pair<int, int> default_schedule(int N, int nframes)
     case 64: cv = 9; rv = 1; break;
     case 128: cv = batch ? 9 : 8; rv = batch ? 2 : 1; break;
     case 256: cv = batch ? 9 : 7; rv = 1; break;
-    case 512: cv = batch ? 9 : 8; rv = batch ? 2 : 1; break;
-    case 1024: cv = batch ? 10 : 9; rv = batch ? 5 : 1; break;
+    case 512: cv = batch ? 5 : 8; rv = batch ? 4 : 1; break;
+    case 1024: cv = 9; rv = batch ? 5 : 1; break;
     case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 1; break;
     default: cv = 4; rv = batch ? 5 : 0; break;  // 4096: radix-8 rows would need 1536 threads
     }

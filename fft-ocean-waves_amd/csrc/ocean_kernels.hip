@@ -98,7 +98,7 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
                                                     SpecConst sc, FrameBatch fb)
 {
     constexpr int T = N / E;
-    constexpr int G = strip_group<C>();
+    constexpr int G = strip_group<N, C>();
     __shared__ float2 lds[C * lds_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
     const int rest  = L / G;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __r
                                                                FrameBatch fb)
 {
     constexpr int T = N / 16;
-    constexpr int G = strip_group<C>();
+    constexpr int G = strip_group<N, C>();
     static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
     __shared__ float2 lds[3 * C * lds_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(2 * C * (N / 16)) void k_cols_lds2(const float4* __
                                                                 const float2* __restrict__ tw, SpecConst sc,
                                                                 FrameBatch fb)
 {
-    constexpr int G = strip_group<C>();
+    constexpr int G = strip_group<N, C>();
     __shared__ float2 lds[2 * C * lds_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
     const int rest  = L / G;
@@ -472,21 +472,21 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
     if constexpr (!cols_fits<N, E, C, K>()) {
         return hipErrorInvalidValue;
     } else if constexpr (K == 1) {
-        constexpr int G      = strip_group<C>();
+        constexpr int G      = strip_group<N, C>();
         constexpr int strips = (N / 2) / C + 1;
         constexpr int groups = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols_lds<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
                            a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 2) {
-        constexpr int G      = strip_group<C>();
+        constexpr int G      = strip_group<N, C>();
         constexpr int strips = (N / 2) / C + 1;
         constexpr int groups = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols_lds2<N, C>), dim3(groups * G * a.nframes), dim3(2 * C * (N / 16)), 0, s, a.h0,
                            a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else {
-        constexpr int G       = strip_group<C>();
+        constexpr int G       = strip_group<N, C>();
         constexpr int strips  = (N / 2) / C + 1;  // half plane tx in [0, N/2]
         constexpr int groups  = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols<N, E, C>), dim3(groups * G * a.nframes), dim3(C * (N / E)), 0, s, a.h0, a.z1,

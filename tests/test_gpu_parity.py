@@ -380,7 +380,7 @@ def test_full_size_multiframe_launch(hip, N):
 
 # ---------------------------------------------------------------- schedules
 
-@pytest.mark.parametrize("N", [64, 256, 1024, 2048])
+@pytest.mark.parametrize("N", [64, 256, 512, 1024, 2048])
 def test_every_schedule_variant_matches_oracle(hip, N):
     kw = dict(c2_kwargs(), N=N, L=N * 2.0)
     lp, op = both_params(**kw)
