@@ -77,6 +77,19 @@ static int batch_and_fit()
         std::printf("batch member %d N=%d: max|height| %.4e, max diff vs single %.3e\n", i, i == 0 ? a.N : b.N, hmax,
                     d);
     }
+    // the render export, a band height and the autotuner leave the frame unchanged
+    const ocean::HostFrame before = batch.download(1);
+    batch.set_render_export(true);
+    batch.set_schedule(128, 4);
+    batch.update(1.25);
+    const bool export_on = batch.fields(0).displacement != nullptr && batch.schedule(1) == 4;
+    const bool same_band = batch.download(1).height == before.height;
+    batch.autotune(2);
+    batch.update_graph(1.25);
+    const bool same_tuned = batch.download(1).height == before.height;
+    std::printf("batch: render export %s, band 4 frame %s, autotuned band %d frame %s\n", export_on ? "on" : "OFF",
+                same_band ? "identical" : "DIFFERS", batch.schedule(1), same_tuned ? "identical" : "DIFFERS");
+    if (!(export_on && same_band && same_tuned)) return 1;
     // noiseless fit: targets are heights of member a at two times
     const std::vector<double> times = {0.5, 1.5};
     std::vector<float>        targets;
