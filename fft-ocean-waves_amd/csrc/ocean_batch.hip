@@ -75,7 +75,9 @@ __device__ __forceinline__ void batch_rows_item(const BatchMember& m, int first_
     if (band >= N / R) return;  // whole group (wave-aligned when GPB > 1)
     // Z2 rows paired too: the walk stays within 256 registers a lane, so two
     // 256-thread blocks still share a CU (the render variant reads them alone
-    // to keep its extra outputs within the same budget)
+    // to keep its extra outputs within the same budget).  Every band walks
+    // downwards: the upward walk of odd bands (ALT) made the 192-cascade frame
+    // 0.7% slower in this kernel, which holds every N in one code object.
     rows_nrm_band<N, E, Sync, RENDER, !RENDER>(m.z1, m.z2, m.height, m.dx, m.dz, m.normals, m.jac,
                                                E == 8 ? m.tw8 : m.tw16, lds + g * lds_len(N), j, band, R, m.lambda,
                                                m.inv2d, m.disp, m.rnorm, m.dscale);

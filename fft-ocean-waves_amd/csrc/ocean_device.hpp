@@ -432,7 +432,7 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
 // normals or jac may be null.  RENDER adds the render export (disp, rnorm,
 // either may be null): (-l Dx, s h, -l Dz, J) and the unit normal of the
 // displaced surface, dP/dv x dP/du.
-template <int N, int E, class Sync, bool RENDER = false, bool PAIR2 = false>
+template <int N, int E, class Sync, bool RENDER = false, bool PAIR2 = false, bool ALT = false>
 __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, const float2* __restrict__ z2,
                                               float* __restrict__ height, float* __restrict__ dx,
                                               float* __restrict__ dz, float4* __restrict__ normals,
@@ -443,30 +443,57 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 {
     constexpr int T = N / E;
     constexpr float scale = 1.0f / ((float)N * (float)N);
-    const int y0 = band * R;
+    // With ALT, odd bands walk their rows upwards, so that the halo tiles two
+    // neighbouring bands share are read by both near the same time (both
+    // walks end at their common boundary and start at the band boundaries
+    // they share with the other neighbours) and come from one L2 fetch.
+    // The walk runs over walk rows w = y0 .. y0+R-1 in its own order: walk row
+    // w is grid row arow(w) = w, or N-1-w on an upward walk; both take the
+    // unwrapped -1 .. N.  The flip maps a two-row tile onto a two-row tile
+    // with its halves exchanged (N-1 is odd), and a Z2 row (rows 2u, 2u+1 in
+    // its real and imaginary parts) onto a Z2 row with the parts exchanged, so
+    // the walk below reads tiles and Z2 parts in walk order.  Signs, output
+    // rows and stencils use grid rows, with the neighbours named by grid
+    // position, so that every row's arithmetic is that of a downward walk
+    // (frames are bit-identical either way).  N=1024 / 2048 frames -0.9%; below
+    // that the selects cost more than the halo reads (N=512 +2%).
+    const bool flip = ALT && (band & 1) != 0;
+    const int  fm   = flip ? -1 : 0;
+    const int  y0   = flip ? N - (band + 1) * R : band * R;
+    auto arow  = [&](int w) { return (w ^ fm) & (N - 1); };
+    auto arow2 = [&](int u) { return (u ^ fm) & (N / 2 - 1); };  // Z2 rows
 
-    // Rows are read from the two-row tiles of Z1 / Z2: load_row takes one row
-    // (8 bytes of every 16), load_pair both rows of the tile holding row y
-    // (y even) -- row y into v, row y+1 into w -- with one 16-byte load per
+    // Rows are read from the two-row tiles of Z1 / Z2: load_row takes one grid
+    // row (8 bytes of every 16), load_pair the tile holding walk rows w, w+1
+    // (w even) -- walk row w into v, w+1 into w -- with one 16-byte load per
     // position.  Thread j holds x = j + r*T (natural order after the transform).
     auto load_row = [&](const float2* z, int y, float2 (&v)[E]) {
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = z[zidx<N>(y, zpos_elem<N, E>(j, r))];
     };
-    auto load_pair = [&](const float2* z, int y, float2 (&v)[E], float2 (&w)[E]) {
-        const float4* t = ztile<N>(z, y >> 1);
+    auto load_pair = [&](const float2* z, int tile, float2 (&v)[E], float2 (&w)[E]) {
+        const float4* t = ztile<N>(z, tile);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
-            const float4 q = t[zpos_elem<N, E>(j, r)];
-            v[r] = make_float2(q.x, q.y);
-            w[r] = make_float2(q.z, q.w);
+            const float4 q  = t[zpos_elem<N, E>(j, r)];
+            const float2 lo = make_float2(q.x, q.y), hi = make_float2(q.z, q.w);
+            v[r] = flip ? hi : lo;
+            w[r] = flip ? lo : hi;
         }
     };
     auto fft = [&](float2 (&v)[E]) { fft_inverse<N, E>(v, j, buf, tw, Sync{}); };
-    // (-1)^(x+y) / N^2; x = j + r*T and T is even, so the parity is that of j + y
+    // a transformed Z2 row in walk order: even walk row in .x, odd in .y
+    auto fft2 = [&](float2 (&v)[E]) {
+        fft(v);
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = flip ? make_float2(v[r].y, v[r].x) : v[r];
+    };
+    // (-1)^(x+y) / N^2 of grid row y; x = j + r*T and T is even, so the parity is that of j + y
     auto sgn = [&](int y) { return ((j + y) & 1) ? -scale : scale; };
+    // d/dz by central differences: walk neighbours (next - previous) times this
+    const float vinv = flip ? -inv2d : inv2d;
 
-    // The band reads Z1 rows y0-1 .. y0+R and Z2 rows (y0-1)/2 .. (y0+R)/2
+    // The band reads walk rows y0-1 .. y0+R of Z1 and Z2 rows (y0-1)/2 .. (y0+R)/2
     // (row N wraps to 0), counted unwrapped: an even Z1 row y >= y0 other
     // than the last is loaded together with row y+1, which waits in p1 for
     // the next step.  With PAIR2 the Z2 rows b2 .. e2 are paired the same way
@@ -476,29 +503,29 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
     float hA[E], xA[E], zA[E], hB[E], xB[E], zB[E], zN[E];
     float2 v[E], p1[E], p2[E];
     {
-        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the imaginary part of Z2 row yA >> 1
-        load_row(z2, yA >> 1, v);
-        fft(v);
-        const float sa = sgn(yA);
+        const int yA = (y0 - 1) & (N - 1);  // odd: Dz from the odd part of Z2 row yA >> 1
+        load_row(z2, arow2(yA >> 1), v);
+        fft2(v);
+        const float sa = sgn(arow(yA));
 #pragma unroll
         for (int r = 0; r < E; ++r) zA[r] = sa * v[r].y;
-        load_row(z1, yA, v);
+        load_row(z1, arow(yA), v);
         fft(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             hA[r] = sa * v[r].x;
             xA[r] = sa * v[r].y;
         }
-        const float sb = sgn(y0);
-        if (PAIR2 && (b2 & 1) == 0) load_pair(z2, b2, v, p2);
-        else load_row(z2, b2, v);
-        fft(v);
+        const float sb = sgn(arow(y0));
+        if (PAIR2 && (b2 & 1) == 0) load_pair(z2, arow2(b2) >> 1, v, p2);
+        else load_row(z2, arow2(b2), v);
+        fft2(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             zB[r] = sb * v[r].x;
-            zN[r] = -sb * v[r].y;  // Dz of row y0 + 1 (opposite sign)
+            zN[r] = -sb * v[r].y;  // Dz of walk row y0 + 1 (opposite sign)
         }
-        load_pair(z1, y0, v, p1);  // R >= 2: row y0 + 1 is in the band
+        load_pair(z1, arow(y0) >> 1, v, p1);  // R >= 2: walk row y0 + 1 is in the band
         fft(v);
 #pragma unroll
         for (int r = 0; r < E; ++r) {
@@ -509,22 +536,22 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 
 #pragma unroll 1
     for (int k = 0; k < R; ++k) {
-        const int   y  = y0 + k;
-        const int   yu = y + 1;              // unwrapped, <= N
+        const int   y  = y0 + k;                 // walk row
+        const int   yu = y + 1;                  // unwrapped, <= N
         const int   yC = yu & (N - 1);
-        const float sc = sgn(yC);
+        const float sc = sgn(arow(yu));
         float zC[E];
         if ((yC & 1) == 0) {
-            const int u = yu >> 1;  // Z2 row, unwrapped
+            const int u = yu >> 1;  // Z2 walk row, unwrapped
             if (PAIR2 && (u & 1)) {
 #pragma unroll
                 for (int r = 0; r < E; ++r) v[r] = p2[r];
             } else if (PAIR2 && u < e2) {
-                load_pair(z2, u, v, p2);
+                load_pair(z2, arow2(u) >> 1, v, p2);
             } else {
-                load_row(z2, yC >> 1, v);
+                load_row(z2, arow2(yC >> 1), v);
             }
-            fft(v);
+            fft2(v);
 #pragma unroll
             for (int r = 0; r < E; ++r) {
                 zC[r] = sc * v[r].x;
@@ -538,21 +565,23 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 #pragma unroll
             for (int r = 0; r < E; ++r) v[r] = p1[r];
         } else if (yu < y0 + R) {
-            load_pair(z1, yC, v, p1);
+            load_pair(z1, arow(yC) >> 1, v, p1);
         } else {
-            load_row(z1, yC, v);
+            load_row(z1, arow(yC), v);
         }
-        fft(v);  // v = height + i Dx of row yC (unsigned)
+        fft(v);  // v = height + i Dx of walk row yC (unsigned)
 
-        const size_t row = (size_t)y * N;
+        const size_t row = (size_t)arow(y) * N;
         if (normals) {
             // v_s = h(y-1) + 2 h(y) + h(y+1), d_s = h(y-1) - h(y+1); nx = v_s(x-1) - v_s(x+1),
-            // nz = d_s(x-1) + 2 d_s(x) + d_s(x+1)  (normal_map_cs.glsl:49-51)
+            // nz = d_s(x-1) + 2 d_s(x) + d_s(x+1)  (normal_map_cs.glsl:49-51); up / dn are the
+            // grid rows y-1 / y+1 (walk rows y+1 / y-1 on an upward walk)
             Sync{}();
 #pragma unroll
             for (int r = 0; r < E; ++r) {
                 const float hC = sc * v[r].x;
-                buf[lds_pad(j + r * T)] = make_float2(hA[r] + 2.0f * hB[r] + hC, hA[r] - hC);
+                const float up = flip ? hC : hA[r], dn = flip ? hA[r] : hC;
+                buf[lds_pad(j + r * T)] = make_float2(up + 2.0f * hB[r] + dn, up - dn);
             }
             Sync{}();
 #pragma unroll
@@ -591,8 +620,8 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 const float2 rt   = buf[lds_pad((x + 1) & (N - 1))];
                 const float  dxdx = (rt.x - lf.x) * inv2d;
                 const float  dzdx = (rt.y - lf.y) * inv2d;
-                const float  dxdz = (sc * v[r].y - xA[r]) * inv2d;
-                const float  dzdz = (zC[r] - zA[r]) * inv2d;
+                const float  dxdz = (sc * v[r].y - xA[r]) * vinv;
+                const float  dzdz = (zC[r] - zA[r]) * vinv;
                 const float  J = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
                 if (jac) jac[row + x] = J;
                 if (!RENDER) continue;
@@ -600,7 +629,7 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 if (rnorm) {
                     // n = dP/dv x dP/du with dP/du = (1 - l dDx/dx, s dh/dx, -l dDz/dx),
                     // dP/dv = (-l dDx/dz, s dh/dz, 1 - l dDz/dz); its y component is J
-                    const float dhdz = (sc * v[r].x - hA[r]) * inv2d;
+                    const float dhdz = (sc * v[r].x - hA[r]) * vinv;
                     const float nx = -dscale * (lambda * dhdz * dzdx + (1.0f - lambda * dzdz) * dhdx[r]);
                     const float nz = -dscale * (lambda * dxdz * dhdx[r] + dhdz * (1.0f - lambda * dxdx));
                     const float inv = rsqrtf(fmaf(nx, nx, fmaf(J, J, nz * nz)));
