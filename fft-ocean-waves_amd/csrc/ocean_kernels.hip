@@ -279,11 +279,13 @@ __global__ __launch_bounds__(N / E >= 64 ? N / E : 64, 2) void k_rows_nrm(
     const int f    = L / NB;
     if (f >= fb.nframes) return;  // whole group (a wave part when T <= 64): padding of the last block
     const size_t fo = (size_t)f * fb.out_stride;
-    // Z2 rows paired up to N=1024 (N=512 frames -4%, N=1024 -1%); at N=2048 the
-    // extra registers spill and cost more than the second Z2 fetch.  Odd bands
-    // walk upwards from N=1024 (halo tiles shared in time with the neighbours)
+    // Z2 rows paired up to N=1024 (N=512 frames -4%, N=1024 -1%), and at N=2048
+    // without the render export (247 registers with the upward walk, N=2048
+    // frames -0.4%; the render variant would spill).  Odd bands walk upwards
+    // from N=1024 (halo tiles shared in time with the neighbours)
     static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
-    rows_nrm_band<N, E, Sync, RENDER, (N <= 1024), (N >= 1024)>(
+    constexpr bool PAIR2 = N <= 1024 || (N <= 2048 && !RENDER);
+    rows_nrm_band<N, E, Sync, RENDER, PAIR2, (N >= 1024)>(
         z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo, dx + fo, dz + fo,
         normals ? normals + fo : nullptr, jac ? jac + fo : nullptr, tw, lds + g * lds_len(N), j, band, R, lambda,
         inv2d, disp ? disp + fo : nullptr, rnorm ? rnorm + fo : nullptr, dscale);
