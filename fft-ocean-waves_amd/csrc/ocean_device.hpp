@@ -465,8 +465,9 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
 
     // Rows are read from the two-row tiles of Z1 / Z2: load_row takes one grid
     // row (8 bytes of every 16), load_pair the tile holding walk rows w, w+1
-    // (w even) -- walk row w into v, w+1 into w -- with one 16-byte load per
-    // position.  Thread j holds x = j + r*T (natural order after the transform).
+    // (w even; the caller passes the tile index arow(w) >> 1) -- walk row w
+    // into v, w+1 into w -- with one 16-byte load per position.  Thread j
+    // holds x = j + r*T (natural order after the transform).
     auto load_row = [&](const float2* z, int y, float2 (&v)[E]) {
 #pragma unroll
         for (int r = 0; r < E; ++r) v[r] = z[zidx<N>(y, zpos_elem<N, E>(j, r))];
