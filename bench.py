@@ -306,6 +306,51 @@ def batched_fit_bench(hip, steps=200, n_oceans=256):
     }
 
 
+def render_bench(hip, steps, warmup):
+    """The N=1024 frame sequence with the render export on: besides the five
+    fields, every frame writes the displacement (-l Dx, s h, -l Dz, J) and the
+    displaced-surface normal (float4 each), what the reference's renderer reads
+    from its displacement and normal textures (grid_tes.glsl:60-64).  Frames
+    go to min(steps, 600) output slices as in the default mode."""
+    import pyocean
+    nn = N * N
+    slots = min(FRAMES, max(1, steps))
+    names = ("height", "dx", "dz", "normals", "jacobian", "displacement", "render_normals")
+    sizes = dict(zip(names, (4, 4, 4, 16, 4, 16, 16)))
+    bufs = {k: hip.malloc(slots * nn * sizes[k]) for k in names}
+    stream = hip.stream()
+    oc = pyocean.Ocean(workload_params(), stream=stream)
+    oc.tilde_h0_k()
+    slot_fields = [pyocean.OcnFields(*[bufs[k] + s * nn * sizes[k] for k in names]) for s in range(slots)]
+
+    def frames(f0, count):
+        f = f0
+        while f < f0 + count:
+            s = f % slots
+            n = min(slots - s, f0 + count - f)
+            oc.update_frames(np.arange(f, f + n, dtype=np.float64) / 60.0, slot_fields[s])
+            f += n
+
+    frames(0, warmup)
+    hip.stream_sync(stream)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, stream)
+    frames(0, steps)
+    hip.record(e1, stream)
+    hip.stream_sync(stream)
+    ms = hip.elapsed_ms(e0, e1)
+    oc.close()
+    for v in bufs.values():
+        hip.free(v)
+    hip.stream_destroy(stream)
+    canon = CANON_FRAME + 32  # + displacement and displaced-surface normal, float4 each
+    gbps = canon * nn / (ms * 1e-3 / steps) / 1e9
+    return {"workload": "plain single ocean N=1024 as in the default mode, render export on, %d slices" % slots,
+            "steps": steps, "frames_per_s": round(steps / (ms / 1e3), 2), "us_per_frame": round(ms * 1e3 / steps, 3),
+            "canonical_bytes_per_texel": canon, "canonical_GBps": round(gbps, 1),
+            "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
 def run_copy(hip):
     """Device-to-device copies of 256 MiB (rocprofv3 counter calibration)."""
     nbytes = 256 << 20
@@ -325,7 +370,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit", "batched-fit"], default="frames")
+    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit", "batched-fit", "render"],
+                    default="frames")
     ap.add_argument("--cascade-frames", type=int, default=60,
                     help="frames of the batched-cascade comparison (0 skips it)")
     ap.add_argument("--fit", type=int, default=1, help="1: time the differentiable fit too (0 skips it)")
@@ -348,6 +394,9 @@ def main():
         return
     if args.mode == "batched-fit":
         print(json.dumps({"mode": "batched-fit", **batched_fit_bench(hip)}), flush=True)
+        return
+    if args.mode == "render":
+        print(json.dumps({"mode": "render", **render_bench(hip, args.steps, args.warmup)}), flush=True)
         return
     if args.mode == "cascades":
         print(json.dumps({"mode": "cascades", **cascades_bench(hip, max(1, args.cascade_frames), 10)}), flush=True)
