@@ -142,7 +142,7 @@ std::pair<int, int> default_schedule(int N, int nframes)
     case 256: cv = batch ? 9 : 7; rv = 1; break;
     case 512: cv = batch ? 5 : 8; rv = batch ? 4 : 1; break;
     case 1024: cv = 9; rv = batch ? 5 : 1; break;
-    case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 1; break;
+    case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 2; break;  // single: fused bands of 4 rows (98.3 -> 95.7 us)
     // 4096: k_cols radix 16 with 2 columns per block and fused bands (32 rows
     // for frame batches, 16 for single frames): per frame 419 -> 377 us and
     // 562 -> 444 us against radix-8 columns with k_rows + k_normals
