@@ -20,6 +20,7 @@ differentiable fit (forward and forward+backward cost, a 200-step Adam fit).
   python bench.py --mode cascades  (the batched-cascade comparison alone)
   python bench.py --mode fit       (the differentiable fit alone)
   python bench.py --mode batched-fit (256 oceans fitted together)
+  python bench.py --mode single    (one ocn_update per frame, eager and graph-replayed)
 """
 import argparse
 import json
@@ -306,6 +307,54 @@ def batched_fit_bench(hip, steps=200, n_oceans=256):
     }
 
 
+def single_frame_bench(hip, frames=600, warmup=60, graph_frames=60):
+    """The drop-in use: the reference runs exactly one frame per update()
+    (tilde_h0_t, three butterfly_fft, generate_normal_map; main.cpp:240-244),
+    always into the same textures.  Here one ocn_update per frame into the
+    handle's own fields at t = f/60 on the N=1024 workload, timed with HIP
+    events on the launch stream: eagerly (host launch cost included) and as
+    replays of a graph that holds `graph_frames` consecutive single-frame
+    updates (launch cost amortised)."""
+    import pyocean
+    nn = N * N
+    stream = hip.stream()
+    e0, e1 = hip.event(), hip.event()
+    out = {"workload": "plain single ocean N=1024 as in the default mode, one ocn_update per frame into the "
+                       "handle's own textures (no frame batching), t=f/60 s"}
+    with pyocean.Ocean(workload_params(), stream=stream) as oc:
+        oc.tilde_h0_k()
+        for f in range(warmup):
+            oc.update(f / 60.0)
+        hip.stream_sync(stream)
+        hip.record(e0, stream)
+        for f in range(frames):
+            oc.update(f / 60.0)
+        hip.record(e1, stream)
+        eager_ms = hip.elapsed_ms(e0, e1) / frames
+        g, ex = hip.capture(stream, lambda: [oc.update(f / 60.0) for f in range(graph_frames)])
+        try:
+            for _ in range(3):
+                hip.graph_launch(ex, stream)
+            reps = max(1, frames // graph_frames)
+            hip.record(e0, stream)
+            for _ in range(reps):
+                hip.graph_launch(ex, stream)
+            hip.record(e1, stream)
+            graph_ms = hip.elapsed_ms(e0, e1) / (reps * graph_frames)
+        finally:
+            hip.stream_sync(stream)
+            hip.graph_destroy(g, ex)
+        sched = oc.schedule(1)
+    hip.stream_destroy(stream)
+    best = min(eager_ms, graph_ms)
+    gbps = CANON_FRAME * nn / (best * 1e-3) / 1e9
+    out.update({"frames": frames, "eager_us_per_frame": round(eager_ms * 1e3, 3),
+                "graph_us_per_frame": round(graph_ms * 1e3, 3), "graph_frames_per_replay": graph_frames,
+                "frames_per_s": round(1e3 / best, 1), "canonical_GBps": round(gbps, 1),
+                "hbm_frac": round(gbps / HBM_PEAK_GBPS, 4), "schedule": sched})
+    return out
+
+
 def render_bench(hip, steps, warmup):
     """The N=1024 frame sequence with the render export on: besides the five
     fields, every frame writes the displacement (-l Dx, s h, -l Dz, J) and the
@@ -320,6 +369,7 @@ def render_bench(hip, steps, warmup):
     bufs = {k: hip.malloc(slots * nn * sizes[k]) for k in names}
     stream = hip.stream()
     oc = pyocean.Ocean(workload_params(), stream=stream)
+    oc.reserve_frames(min(LAUNCH_FRAMES, slots))  # setup: no allocation once frames run
     oc.tilde_h0_k()
     slot_fields = [pyocean.OcnFields(*[bufs[k] + s * nn * sizes[k] for k in names]) for s in range(slots)]
 
@@ -370,7 +420,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit", "batched-fit", "render"],
+    ap.add_argument("--mode", choices=["frames", "copy", "cascades", "fit", "batched-fit", "render", "single"],
                     default="frames")
     ap.add_argument("--cascade-frames", type=int, default=60,
                     help="frames of the batched-cascade comparison (0 skips it)")
@@ -398,6 +448,9 @@ def main():
     if args.mode == "render":
         print(json.dumps({"mode": "render", **render_bench(hip, args.steps, args.warmup)}), flush=True)
         return
+    if args.mode == "single":
+        print(json.dumps({"mode": "single", **single_frame_bench(hip)}), flush=True)
+        return
     if args.mode == "cascades":
         print(json.dumps({"mode": "cascades", **cascades_bench(hip, max(1, args.cascade_frames), 10)}), flush=True)
         return
@@ -408,6 +461,10 @@ def main():
     bufs = {k: hip.malloc(slots * nn * b) for k, b in sizes.items()}
     stream = hip.stream()
     oc = pyocean.Ocean(workload_params(), stream=stream)
+    # Setup, before any timing: the intermediates for 32-frame launches.  The
+    # frame calls themselves never allocate, so warm-up and timed calls of any
+    # length run the same kernels on the same buffers.
+    oc.reserve_frames(min(LAUNCH_FRAMES, slots))
     oc.tilde_h0_k()
     hip.stream_sync(stream)
 
@@ -474,6 +531,7 @@ def main():
         hip.free(v)
     hip.stream_destroy(stream)
 
+    single = single_frame_bench(hip)
     casc = cascades_bench(hip, args.cascade_frames, 10) if args.cascade_frames > 0 else None
     fit = fit_bench(hip) if args.fit else None
     bfit = batched_fit_bench(hip) if args.fit else None
@@ -516,6 +574,7 @@ def main():
             "algorithmic_bytes_per_launch": dom_bytes,
         },
         "cpu_baseline": cpu,
+        "single_frame": single,
         "batched_cascades": casc,
         "fit": fit,
         "batched_fit": bfit,

@@ -118,9 +118,10 @@ ocn::SpecConst spec_const(const ocn_params& p)
     return s;
 }
 
-// Schedules the autotuner found, per (device, N, frames-per-launch class).
-std::mutex                                          g_sched_mu;
-std::map<std::tuple<int, int, int>, std::pair<int, int>> g_sched;
+// Schedules the autotuner found, per (device, N, frames-per-launch class,
+// normal mode): the fused row variants only run with nearest-tap normals.
+std::mutex                                               g_sched_mu;
+std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_sched;
 
 int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn::kMaxFrames; }
 
@@ -229,9 +230,12 @@ int upload_twiddles(ocn_handle* h)
     return OCN_OK;
 }
 
-// Grows the intermediate buffers to hold `frames` frames (stream-ordered
-// users are finished first: hipFree synchronises).
-int ensure_zcap(ocn_handle* h, int frames)
+// Sizes the intermediate buffers for launches of up to `frames` frames.  Only
+// setup calls reach this (ocn_create, ocn_reserve_frames, ocn_autotune): the
+// per-frame entry points never allocate or free, they split their frames into
+// launches of at most zcap frames.  Growing waits for the device (the old
+// buffers may still be in use by queued work) before freeing them.
+int reserve_z(ocn_handle* h, int frames)
 {
     if (frames <= h->zcap) return OCN_OK;
     const size_t nn = (size_t)h->p.N * h->p.N;
@@ -242,8 +246,16 @@ int ensure_zcap(ocn_handle* h, int frames)
         (void)hipFree(z1);
         return rc;
     }
-    if (h->z1) (void)hipFree(h->z1);
-    if (h->z2) (void)hipFree(h->z2);
+    if (h->z1 || h->z2) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) {
+            (void)hipFree(z1);
+            (void)hipFree(z2);
+            return hip_fail(e, "hipDeviceSynchronize (reserve)");
+        }
+        if (h->z1 && (e = hipFree(h->z1)) != hipSuccess) return hip_fail(e, "hipFree(z1)");
+        if (h->z2 && (e = hipFree(h->z2)) != hipSuccess) return hip_fail(e, "hipFree(z2)");
+    }
     h->z1   = z1;
     h->z2   = z2;
     h->zcap = frames;
@@ -259,7 +271,7 @@ void pick_schedule(ocn_handle* h, int nframes, int* cv, int* rv)
     }
     {
         std::lock_guard<std::mutex> lk(g_sched_mu);
-        auto it = g_sched.find({h->device, h->p.N, frames_class(nframes)});
+        auto it = g_sched.find({h->device, h->p.N, frames_class(nframes), h->p.normal_mode});
         if (it != g_sched.end()) {
             *cv = it->second.first;
             *rv = it->second.second;
@@ -317,8 +329,7 @@ int check_fields(const ocn_fields* out)
 int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride, int stages,
                int cv, int rv, hipStream_t s)
 {
-    int rc;
-    if ((rc = ensure_zcap(h, nframes))) return rc;
+    if (nframes > h->zcap) return fail(OCN_ESTATE, "internal: %d frames exceed the reserved %d", nframes, h->zcap);
     ocn::FrameArgs a{};
     frame_args(h, t, nframes, out, out_stride, &a);
     a.cols_variant = cv;
@@ -580,7 +591,7 @@ int ocn_create(const ocn_params* p, ocn_handle** out)
     h->cols_variant = sch.first;
     h->rows_variant = sch.second;
     const size_t nn = (size_t)p->N * p->N;
-    if ((rc = dalloc(&h->h0, nn, "h0")) || (rc = ensure_zcap(h, 1)) || (rc = dalloc(&h->height, nn, "height")) ||
+    if ((rc = dalloc(&h->h0, nn, "h0")) || (rc = reserve_z(h, 1)) || (rc = dalloc(&h->height, nn, "height")) ||
         (rc = dalloc(&h->dx, nn, "dx")) || (rc = dalloc(&h->dz, nn, "dz")) ||
         (rc = dalloc(&h->jac, nn, "jacobian")) || (rc = dalloc(&h->normals, nn, "normals")) ||
         (rc = upload_twiddles(h))) {
@@ -715,10 +726,12 @@ int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn
     if (nframes == 0) return OCN_OK;
     if ((rc = set_device(h))) return rc;
     const size_t nn = (size_t)h->p.N * h->p.N;
+    // launches of at most the reserved frame count: nothing is allocated here
+    const int chunk = std::min(ocn::kMaxFrames, h->zcap);
     int cv, rv;
-    pick_schedule(h, nframes, &cv, &rv);
-    for (int f0 = 0; f0 < nframes; f0 += ocn::kMaxFrames) {
-        const int nf = std::min(ocn::kMaxFrames, nframes - f0);
+    pick_schedule(h, std::min(nframes, chunk), &cv, &rv);
+    for (int f0 = 0; f0 < nframes; f0 += chunk) {
+        const int nf = std::min(chunk, nframes - f0);
         ocn_fields s{out->height + f0 * nn,
                      out->dx + f0 * nn,
                      out->dz + f0 * nn,
@@ -747,6 +760,25 @@ int ocn_update(ocn_handle* h, double t, void* stream)
 int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream)
 {
     return ocn_update_stage(h, t_host, nframes, out, OCN_STAGE_ALL, stream);
+}
+
+int ocn_reserve_frames(ocn_handle* h, int nframes)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (nframes < 1 || nframes > ocn::kMaxFrames)
+        return fail(OCN_EINVAL, "nframes=%d must be in [1, %d]", nframes, ocn::kMaxFrames);
+    if ((rc = set_device(h))) return rc;
+    return reserve_z(h, nframes);
+}
+
+int ocn_frames_reserved(const ocn_handle* h, int* nframes)
+{
+    int rc = check_handle(h);
+    if (rc) return rc;
+    if (!nframes) return fail(OCN_EINVAL, "nframes is NULL");
+    *nframes = std::min(ocn::kMaxFrames, h->zcap);
+    return OCN_OK;
 }
 
 int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant)
@@ -785,7 +817,7 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
     if (rc) return rc;
     if (nframes < 1 || nframes > ocn::kMaxFrames || reps < 1 || reps > 10000)
         return fail(OCN_EINVAL, "nframes must be in [1, %d] and reps in [1, 10000]", ocn::kMaxFrames);
-    if ((rc = set_device(h)) || (rc = ensure_zcap(h, nframes))) return rc;
+    if ((rc = set_device(h)) || (rc = reserve_z(h, nframes))) return rc;
     hipStream_t s = (hipStream_t)stream;
     const size_t nn = (size_t)h->p.N * h->p.N;
     float* outb = nullptr;
@@ -802,6 +834,9 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
     for (int cv = 0; cv < ocn::kNumColsVariants && rc == OCN_OK; ++cv)
         for (int rv = 0; rv < ocn::kNumRowsVariants && rc == OCN_OK; ++rv) {
             if (!ocn::schedule_valid(h->p.N, cv, rv)) continue;
+            // bilinear normals always run the separate normal-map kernel, so a
+            // fused row variant would time the same kernels as variant 0
+            if (h->p.normal_mode != OCN_NORMALS_NEAREST && ocn::rows_variant_band(rv) > 0) continue;
             const int stages = OCN_STAGE_ALL;
             if ((rc = run_frames(h, t.data(), nframes, &out, nn, stages, cv, rv, s))) break;  // warm-up
             (void)hipEventRecord(e0, s);
@@ -828,7 +863,7 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
     if (bc < 0) return fail(OCN_ENOTSUP, "no schedule available for N=%d", h->p.N);
     {
         std::lock_guard<std::mutex> lk(g_sched_mu);
-        g_sched[{h->device, h->p.N, frames_class(nframes)}] = {bc, br};
+        g_sched[{h->device, h->p.N, frames_class(nframes), h->p.normal_mode}] = {bc, br};
     }
     h->tuned = false;  // use the tuned table entry for this frame count class
     return OCN_OK;

@@ -29,6 +29,7 @@ EXPORTS = [
     "ocn_tilde_h0_k", "ocn_tilde_h0_k_from_noise", "ocn_generate_twiddle_factors",
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
+    "ocn_reserve_frames", "ocn_frames_reserved",
     "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
@@ -113,6 +114,8 @@ def lib():
             "ocn_update_into": [vp, dbl, F, vp],
             "ocn_update_stage": [vp, ctypes.POINTER(ctypes.c_double), i32, F, i32, vp],
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
+            "ocn_reserve_frames": [vp, i32],
+            "ocn_frames_reserved": [vp, ctypes.POINTER(i32)],
             "ocn_autotune": [vp, i32, i32, vp],
             "ocn_set_schedule": [vp, i32, i32],
             "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 6,
@@ -392,6 +395,16 @@ class Ocean:
         ts = np.ascontiguousarray(times, dtype=np.float64)
         check(lib().ocn_update_frames(self.h, ts.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(ts),
                                       ctypes.byref(fields), self.stream))
+
+    def reserve_frames(self, nframes):
+        """Sizes the intermediates for launches of up to nframes frames (a
+        setup call: allocates, and waits for the device when it grows)."""
+        check(lib().ocn_reserve_frames(self.h, int(nframes)))
+
+    def frames_reserved(self):
+        v = ctypes.c_int()
+        check(lib().ocn_frames_reserved(self.h, ctypes.byref(v)))
+        return v.value
 
     # schedules
     def autotune(self, nframes=1, reps=20):

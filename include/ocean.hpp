@@ -101,6 +101,15 @@ public:
     {
         check(ocn_update_frames(h_, t.data(), static_cast<int>(t.size()), &out, stream_));
     }
+    // Setup: intermediates for launches of up to nframes frames (update_frames
+    // never allocates; it launches at most this many frames at a time).
+    void reserve_frames(int nframes) { check(ocn_reserve_frames(h_, nframes)); }
+    int frames_reserved() const
+    {
+        int n = 0;
+        check(ocn_frames_reserved(h_, &n));
+        return n;
+    }
 
     void set_params(const ocn_params& p) { check(ocn_set_params(h_, &p)); }
     ocn_params params() const

@@ -15,10 +15,17 @@
  *     message of the calling thread's last failure.
  *   - Device pointers are HIP device memory on the handle's device; `stream`
  *     is a hipStream_t passed as void* (NULL = the null stream).  Work is
- *     stream-ordered; no call synchronises the host except ocn_create,
- *     ocn_set_params (buffer setup), ocn_destroy, the copies to host memory
- *     and ocn_fit_get (which wait for their stream) and
- *     ocn_batch_set_render_export.
+ *     stream-ordered.  The per-frame calls (ocn_butterfly_fft*,
+ *     ocn_generate_normal_map, ocn_update, ocn_update_into, ocn_update_stage,
+ *     ocn_update_frames, ocn_batch_update, ocn_fit_forward, ocn_fit_backward,
+ *     ocn_fit_step) neither allocate nor free device memory nor wait for the
+ *     host, so they may be recorded by stream capture.  The setup calls may
+ *     allocate and wait: ocn_create, ocn_destroy, ocn_reserve_frames,
+ *     ocn_autotune, ocn_tilde_h0_k_from_noise, the ocn_batch_ and ocn_fit_
+ *     create/destroy/set/autotune calls, the copies to host memory and
+ *     ocn_fit_get (which wait for their stream).  ocn_set_params is host-only.
+ *     The first ocn_tilde_h0_t allocates its three spectrum textures and the
+ *     first ocn_batch_update_graph builds its graph.
  *   - Layouts match the reference textures: texel (tx, ty) is element
  *     ty*N + tx; tx is the first texture coordinate (world x), ty the second
  *     (world z).  R32F fields are float[N*N]; RGBA32F fields are float[N*N*4].
@@ -35,7 +42,7 @@
 extern "C" {
 #endif
 
-#define OCN_ABI_VERSION 2
+#define OCN_ABI_VERSION 3
 
 /* status codes */
 #define OCN_OK        0
@@ -144,9 +151,10 @@ int ocn_create(const ocn_params* p, ocn_handle** out);
 /* Frees everything ocn_create allocated.  NULL is accepted. */
 int ocn_destroy(ocn_handle* h);
 
-/* Replaces the parameters without reallocating (N must not change).  The
- * reference never re-runs tilde_h0_k after a GUI edit (main.cpp:352-353); here
- * the next ocn_tilde_h0_k uses the new values. */
+/* Replaces the parameters without reallocating (N must not change); host-only,
+ * no HIP call.  The reference never re-runs tilde_h0_k after a GUI edit
+ * (main.cpp:352-353); here the next ocn_tilde_h0_k uses the new values, and
+ * frame calls return OCN_ESTATE until it has run. */
 int ocn_set_params(ocn_handle* h, const ocn_params* p);
 int ocn_get_params(const ocn_handle* h, ocn_params* p);
 
@@ -215,8 +223,22 @@ int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn
 
 /* nframes frames at times t_host[0..nframes) into consecutive output slices:
  * frame f writes out->height + f*N*N, ..., out->normals + f*N*N*4, ...
- * (the per-frame slices of a recorded sequence). */
+ * (the per-frame slices of a recorded sequence).  The frames run in launches
+ * of at most ocn_frames_reserved() frames each (1 after ocn_create); the call
+ * never allocates, so reserve once before the first multi-frame update. */
 int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream);
+
+/* Sizes the handle's intermediate buffers for launches of up to nframes frames
+ * (1..32; 12 bytes per texel and frame): a setup call, which allocates and,
+ * when it grows the buffers, waits for the device and frees the old ones.
+ * Graphs captured from this handle before a growth must be captured again.
+ * A smaller nframes than the current reservation keeps the buffers.  No
+ * reference counterpart (the reference runs one frame per update(),
+ * main.cpp:240-244, on textures allocated once, main.cpp:1089-1099). */
+int ocn_reserve_frames(ocn_handle* h, int nframes);
+/* Frames per launch the handle's buffers hold (the launch size of
+ * ocn_update_frames). */
+int ocn_frames_reserved(const ocn_handle* h, int* nframes);
 
 /* ---------------------------------------------------------------- schedules */
 
@@ -226,8 +248,10 @@ int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const oc
  *
  * ocn_autotune times every variant available for this N on the handle's
  * device (nframes frames per launch, reps launches each, on `stream`, which
- * it synchronises) and records the fastest for (device, N, frames-per-launch
- * class); later launches of any handle with that N use it.  ocn_set_schedule
+ * it synchronises; it reserves nframes frames as ocn_reserve_frames does) and
+ * records the fastest for (device, N, frames-per-launch class, normal mode);
+ * later launches of any handle with that N and normal mode use it.  With
+ * bilinear normals only the unfused row variants are timed.  ocn_set_schedule
  * pins a variant pair on this handle; ocn_get_schedule reports the pair used
  * for launches of nframes frames and its shape (any out-pointer may be NULL);
  * rows_band > 0 means the normal map and Jacobian are fused into the row pass,

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     dyn = subprocess.run(["nm", "-D", "--defined-only", pyocean.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r"\bT (ocn_\w+)", dyn))
     assert set(names) <= exported
-    assert L.ocn_abi_version() == 2
+    assert L.ocn_abi_version() == 3
 
 
 def test_params_default_matches_reference_members():
@@ -88,6 +88,8 @@ def test_null_handles_and_pointers_rejected():
     assert L.ocn_update_into(None, 0.0, None, None) == pyocean.OCN_EINVAL
     assert L.ocn_generate_normal_map(None, None) == pyocean.OCN_EINVAL
     assert L.ocn_get_fields(None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_reserve_frames(None, 32) == pyocean.OCN_EINVAL
+    assert L.ocn_frames_reserved(None, None) == pyocean.OCN_EINVAL
     assert L.ocn_destroy(None) == pyocean.OCN_OK
     assert L.ocn_params_default(None) == pyocean.OCN_EINVAL
     assert L.ocn_bit_reversed_indices(256, None) == pyocean.OCN_EINVAL

@@ -118,6 +118,74 @@ def test_fit_members_equal_single_fits(hip):
         np.testing.assert_array_equal(together[i][3], gh)
 
 
+def _orc_from_lib(p):
+    return oracle.make_params(N=p.N, L=p.L, wind_speed=p.wind_speed, wind_dir=(p.wind_dir_x, p.wind_dir_y),
+                              amplitude=p.amplitude, suppression=p.suppression, choppiness=p.choppiness,
+                              gravity=p.gravity, seed=p.seed, ocean_id=p.ocean_id, cascade_id=p.cascade_id,
+                              conj_mode=p.conj_mode, normal_mode=p.normal_mode)
+
+
+def test_c5_batched_fit_at_size(hip):
+    """The batched-fit workload at its stated size: 256 oceans at N=256,
+    L=500 m, K=4 (t = 0.5..2.0 s), every ocean with its own hidden (V, theta,
+    A) and noise seed, all at the common start (17.5 m/s, 0 rad, 3.5).
+    Targets are each ocean's heights at its hidden variables plus
+    N(0, 0.05^2) noise (made on the GPU by a batch of the hidden oceans; both
+    sides of every comparison get the same targets).  One forward+backward of
+    all 256 members in one set of launches: members 0, 128 and 255 against
+    the FP64 oracle adjoint (loss, dL/d(V, theta, A), dL/dh0 at rel <= 1e-3),
+    and every member bit-equal to a fit of that member alone."""
+    times = [0.5, 1.0, 1.5, 2.0]
+    M = 256
+    hidden, starts = pyocean.fit_batch_params(M)
+    N = starts[0].N
+    assert (N, starts[0].L, len(times)) == (256, 500.0, 4)
+    rng = np.random.Generator(np.random.Philox(key=777))
+    tgs = np.empty((M, len(times), N, N), dtype=np.float32)
+    _say("c5: targets of %d oceans" % M)
+    with pyocean.Batch(hidden) as b:
+        b.tilde_h0_k()
+        for k, t in enumerate(times):
+            b.update(t)
+            hip.sync()
+            for o in range(M):
+                tgs[o, k] = hip.download(b.fields(o).height, (N, N))
+    tgs += (0.05 * rng.standard_normal(tgs.shape)).astype(np.float32)
+    _say("c5: forward+backward of %d members" % M)
+    with pyocean.Fit(starts, times) as f:
+        for o in range(M):
+            f.upload_targets(o, tgs[o])
+        f.forward()
+        loss_fwd = [f.get(o)[2] for o in range(M)]
+        f.backward()
+        together = [f.get(o) + (f.grad_h0(o),) for o in range(M)]
+    for o in range(M):
+        x, g, loss, gh = together[o]
+        assert np.all(np.isfinite(g)) and np.isfinite(loss), o
+        assert abs(loss_fwd[o] - loss) <= 1e-6 * loss, o
+    for o in (0, 128, 255):
+        op = _orc_from_lib(starts[o])
+        loss_ref, gh_ref = oracle.fit_eval(op, oracle.h0(op), times, tgs[o].astype(np.float64), nthreads=NTHREADS)
+        g_ref = oracle.fit_param_grad(op, gh_ref)
+        _, g, loss, gh = together[o]
+        _say("  member %d: loss %.6e ref %.6e; grad %s ref %s" % (o, loss, loss_ref, g, g_ref))
+        assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+        np.testing.assert_allclose(g, g_ref, rtol=GRAD_RTOL, atol=1e-6 * np.abs(g_ref).max())
+        err = gh.astype(np.float64) - gh_ref
+        assert np.sqrt((err ** 2).sum() / (gh_ref ** 2).sum()) <= GRAD_RTOL
+        assert np.abs(err).max() <= GRAD_RTOL * np.abs(gh_ref).max()
+    _say("c5: %d single-member fits" % M)
+    for o in range(M):
+        with pyocean.Fit([starts[o]], times) as f1:
+            f1.upload_targets(0, tgs[o])
+            f1.backward()
+            x, g, loss = f1.get(0)
+            gh = f1.grad_h0(0)
+        np.testing.assert_array_equal(together[o][1], g, err_msg="member %d" % o)
+        assert together[o][2] == loss, o
+        np.testing.assert_array_equal(together[o][3], gh, err_msg="member %d" % o)
+
+
 def test_fit_adam_recovers_hidden_variables(hip):
     """The N=512 workload: targets from (22 m/s, 30 deg, 3.5) plus N(0, 0.05^2)
     noise at t = 0.5..4.0 s; 200 Adam steps from (18 m/s, 15 deg, 2.5)."""

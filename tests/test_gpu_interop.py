@@ -134,6 +134,7 @@ def test_multiframe_launch_into_imported_memory_at_an_offset(hip, vmm):
             times = np.array([0.75, 0.75 + 1.0 / 60.0])
             with pyocean.Ocean(pyocean.make_params(**kw)) as oc:
                 oc.tilde_h0_k()
+                oc.reserve_frames(F)
                 oc.update_frames(times, fields)
                 hip.sync()
                 got = [_download(hip, view, N, f) for f in range(F)]

@@ -221,6 +221,7 @@ def test_update_frames_equals_per_frame_update(hip):
     times = np.arange(F) / 60.0 + 0.25
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
+        oc.reserve_frames(F)
         # one schedule for both paths (batched and single launches default to
         # different kernels; each is checked against the oracle elsewhere)
         sch = oc.schedule(F)
@@ -283,6 +284,160 @@ def test_updates_captured_in_a_graph_replay_like_eager_launches(hip):
             for b in bufs:
                 for v in b.values():
                     hip.free(v)
+    hip.stream_destroy(stream)
+
+
+def test_c2_sequence_as_benched_matches_oracle(hip):
+    """The headline workload exactly as bench.py launches it: N=1024 with the
+    c2 parameters, 32 frames reserved, frames t = f/60 of the 600-frame
+    sequence through ocn_update_frames in 32-frame launches with the built-in
+    multi-frame schedule (column variant 9: k_cols_lds with 4 columns; row
+    variant 5: the fused row + normal-map + Jacobian walk with bands of 32
+    rows, odd bands walking upwards).  Frames 0..63 in one call (two
+    launches) and frames 568..599 in a second call; frames 0, 31, 32, 33, 63
+    and 599 against the oracle at the c2 tolerance."""
+    lp, op = both_params(**c2_kwargs())
+    N, nn = 1024, 1024 * 1024
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    F = 64
+    bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
+    fl = pyocean.OcnFields(*[bufs[k] for k in sizes])
+    got = {}
+    _say("c2 as benched: N=1024, 32-frame launches")
+    try:
+        with pyocean.Ocean(lp) as oc:
+            oc.reserve_frames(32)
+            assert oc.frames_reserved() == 32
+            oc.tilde_h0_k()
+            sch = oc.schedule(32)
+            assert (sch["cols_variant"], sch["rows_variant"]) == (9, 5), sch
+            assert sch["rows_band"] == 32 and sch["cols_per_block"] == 4, sch
+            oc.update_frames(np.arange(F, dtype=np.float64) / 60.0, fl)
+            hip.sync()
+
+            def take(slot):
+                return {k: hip.download(bufs[k] + slot * nn * b, (N, N, 4) if k == "normals" else (N, N))
+                        for k, b in sizes.items()}
+            for f in (0, 31, 32, 33, 63):
+                got[f] = take(f)
+            oc.update_frames(np.arange(568, 600, dtype=np.float64) / 60.0, fl)
+            hip.sync()
+            got[599] = take(31)
+    finally:
+        for v in bufs.values():
+            hip.free(v)
+    h0 = oracle.h0(op)
+    for f, g in got.items():
+        ref = oracle.frame(op, h0, f / 60.0, nthreads=NTHREADS)
+        for fld in sizes:
+            r, m = assert_field_close("%s[f=%d]" % (fld, f), g[fld], ref[fld])
+        _say("  frame %d: jacobian rel-L2 %.2e" % (f, r))
+
+
+def test_set_params_then_tilde_h0_k_reinitialises_the_ocean(hip):
+    """Live re-initialisation (the reference's GUI edits wind speed and
+    amplitude, main.cpp:350-353, but never regenerates the spectrum):
+    ocn_set_params with a new wind and amplitude, then frames are refused
+    until ocn_tilde_h0_k has run, and the frame after it equals the oracle's
+    at the new parameters.  Changing N is refused."""
+    kw1 = c1_kwargs()
+    kw2 = dict(c1_kwargs(), wind_speed=23.0, wind_dir=(0.3, -1.0), amplitude=2.5, choppiness=0.8)
+    lp1, op1 = both_params(**kw1)
+    lp2, op2 = both_params(**kw2)
+    with pyocean.Ocean(lp1) as oc:
+        oc.tilde_h0_k()
+        oc.update(2.0)
+        first = oc.download_fields()
+        oc.set_params(lp2)
+        with pytest.raises(pyocean.OceanError) as ei:
+            oc.update(2.0)
+        assert ei.value.status == "OCN_ESTATE"
+        with pytest.raises(pyocean.OceanError) as ei:
+            oc.set_params(pyocean.make_params(**dict(kw2, N=512)))
+        assert ei.value.status == "OCN_EINVAL"
+        oc.tilde_h0_k()
+        oc.update(2.0)
+        second = oc.download_fields()
+        h0_2 = oc.download_h0()
+    assert_field_close("h0 (new params)", h0_2, oracle.h0(op2), rel_tol=2e-6, abs_rel_tol=1e-5)
+    for got, op in ((first, op1), (second, op2)):
+        ref = oracle.frame(op, oracle.h0(op), 2.0, nthreads=NTHREADS)
+        for fld in ("height", "dx", "dz", "normals", "jacobian"):
+            assert_field_close(fld, got[fld], ref[fld])
+    assert rel_l2(second["height"], first["height"]) > 0.1  # the new parameters took effect
+
+
+def test_graph_of_single_frames_survives_multiframe_updates(hip):
+    """A graph captured from ocn_update (one frame reserved) keeps pointing at
+    live buffers after ocn_update_frames of 32 frames on the same handle: the
+    frame calls never reallocate the intermediates, they launch at most the
+    reserved frame count at a time.  Replays equal the eager frame bit for bit."""
+    lp, _ = both_params(**dict(c1_kwargs(), N=256))
+    N, nn, F = 256, 256 * 256, 32
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    stream = hip.stream()
+    bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
+    try:
+        with pyocean.Ocean(lp, stream=stream) as oc:
+            oc.tilde_h0_k()
+            assert oc.frames_reserved() == 1
+            oc.update(1.25)
+            hip.stream_sync(stream)
+            eager = oc.download_fields()
+            g, ex = hip.capture(stream, lambda: oc.update(1.25))
+            try:
+                oc.update_frames(np.arange(F) / 60.0, pyocean.OcnFields(*[bufs[k] for k in sizes]))
+                assert oc.frames_reserved() == 1
+                for rep in range(3):
+                    oc.update(7.0)  # eager work between replays
+                    hip.graph_launch(ex, stream)
+                    hip.stream_sync(stream)
+                    got = oc.download_fields()
+                    for k in sizes:
+                        np.testing.assert_array_equal(got[k], eager[k], err_msg="%s replay %d" % (k, rep))
+            finally:
+                hip.graph_destroy(g, ex)
+    finally:
+        for v in bufs.values():
+            hip.free(v)
+    hip.stream_destroy(stream)
+
+
+def test_first_32_frame_update_captured_in_a_graph(hip):
+    """With 32 frames reserved at setup, the first-ever ocn_update_frames of 32
+    frames is recorded by stream capture (nothing allocates or synchronises
+    inside it) and every replay equals the eager launch bit for bit."""
+    lp, _ = both_params(**dict(c2_kwargs(), N=512, L=1000.0))
+    N, nn, F = 512, 512 * 512, 32
+    sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
+    times = 2.0 + np.arange(F) / 60.0
+    stream = hip.stream()
+    bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
+    fl = pyocean.OcnFields(*[bufs[k] for k in sizes])
+    down = lambda: {k: hip.download(bufs[k], (F, N, N, 4) if k == "normals" else (F, N, N)) for k in sizes}
+    try:
+        with pyocean.Ocean(lp, stream=stream) as oc:
+            oc.reserve_frames(F)
+            oc.tilde_h0_k()
+            hip.stream_sync(stream)
+            g, ex = hip.capture(stream, lambda: oc.update_frames(times, fl))
+            try:
+                for rep in range(3):
+                    for k, v in bufs.items():
+                        hip.memset(v, 0, F * nn * sizes[k])
+                    hip.graph_launch(ex, stream)
+                    hip.stream_sync(stream)
+                    replay = down()
+                    oc.update_frames(times, fl)
+                    hip.stream_sync(stream)
+                    eager = down()
+                    for k in sizes:
+                        np.testing.assert_array_equal(replay[k], eager[k], err_msg="%s replay %d" % (k, rep))
+            finally:
+                hip.graph_destroy(g, ex)
+    finally:
+        for v in bufs.values():
+            hip.free(v)
     hip.stream_destroy(stream)
 
 
@@ -354,6 +509,7 @@ def test_full_size_multiframe_launch(hip, N):
     try:
         with pyocean.Ocean(lp) as oc:
             oc.tilde_h0_k()
+            oc.reserve_frames(F)
             sch = oc.schedule(F)
             assert sch["rows_band"] == 32, sch
             oc.update_frames(times, pyocean.OcnFields(*[bufs[k] for k in names]))
@@ -424,6 +580,7 @@ def test_multiframe_launch_matches_oracle_per_frame(hip):
     h0 = oracle.h0(op)
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
+        oc.reserve_frames(32)
         bufs = {k: hip.malloc(F * N * N * (16 if k == "normals" else 4))
                 for k in ("height", "dx", "dz", "normals", "jacobian")}
         try:
@@ -509,6 +666,7 @@ def test_render_export_matches_oracle(hip, rows_variant):
     _say("render export rows_variant=%d" % rows_variant)
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
+        oc.reserve_frames(F)
         oc.set_schedule(9, rows_variant)
         bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
         try:
