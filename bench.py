@@ -68,40 +68,66 @@ def workload_params():
                                amplitude=4.0, suppression=1.0, choppiness=1.3, seed=42)
 
 
+def host_cpu():
+    """CPU model (first 'model name' of /proc/cpuinfo) and logical CPUs."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1
+
+
+def cpu_share():
+    """Threads for the CPU baseline: the CPU share this process may use.  On a
+    shared GPU host that is what OMP_NUM_THREADS states (the host's share for
+    one GPU), else the CPUs of this process's affinity mask."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return max(1, min(aff, omp) if omp > 0 else aff)
+
+
 def cpu_baseline(seconds):
     """The FP64 oracle (a scalar C port of the reference shaders) timed on
-    this host: whole N=1024 frames, one thread, until `seconds` have passed."""
+    this host: whole N=1024 frames (spectrum, three radix-2 butterfly IFFTs
+    with inversion, normals, Jacobian), once on one thread and once with every
+    pass split by rows over the host's CPU share, each until its share of
+    `seconds` has passed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     th = math.radians(30.0)
     op = oracle.make_params(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(th), math.sin(th)),
                             amplitude=4.0, suppression=1.0, choppiness=1.3, seed=42)
     h0 = oracle.h0(op)
-    oracle.frame(op, h0, 0.0, nthreads=1)  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        oracle.frame(op, h0, n / 60.0, nthreads=1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    out = {"value": round(n / el, 4), "unit": "frames/s", "cores": 1, "kind": "port",
-           "sample": "%d whole N=1024 frames (spectrum, 3 radix-2 butterfly IFFTs + inversion, normals, "
-                     "Jacobian) of the FP64 oracle in %.1f s, 1 thread" % (n, el)}
-    # the same frames with the oracle's butterfly stages spread over the host's
-    # cores (one pthread per row range; spectrum, inversion, normals and
-    # Jacobian stay single-threaded)
-    cores = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()))
-    n2, t0 = 0, time.perf_counter()
-    while True:
-        oracle.frame(op, h0, n2 / 60.0, nthreads=cores)
-        n2 += 1
-        el2 = time.perf_counter() - t0
-        if el2 >= seconds / 2:
-            break
-    out["threaded"] = {"value": round(n2 / el2, 4), "unit": "frames/s", "cores": cores,
-                       "sample": "%d frames in %.1f s, butterfly stages on %d threads" % (n2, el2, cores)}
-    return out
+    model, logical = host_cpu()
+    cores = cpu_share()
+
+    def run(nthreads, budget):
+        oracle.frame(op, h0, 0.0, nthreads=nthreads)  # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            oracle.frame(op, h0, n / 60.0, nthreads=nthreads)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                return n, el
+
+    n1, el1 = run(1, seconds / 3.0)
+    nt, elt = run(cores, seconds * 2.0 / 3.0)
+    desc = ("whole N=1024 frames (spectrum, 3 radix-2 butterfly IFFTs + inversion, normals, Jacobian) of the "
+            "FP64 oracle")
+    return {"value": round(nt / elt, 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": "%d %s in %.1f s, every pass split by rows over %d threads" % (nt, desc, elt, cores),
+            "cpu_model": model, "logical_cpus": logical,
+            "single_thread": {"value": round(n1 / el1, 4), "unit": "frames/s", "cores": 1,
+                              "sample": "%d frames in %.1f s, 1 thread" % (n1, el1)}}
 
 
 def cascades_bench(hip, frames, warmup):

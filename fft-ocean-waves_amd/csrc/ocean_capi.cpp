@@ -1351,6 +1351,16 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
         if (params[i].N < ocn::kFitMinN || params[i].N > ocn::kFitMaxN)
             return fail(OCN_EINVAL, "member %d: fitted N must be in [%d, %d]", i, ocn::kFitMinN, ocn::kFitMaxN);
     }
+    // Equally spaced times (to 1e-12 of their scale) let the kernels step the
+    // dispersion phase from frame to frame instead of evaluating it per frame.
+    bool   uniform_times = K >= 2;
+    double uniform_dt    = K >= 2 ? (times[K - 1] - times[0]) / (K - 1) : 0.0;
+    {
+        double scale = 1.0;
+        for (int k = 0; k < K; ++k) scale = std::max(scale, std::fabs(times[k]));
+        for (int k = 0; k < K && uniform_times; ++k)
+            uniform_times = std::fabs(times[k] - (times[0] + k * uniform_dt)) <= 1e-12 * scale;
+    }
     ocn_fit* f = new (std::nothrow) ocn_fit();
     if (!f) return fail(OCN_ENOMEM, "host allocation of the fit failed");
     f->K = K;
@@ -1398,6 +1408,8 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.g2_d      = p.gravity * p.gravity;
             m.g_d       = p.gravity;
             for (int k = 0; k < K; ++k) m.t_rev[k] = times[k] / (2.0 * kPi);
+            m.uniform = uniform_times ? 1 : 0;
+            m.dt_rev  = uniform_dt / (2.0 * kPi);
             m.xi        = reinterpret_cast<const float4*>(q), q += al(nn * 16);
             m.z         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
             m.y         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);

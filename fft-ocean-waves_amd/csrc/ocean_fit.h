@@ -43,6 +43,9 @@ struct FitMember {
     double  k2_d, g2_d;     // (2 pi / L)^2, g^2
     double  g_d;            // g
     double  t_rev[kFitMaxK];  // t_k / (2 pi)
+    double  dt_rev;         // (t_{k+1} - t_k) / (2 pi) when `uniform`
+    int     uniform;        // 1: the times are equally spaced (phases by recurrence)
+    int     pad0_;
     const float4* xi;       // unit-amplitude noise, N*N
     const float2* tw8;      // Stockham twiddles (N, 8) and (N, 16)
     const float2* tw16;

@@ -78,27 +78,56 @@ __device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars&
     return sc;
 }
 
-// Hermitian part (H(t) + conj H(t'))/2 of the height spectrum at one time;
-// p = h0 at t, q = h0 at its mirror t', w = omega_fp64 (shared by t and t').
-__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, double w, double t_rev)
+// e^{i phi} = (cos, sin) of the dispersion phase phi = w t of frame k.  With
+// equally spaced times the kernels instead start from frame 0 and step by
+// e^{i w dt} (one complex multiply per frame in place of the FP64 phase
+// reduction and the sin/cos polynomials); up to 16 frames the recurrence adds
+// at most ~2e-6 of relative error.
+struct PhaseWalk {
+    float2 cur, step;
+    __device__ __forceinline__ void start(double w, double t0_rev, double dt_rev)
+    {
+        float s, c;
+        sincos_turns(turns(w, t0_rev), s, c);
+        cur = make_float2(c, s);
+        sincos_turns(turns(w, dt_rev), s, c);
+        step = make_float2(c, s);
+    }
+    // e^{i phi} of the current frame, then advance to the next
+    __device__ __forceinline__ float2 next()
+    {
+        const float2 r = cur;
+        cur = make_float2(fmaf(r.x, step.x, -r.y * step.y), fmaf(r.x, step.y, r.y * step.x));
+        return r;
+    }
+};
+
+__device__ __forceinline__ float2 phase_at(double w, double t_rev)
 {
     float s, c;
     sincos_turns(turns(w, t_rev), s, c);
-    const float2 H = evolve_amp(sc, p, s, c);
-    const float2 G = evolve_amp(sc, q, s, c);
+    return make_float2(c, s);
+}
+
+// Hermitian part (H(t) + conj H(t'))/2 of the height spectrum at one time;
+// p = h0 at t, q = h0 at its mirror t' (which shares t's |k| and so its phase
+// e^{i phi} = (e.x, e.y)).
+__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, float2 e)
+{
+    const float2 H = evolve_amp(sc, p, e.y, e.x);
+    const float2 G = evolve_amp(sc, q, e.y, e.x);
     return make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
 }
 
 // dL/dh0 += the pull-back of the frame spectrum gradient G through
 // H = h0k e^{i phi} + C(h0mk) e^{-i phi}: ga += e^{-i phi} G, gb += e^{+i phi} G
-// (identity C) or e^{-i phi} conj(G) (conjugation), w = omega_fp64 of the texel;
+// (identity C) or e^{-i phi} conj(G) (conjugation), e = e^{i phi} of the texel;
 // done for a texel u (ga, gb) and its mirror -u (gam, gbm), which share the
 // phase and whose gradients are conjugate: G(-u) = conj G(u).
-__device__ __forceinline__ void pull_back_pair(const SpecConst& sc, float2 G, double w, double t_rev, float2& ga,
-                                               float2& gb, float2& gam, float2& gbm)
+__device__ __forceinline__ void pull_back_pair(const SpecConst& sc, float2 G, float2 e, float2& ga, float2& gb,
+                                               float2& gam, float2& gbm)
 {
-    float s, c;
-    sincos_turns(turns(w, t_rev), s, c);
+    const float c = e.x, s = e.y;
     const float cx = c * G.x, cy = c * G.y, sx = s * G.x, sy = s * G.y;
     ga.x += cx + sy;  // e^{-i phi} G
     ga.y += cy - sx;
@@ -135,7 +164,7 @@ __device__ __forceinline__ T block_sum(T x, T* red)
 
 // Column transforms run two frame pairs at a time: 2 pairs x 2C columns of T
 // threads fill the 256-thread block.
-template <int N>
+template <int N, bool UNIFORM>
 __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)
 {
     constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
@@ -145,8 +174,9 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
     const SpecConst sc = fit_spec(m, *m.vars);
     const size_t    nn = (size_t)N * N;
 
-    float4 hp[IT], hq[IT];
-    double w[IT];
+    float4    hp[IT], hq[IT];
+    double    w[UNIFORM ? 1 : IT];    // per-frame phases: omega of each texel pair
+    PhaseWalk walk[UNIFORM ? IT : 1]; // equally spaced frames: the phase recurrence
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int k = threadIdx.x + it * NT;
@@ -161,8 +191,10 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         const int nx = txc - N / 2, ny = ty - N / 2;
-        float kinv;
-        w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
+        float        kinv;
+        const double om = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
+        if constexpr (UNIFORM) walk[it].start(om, m.t_rev[0], m.dt_rev);
+        else w[it] = om;
     }
     for (int p0 = 0; p0 < m.P; p0 += 2) {
         const int np = min(2, m.P - p0);
@@ -170,13 +202,23 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         for (int q = 0; q < 2; ++q) {
             if (q >= np) break;
             const int    p   = p0 + q;
-            const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[min(2 * p + 1, m.K - 1)];
+            const bool   two = 2 * p + 1 < m.K;
+            const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[two ? 2 * p + 1 : m.K - 1];
 #pragma unroll
             for (int it = 0; it < IT; ++it) {
                 const int k = threadIdx.x + it * NT;
                 const int c = k % C, ty = k / C;
-                lds[(q * S + c) * LEN + lds_pad(ty)]     = fold_height(sc, hp[it], hq[it], w[it], tr1);
-                lds[(q * S + C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], w[it], tr2);
+                // frames in order 2p, 2p + 1; a lone last frame is folded twice
+                float2 e1, e2;
+                if constexpr (UNIFORM) {
+                    e1 = walk[it].next();
+                    e2 = two ? walk[it].next() : e1;
+                } else {
+                    e1 = phase_at(w[it], tr1);
+                    e2 = phase_at(w[it], tr2);
+                }
+                lds[(q * S + c) * LEN + lds_pad(ty)]     = fold_height(sc, hp[it], hq[it], e1);
+                lds[(q * S + C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], e2);
             }
         }
         __syncthreads();
@@ -277,7 +319,7 @@ __device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* _
     const float k2   = kmag * kmag;
     const float d    = (kx * sc.wx + ky * sc.wy) / kmag;
     const float dp   = (ky * sc.wx - kx * sc.wy) / kmag;
-    const float c    = sqrtf(sc.A * expf(-1.0f / (k2 * sc.Lp2)) * expf(-k2 * sc.l2) / (k2 * k2)) * 0.70710678118654752f;
+    const float c    = sqrtf(sc.A * expf(-1.0f / (k2 * sc.Lp2) - k2 * sc.l2) / (k2 * k2)) * 0.70710678118654752f;
     const float s    = c * fabsf(d);
     if (!(s > 0.0f) || s > 4000.0f) return;
     const float4 xi   = xi_tex[(size_t)ty * N + tx];
@@ -296,8 +338,9 @@ __device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* _
 // one phase and one sincos per frame serve both.  Self-mirror columns (tx = 0,
 // N/2) map -u into the same column, where it is a u of another row: there
 // only u is accumulated.
-template <int N>
-__device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, float2* lds, float* red)
+template <int N, bool UNIFORM>
+__device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool store_grad_h0, float2* lds,
+                                             float* red)
 {
     constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
     constexpr int S  = 2 * C;           // transforms per frame pair: the strip's columns and their mirrors
@@ -307,8 +350,9 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
     const SpecConst sc = fit_spec(m, vars);
     const size_t    nn = (size_t)N * N;
 
-    float2 ga[IT], gb[IT], gam[IT], gbm[IT];  // u, then -u
-    double w[IT];
+    float2    ga[IT], gb[IT], gam[IT], gbm[IT];  // u, then -u
+    double    w[UNIFORM ? 1 : IT];
+    PhaseWalk walk[UNIFORM ? IT : 1];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         ga[it] = gb[it] = gam[it] = gbm[it] = make_float2(0.0f, 0.0f);
@@ -316,8 +360,10 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         const int c = k % C, ty = k / C;
         const int tx = min(strip * C + c, N / 2);
         const int nx = tx - N / 2, ny = ty - N / 2;
-        float kinv;
-        w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);
+        float        kinv;
+        const double om = omega_fp64(sc, nx * nx + ny * ny, kinv);
+        if constexpr (UNIFORM) walk[it].start(om, m.t_rev[0], m.dt_rev);
+        else w[it] = om;
     }
 
     for (int p0 = 0; p0 < m.P; p0 += 2) {
@@ -366,8 +412,16 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
                 const float2 fm = lds[(q * S + C + c) * LEN + lds_pad(my)];
                 const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
                 const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
-                pull_back_pair(sc, G1, w[it], tr1, ga[it], gb[it], gam[it], gbm[it]);
-                if (two) pull_back_pair(sc, G2, w[it], tr2, ga[it], gb[it], gam[it], gbm[it]);
+                float2 e1;
+                if constexpr (UNIFORM) e1 = walk[it].next();
+                else e1 = phase_at(w[it], tr1);
+                pull_back_pair(sc, G1, e1, ga[it], gb[it], gam[it], gbm[it]);
+                if (two) {
+                    float2 e2;
+                    if constexpr (UNIFORM) e2 = walk[it].next();
+                    else e2 = phase_at(w[it], tr2);
+                    pull_back_pair(sc, G2, e2, ga[it], gb[it], gam[it], gbm[it]);
+                }
             }
         }
         __syncthreads();
@@ -383,11 +437,11 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, floa
         const int c = k % C, ty = k / C;
         const int tx = strip * C + c;
         if (tx > N / 2) continue;  // past the half plane (the last strip)
-        m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
+        if (store_grad_h0) m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
         param_terms(sc, m.xi, N, tx, ty, ga[it], gb[it], dv_scale, da_scale, dV, dT, dA);
         if (tx == 0 || tx == N / 2) continue;  // -u lies in the same column: a u of its own
         const int mx = N - tx, my = (N - ty) & (N - 1);
-        m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
+        if (store_grad_h0) m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
         param_terms(sc, m.xi, N, mx, my, gam[it], gbm[it], dv_scale, da_scale, dV, dT, dA);
     }
     dV = block_sum(dV, red);
@@ -429,7 +483,9 @@ __global__ __launch_bounds__(kFitColsThreads) void k_fit_cols_fwd(const FitMembe
     __shared__ float2 lds[kColsLds];
     const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
     const FitMember& m  = members[it.x];
-#define OCN_CALL(n) fit_cols_fwd<n>(m, it.y, lds)
+#define OCN_CALL(n) \
+    if (m.uniform) fit_cols_fwd<n, true>(m, it.y, lds); \
+    else fit_cols_fwd<n, false>(m, it.y, lds)
     OCN_FIT_SWITCH(m.N, OCN_CALL)
 #undef OCN_CALL
 }
@@ -449,13 +505,15 @@ __global__ __launch_bounds__(kFitRowsThreads) void k_fit_rows(const FitMember* _
 // Capped at three waves per SIMD (a few registers spill): 0.80 -> 0.68 ms per
 // step of a 256-ocean N=256 fit against the uncapped two waves.
 __global__ __launch_bounds__(kFitColsThreads, 3) void k_fit_cols_bwd(const FitMember* __restrict__ members,
-                                                                  const int2* __restrict__ items)
+                                                                  const int2* __restrict__ items, int store_grad_h0)
 {
     __shared__ float2 lds[kColsLds];
     __shared__ float  red[kFitColsThreads / 64];
     const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
     const FitMember& m  = members[it.x];
-#define OCN_CALL(n) fit_cols_bwd<n>(m, it.y, lds, red)
+#define OCN_CALL(n) \
+    if (m.uniform) fit_cols_bwd<n, true>(m, it.y, store_grad_h0 != 0, lds, red); \
+    else fit_cols_bwd<n, false>(m, it.y, store_grad_h0 != 0, lds, red)
     OCN_FIT_SWITCH(m.N, OCN_CALL)
 #undef OCN_CALL
 }
@@ -514,8 +572,11 @@ hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_
     const int adjoint = mode != kFitLoss;
     hipLaunchKernelGGL(k_fit_cols_fwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
     hipLaunchKernelGGL(k_fit_rows, dim3(n_rows), dim3(kFitRowsThreads), 0, s, members, rows_items, adjoint);
+    // an Adam step needs the parameter gradient only; dL/dh0 (16 bytes per
+    // texel) is written by ocn_fit_backward
     if (adjoint)
-        hipLaunchKernelGGL(k_fit_cols_bwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
+        hipLaunchKernelGGL(k_fit_cols_bwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
+                           mode == kFitGrad ? 1 : 0);
     hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam);
     return hipGetLastError();
 }

@@ -389,14 +389,17 @@ int ocn_fit_forward(ocn_fit* f, void* stream);
 /* Backward operator: forward recomputed, then the adjoint: loss, dL/d(V, theta, A)
  * and dL/dh0 of every member. */
 int ocn_fit_backward(ocn_fit* f, void* stream);
-/* ocn_fit_backward followed by one Adam step on every member's variables, on the
- * device (no host round trip; a sequence of steps can be queued at once). */
+/* The loss and dL/d(V, theta, A) of ocn_fit_backward followed by one Adam step on
+ * every member's variables, on the device (no host round trip; a sequence of
+ * steps can be queued at once).  It does not write dL/dh0, which only
+ * ocn_fit_backward produces. */
 int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream);
 
 /* After the work on `stream` has finished (the call synchronises it): one
  * member's variables, last gradient and last loss (any pointer may be NULL). */
 int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* loss, void* stream);
-/* Device pointer of one member's dL/dh0 (float4[N*N]) from the last backward. */
+/* Device pointer of one member's dL/dh0 (float4[N*N]) from the last
+ * ocn_fit_backward. */
 int ocn_fit_grad_h0(const ocn_fit* f, int member, void** grad_h0);
 
 /* ---------------------------------------------------------------- graphics interop

@@ -36,6 +36,7 @@
  *   orc_fit_param_grad                           : dL/d(V, theta, A) vs FP64 central
  *                                                  differences of the loss (h = 1e-6).
  */
+#define _POSIX_C_SOURCE 200809L /* pthread barriers under -std=c11 */
 #include "ocean_oracle.h"
 
 #include <math.h>
@@ -113,88 +114,161 @@ int orc_twiddle_factors(int N, double* out)
 
 /* ------------------------------------------------------------------ A5 */
 
-typedef struct {
-    int N, stage, dir, row0, row1;
-    const double* tw;
-    const double* in;
-    double* out;
-} bfly_job;
+/* A team of nthreads workers (the caller is worker 0) that share a barrier:
+ * each worker owns a contiguous range of rows and the passes meet at the
+ * barrier, so a frame creates its threads once. */
+typedef struct team {
+    int               nthreads;
+    pthread_barrier_t bar;
+    pthread_mutex_t   mu;
+    pthread_cond_t    cv;
+    int               state; /* 0: workers wait; 1: run; -1: a thread failed to start, exit */
+    void (*fn)(struct team*, int tid, void* ctx);
+    void* ctx;
+} team;
 
-/* One butterfly dispatch (butterfly_cs.glsl:54-91 horizontal, :95-132 vertical):
- * out = in[a] + w * in[b] with (w, a, b) = twiddle[stage][x or y]. */
-static void* bfly_run(void* arg)
+typedef struct {
+    team* T;
+    int   tid;
+} team_arg;
+
+static void* team_main(void* a)
 {
-    const bfly_job* J = (const bfly_job*)arg;
-    const int N = J->N;
-    for (int y = J->row0; y < J->row1; ++y) {
-        for (int x = 0; x < N; ++x) {
-            const double* d = J->tw + ((size_t)J->stage * N + (J->dir == 0 ? x : y)) * 4;
-            int a = (int)d[2], b = (int)d[3];
-            size_t ip, iq;
-            if (J->dir == 0) { ip = (size_t)y * N + a; iq = (size_t)y * N + b; }
-            else             { ip = (size_t)a * N + x; iq = (size_t)b * N + x; }
-            double pr = J->in[2 * ip], pi = J->in[2 * ip + 1];
-            double qr = J->in[2 * iq], qi = J->in[2 * iq + 1];
-            double wr = d[0], wi = d[1];
-            size_t o = (size_t)y * N + x;
-            J->out[2 * o]     = pr + (wr * qr - wi * qi);
-            J->out[2 * o + 1] = pi + (wr * qi + wi * qr);
-        }
-    }
+    team_arg* A = (team_arg*)a;
+    team*     T = A->T;
+    pthread_mutex_lock(&T->mu);
+    while (T->state == 0) pthread_cond_wait(&T->cv, &T->mu);
+    const int go = T->state == 1;
+    pthread_mutex_unlock(&T->mu);
+    if (go) T->fn(T, A->tid, T->ctx);
     return NULL;
 }
 
-/* butterfly_fft (main.cpp:612-664): log2N horizontal stages then log2N
- * vertical stages, ping-ponging between data (pingpong0) and scratch
- * (pingpong1).  After the even number of stages the result is back in data,
- * which is where inversion reads it (main.cpp:667-680 with u_PingPong = 0). */
-int orc_butterfly_fft2d(int N, const double* tw, double* data, double* scratch, int nthreads)
+/* Runs fn on nthreads workers; returns -1 (having run nothing) when a thread
+ * cannot be created. */
+static int team_run(int nthreads, void (*fn)(team*, int, void*), void* ctx)
 {
-    if (N < 2 || (N & (N - 1)) || !tw || !data || !scratch) return -1;
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 256) nthreads = 256;
-    int nb = orc_log2n(N);
-    int pingpong = 0;
+    team T;
+    T.nthreads = nthreads;
+    T.fn = fn;
+    T.ctx = ctx;
+    T.state = 0;
+    if (pthread_barrier_init(&T.bar, NULL, (unsigned)nthreads)) return -1;
+    pthread_mutex_init(&T.mu, NULL);
+    pthread_cond_init(&T.cv, NULL);
     pthread_t th[256];
-    bfly_job jobs[256];
-    for (int dir = 0; dir < 2; ++dir) {
+    team_arg  args[256];
+    int started = 1;
+    for (int t = 1; t < nthreads; ++t, ++started) {
+        args[t] = (team_arg){&T, t};
+        if (pthread_create(&th[t], NULL, team_main, &args[t])) break;
+    }
+    pthread_mutex_lock(&T.mu);
+    T.state = started == nthreads ? 1 : -1;
+    pthread_cond_broadcast(&T.cv);
+    pthread_mutex_unlock(&T.mu);
+    if (T.state == 1) fn(&T, 0, ctx);
+    for (int t = 1; t < started; ++t) pthread_join(th[t], NULL);
+    pthread_cond_destroy(&T.cv);
+    pthread_mutex_destroy(&T.mu);
+    pthread_barrier_destroy(&T.bar);
+    return T.state == 1 ? 0 : -1;
+}
+
+static void team_sync(team* T)
+{
+    if (T->nthreads > 1) pthread_barrier_wait(&T->bar);
+}
+
+/* rows [*y0, *y1) of N owned by worker tid */
+static void team_rows(const team* T, int tid, int N, int* y0, int* y1)
+{
+    const int per = (N + T->nthreads - 1) / T->nthreads;
+    *y0 = tid * per < N ? tid * per : N;
+    *y1 = *y0 + per < N ? *y0 + per : N;
+}
+
+/* One butterfly dispatch over rows [y0, y1) (butterfly_cs.glsl:54-91
+ * horizontal, :95-132 vertical): out = in[a] + w * in[b] with
+ * (w, a, b) = twiddle[stage][x or y]. */
+static void bfly_rows(int N, const double* tw, int stage, int dir, const double* in, double* out, int y0, int y1)
+{
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < N; ++x) {
+            const double* d = tw + ((size_t)stage * N + (dir == 0 ? x : y)) * 4;
+            int a = (int)d[2], b = (int)d[3];
+            size_t ip, iq;
+            if (dir == 0) { ip = (size_t)y * N + a; iq = (size_t)y * N + b; }
+            else          { ip = (size_t)a * N + x; iq = (size_t)b * N + x; }
+            double pr = in[2 * ip], pi = in[2 * ip + 1];
+            double qr = in[2 * iq], qi = in[2 * iq + 1];
+            double wr = d[0], wi = d[1];
+            size_t o = (size_t)y * N + x;
+            out[2 * o]     = pr + (wr * qr - wi * qi);
+            out[2 * o + 1] = pi + (wr * qi + wi * qr);
+        }
+    }
+}
+
+/* butterfly_fft (main.cpp:612-664) as run by one team worker: log2N
+ * horizontal stages then log2N vertical stages, ping-ponging between data
+ * (pingpong0) and scratch (pingpong1), a barrier after every stage (the
+ * reference's glMemoryBarrier, main.cpp:636,657).  After the even number of
+ * stages the result is back in data, which is where inversion reads it
+ * (main.cpp:667-680 with u_PingPong = 0). */
+static void bfly_team(team* T, int tid, int N, const double* tw, double* data, double* scratch)
+{
+    int y0, y1;
+    team_rows(T, tid, N, &y0, &y1);
+    const int nb = orc_log2n(N);
+    int pingpong = 0;
+    for (int dir = 0; dir < 2; ++dir)
         for (int s = 0; s < nb; ++s) {
             const double* in  = pingpong == 0 ? data : scratch;
             double*       out = pingpong == 0 ? scratch : data;
-            int per = (N + nthreads - 1) / nthreads;
-            int nt = 0;
-            for (int t = 0; t < nthreads; ++t) {
-                int r0 = t * per, r1 = r0 + per > N ? N : r0 + per;
-                if (r0 >= r1) break;
-                jobs[t] = (bfly_job){N, s, dir, r0, r1, tw, in, out};
-                nt++;
-            }
-            if (nt == 1) {
-                bfly_run(&jobs[0]);
-            } else {
-                for (int t = 1; t < nt; ++t) pthread_create(&th[t], NULL, bfly_run, &jobs[t]);
-                bfly_run(&jobs[0]);
-                for (int t = 1; t < nt; ++t) pthread_join(th[t], NULL);
-            }
+            bfly_rows(N, tw, s, dir, in, out, y0, y1);
+            team_sync(T);
             pingpong ^= 1;
         }
-    }
-    return 0;
+}
+
+typedef struct {
+    int           N;
+    const double* tw;
+    double*       data;
+    double*       scratch;
+} bfly_ctx;
+
+static void bfly_worker(team* T, int tid, void* c)
+{
+    bfly_ctx* C = (bfly_ctx*)c;
+    bfly_team(T, tid, C->N, C->tw, C->data, C->scratch);
+}
+
+int orc_butterfly_fft2d(int N, const double* tw, double* data, double* scratch, int nthreads)
+{
+    if (N < 2 || (N & (N - 1)) || !tw || !data || !scratch) return -1;
+    bfly_ctx C = {N, tw, data, scratch};
+    return team_run(nthreads, bfly_worker, &C);
 }
 
 /* ------------------------------------------------------------------ A6 */
 
 /* inversion_cs.glsl:25-43: perm = (-1)^(x+y); out = perm * Re(in) / N^2. */
-void orc_inversion(int N, const double* data, double* out)
+static void inversion_rows(int N, const double* data, double* out, int y0, int y1)
 {
     const double inv = 1.0 / ((double)N * (double)N);
-    for (int y = 0; y < N; ++y)
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < N; ++x) {
             size_t i = (size_t)y * N + x;
             double perm = ((x + y) & 1) ? -1.0 : 1.0;
             out[i] = perm * (data[2 * i] * inv);
         }
 }
+
+void orc_inversion(int N, const double* data, double* out) { inversion_rows(N, data, out, 0, N); }
 
 /* ------------------------------------------------------------------ A7 */
 
@@ -216,9 +290,9 @@ static inline double tap_bilinear(int N, const double* h, int x, int y)
 
 /* normal_map_cs.glsl:24-54.  Taps z0..z7 at (x-1,y-1) (x,y-1) (x+1,y-1)
  * (x-1,y) (x+1,y) (x-1,y+1) (x,y+1) (x+1,y+1); Sobel; ny = 1; normalize. */
-void orc_normal_map(int N, const double* h, int mode, double* out)
+static void normal_rows(int N, const double* h, int mode, double* out, int y0, int y1)
 {
-    for (int y = 0; y < N; ++y)
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < N; ++x) {
             double z[8];
             static const int off[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
@@ -234,6 +308,8 @@ void orc_normal_map(int N, const double* h, int mode, double* out)
             o[0] = nx * inv; o[1] = ny * inv; o[2] = nz * inv; o[3] = 1.0;     /* :53 */
         }
 }
+
+void orc_normal_map(int N, const double* h, int mode, double* out) { normal_rows(N, h, mode, out, 0, N); }
 
 /* ------------------------------------------------------------------ noise */
 
@@ -380,12 +456,11 @@ int orc_h0_from_noise(const orc_params* p, const uint8_t* noise4, double* h0)
  * C(h0mk) e^{-i w t} where C is the identity in conj_mode 0 (the reference's
  * conjugate() returns its argument unchanged, :42-48) and complex conjugation
  * in conj_mode 1.  Hx = -i (kx/|k|) H, Hz = -i (ky/|k|) H. */
-int orc_spectrum_t(const orc_params* p, const double* h0, double t,
-                   double* hy, double* hx, double* hz)
+static void spectrum_rows(const orc_params* p, const double* h0, double t, double* hy, double* hx, double* hz,
+                          int y0, int y1)
 {
-    if (check_params(p) || !h0) return -1;
     const int N = p->N;
-    for (int ty = 0; ty < N; ++ty)
+    for (int ty = y0; ty < y1; ++ty)
         for (int tx = 0; tx < N; ++tx) {
             double kx, ky, kmag;
             wave_vector(p, tx, ty, &kx, &ky, &kmag);
@@ -402,6 +477,13 @@ int orc_spectrum_t(const orc_params* p, const double* h0, double t,
             if (hx) { hx[2 * i] = mx * Hi; hx[2 * i + 1] = -mx * Hr; }  /* :113-118 */
             if (hz) { hz[2 * i] = mz * Hi; hz[2 * i + 1] = -mz * Hr; }  /* :121-126 */
         }
+}
+
+int orc_spectrum_t(const orc_params* p, const double* h0, double t,
+                   double* hy, double* hx, double* hz)
+{
+    if (check_params(p) || !h0) return -1;
+    spectrum_rows(p, h0, t, hy, hx, hz, 0, p->N);
     return 0;
 }
 
@@ -410,10 +492,11 @@ int orc_spectrum_t(const orc_params* p, const double* h0, double t,
 /* J = (1 - l dDx/dx)(1 - l dDz/dz) - l^2 (dDx/dz)(dDz/dx) for the world
  * displacement (-l Dx, -l Dz) applied at render (grid_tes.glsl:60-62);
  * derivatives are periodic central differences with spacing L/N. */
-void orc_jacobian(int N, double L, double lambda, const double* dx, const double* dz, double* out)
+static void jacobian_rows(int N, double L, double lambda, const double* dx, const double* dz, double* out, int y0,
+                          int y1)
 {
     const double inv2d = (double)N / (2.0 * L);
-    for (int y = 0; y < N; ++y)
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < N; ++x) {
             double dxdx = (tap_nearest(N, dx, x + 1, y) - tap_nearest(N, dx, x - 1, y)) * inv2d;
             double dxdz = (tap_nearest(N, dx, x, y + 1) - tap_nearest(N, dx, x, y - 1)) * inv2d;
@@ -422,6 +505,11 @@ void orc_jacobian(int N, double L, double lambda, const double* dx, const double
             out[(size_t)y * N + x] =
                 (1.0 - lambda * dxdx) * (1.0 - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
         }
+}
+
+void orc_jacobian(int N, double L, double lambda, const double* dx, const double* dz, double* out)
+{
+    jacobian_rows(N, L, lambda, dx, dz, out, 0, N);
 }
 
 /* Render export (no reference counterpart; what grid_tes.glsl:58-62 applies per
@@ -599,7 +687,38 @@ int orc_fit_param_grad(const orc_params* p, const double* grad_h0, double grad[3
 /* ------------------------------------------------------------------ frame */
 
 /* update() (main.cpp:240-244): tilde_h0_t, butterfly_fft for dy, dx, dz,
- * generate_normal_map; plus the Jacobian. */
+ * generate_normal_map; plus the Jacobian.  With nthreads > 1 every pass is
+ * split by rows over one team of threads (each pass waits for the one before
+ * at the team's barrier); the results do not depend on nthreads. */
+typedef struct {
+    const orc_params* p;
+    const double*     h0;
+    double            t;
+    const double*     tw;
+    double*           f[3];
+    double*           scratch;
+    double*           outs[3];
+    double*           normals;
+    double*           jacobian;
+} frame_ctx;
+
+static void frame_worker(team* T, int tid, void* c)
+{
+    frame_ctx* F = (frame_ctx*)c;
+    const int  N = F->p->N;
+    int y0, y1;
+    team_rows(T, tid, N, &y0, &y1);
+    spectrum_rows(F->p, F->h0, F->t, F->f[0], F->f[1], F->f[2], y0, y1);
+    team_sync(T);
+    for (int i = 0; i < 3; ++i) {
+        bfly_team(T, tid, N, F->tw, F->f[i], F->scratch);
+        inversion_rows(N, F->f[i], F->outs[i], y0, y1);
+    }
+    team_sync(T);
+    if (F->normals) normal_rows(N, F->outs[0], F->p->normal_mode, F->normals, y0, y1);
+    if (F->jacobian) jacobian_rows(N, F->p->L, F->p->choppiness, F->outs[1], F->outs[2], F->jacobian, y0, y1);
+}
+
 int orc_frame(const orc_params* p, const double* h0, double t,
               double* height, double* dx, double* dz, double* normals, double* jacobian,
               int nthreads)
@@ -617,17 +736,11 @@ int orc_frame(const orc_params* p, const double* h0, double t,
     int rc = -1;
     if (!tw || !f[0] || !f[1] || !f[2] || !scratch || !hbuf || !xbuf || !zbuf) goto done;
     orc_twiddle_factors(N, tw);
-    orc_spectrum_t(p, h0, t, f[0], f[1], f[2]);
-    double* outs[3] = {hbuf, xbuf, zbuf};
-    for (int i = 0; i < 3; ++i) {
-        orc_butterfly_fft2d(N, tw, f[i], scratch, nthreads);
-        orc_inversion(N, f[i], outs[i]);
-    }
+    frame_ctx F = {p, h0, t, tw, {f[0], f[1], f[2]}, scratch, {hbuf, xbuf, zbuf}, normals, jacobian};
+    if (team_run(nthreads, frame_worker, &F)) goto done;
     if (height) memcpy(height, hbuf, sizeof(double) * nn);
     if (dx) memcpy(dx, xbuf, sizeof(double) * nn);
     if (dz) memcpy(dz, zbuf, sizeof(double) * nn);
-    if (normals) orc_normal_map(N, hbuf, p->normal_mode, normals);
-    if (jacobian) orc_jacobian(N, p->L, p->choppiness, xbuf, zbuf, jacobian);
     rc = 0;
 done:
     free(tw); free(f[0]); free(f[1]); free(f[2]); free(scratch);

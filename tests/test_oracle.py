@@ -381,3 +381,15 @@ def test_render_normal_is_orthogonal_to_the_displaced_tangents():
     assert np.allclose(disp[..., 0], -lam * dx) and np.allclose(disp[..., 2], -lam * dz)
     # J agrees with the Jacobian function
     assert np.allclose(disp[..., 3], oracle.jacobian(N, L, lam, dx, dz))
+
+
+@pytest.mark.parametrize("nthreads", [2, 3, 7])
+def test_threaded_frame_equals_single_thread(nthreads):
+    """The frame split over a team of threads (every pass by rows, a barrier
+    between passes) computes each texel exactly as one thread does."""
+    p = oracle.make_params(N=64, L=200.0, wind_speed=20.0, wind_dir=(1.0, 0.4), amplitude=3.0, choppiness=1.1)
+    h0 = oracle.h0(p)
+    one = oracle.frame(p, h0, 1.7, nthreads=1)
+    many = oracle.frame(p, h0, 1.7, nthreads=nthreads)
+    for k in one:
+        np.testing.assert_array_equal(one[k], many[k], err_msg=k)
