@@ -283,13 +283,13 @@ void pick_schedule(ocn_handle* h, int nframes, int* cv, int* rv)
     *rv = d.second;
 }
 
-void frame_args(const ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride,
-                ocn::FrameArgs* a)
+void frame_args(const ocn_handle* h, float2* z1, float2* z2, const double* t, int nframes, const ocn_fields* out,
+                size_t out_stride, ocn::FrameArgs* a)
 {
     const size_t nn = (size_t)h->p.N * h->p.N;
     a->h0 = h->h0;
-    a->z1 = h->z1;
-    a->z2 = h->z2;
+    a->z1 = z1;
+    a->z2 = z2;
     for (int i = 0; i < 3; ++i) a->tw[i] = h->tw[i];
     a->sc = spec_const(h->p);
     for (int f = 0; f < nframes; ++f) a->fb.t_rev[f] = t[f] / (2.0 * kPi);
@@ -325,17 +325,58 @@ int check_fields(const ocn_fields* out)
     return OCN_OK;
 }
 
-// Launches `nframes` (<= kMaxFrames) frames with the given stages.
-int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride, int stages,
-               int cv, int rv, hipStream_t s)
+// Launches `nframes` (<= kMaxFrames) frames with the given stages, with the
+// intermediates at z1 / z2 (room for nframes frames).
+int run_frames_z(ocn_handle* h, float2* z1, float2* z2, const double* t, int nframes, const ocn_fields* out,
+                 size_t out_stride, int stages, int cv, int rv, hipStream_t s)
 {
-    if (nframes > h->zcap) return fail(OCN_ESTATE, "internal: %d frames exceed the reserved %d", nframes, h->zcap);
     ocn::FrameArgs a{};
-    frame_args(h, t, nframes, out, out_stride, &a);
+    frame_args(h, z1, z2, t, nframes, out, out_stride, &a);
     a.cols_variant = cv;
     a.rows_variant = rv;
     hipError_t e = ocn::launch_frame(h->p.N, a, stages, s);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "update launch");
+}
+
+// The same on the handle's own intermediates.
+int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* out, size_t out_stride, int stages,
+               int cv, int rv, hipStream_t s)
+{
+    if (nframes > h->zcap) return fail(OCN_ESTATE, "internal: %d frames exceed the reserved %d", nframes, h->zcap);
+    return run_frames_z(h, h->z1, h->z2, t, nframes, out, out_stride, stages, cv, rv, s);
+}
+
+// Frames t[0..nframes) into consecutive output slices in launches of at
+// most `chunk` frames, intermediates at z1 / z2 (room for `chunk` frames).
+int frames_in_chunks(ocn_handle* h, float2* z1, float2* z2, const double* t_host, int nframes,
+                     const ocn_fields* out, int stage, int chunk, hipStream_t s)
+{
+    const size_t nn = (size_t)h->p.N * h->p.N;
+    int cv, rv, rc;
+    pick_schedule(h, std::min(nframes, chunk), &cv, &rv);
+    for (int f0 = 0; f0 < nframes; f0 += chunk) {
+        const int nf = std::min(chunk, nframes - f0);
+        ocn_fields sl{out->height + f0 * nn,
+                      out->dx + f0 * nn,
+                      out->dz + f0 * nn,
+                      out->normals ? out->normals + f0 * nn * 4 : nullptr,
+                      out->jacobian ? out->jacobian + f0 * nn : nullptr,
+                      out->displacement ? out->displacement + f0 * nn * 4 : nullptr,
+                      out->render_normals ? out->render_normals + f0 * nn * 4 : nullptr};
+        if ((rc = run_frames_z(h, z1, z2, t_host + f0, nf, &sl, nn, stage, cv, rv, s))) return rc;
+    }
+    return OCN_OK;
+}
+
+// Checks shared by the frame entry points (before any HIP call).
+int check_frames(const ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out)
+{
+    int rc = check_handle(h);
+    if (rc || (rc = check_fields(out))) return rc;
+    if (!t_host || nframes < 0) return fail(OCN_EINVAL, "t_host must be non-NULL and nframes >= 0");
+    for (int f = 0; f < nframes; ++f)
+        if ((rc = ready_for_frame(h, t_host[f]))) return rc;
+    return OCN_OK;
 }
 
 }  // namespace
@@ -599,6 +640,14 @@ int ocn_create(const ocn_params* p, ocn_handle** out)
         delete h;
         return rc;
     }
+    // set-up of the static scratch of ocn_update_frames_noalloc on this device
+    // (its address is looked up once per device, never in the frame call)
+    if ((e = ocn::static_scratch_resolve(h->device)) != hipSuccess) {
+        rc = hip_fail(e, "hipGetSymbolAddress(static scratch)");
+        free_all(h);
+        delete h;
+        return rc;
+    }
     *out = h;
     return OCN_OK;
 }
@@ -716,32 +765,36 @@ int ocn_generate_normal_map(ocn_handle* h, void* stream)
 int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, int stage,
                      void* stream)
 {
-    int rc = check_handle(h);
-    if (rc || (rc = check_fields(out))) return rc;
-    if (!t_host || nframes < 0) return fail(OCN_EINVAL, "t_host must be non-NULL and nframes >= 0");
+    int rc = check_frames(h, t_host, nframes, out);
+    if (rc) return rc;
     if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
         return fail(OCN_EINVAL, "stage %d is not a combination of OCN_STAGE_* bits", stage);
-    for (int f = 0; f < nframes; ++f)
-        if ((rc = ready_for_frame(h, t_host[f]))) return rc;
     if (nframes == 0) return OCN_OK;
     if ((rc = set_device(h))) return rc;
-    const size_t nn = (size_t)h->p.N * h->p.N;
     // launches of at most the reserved frame count: nothing is allocated here
     const int chunk = std::min(ocn::kMaxFrames, h->zcap);
-    int cv, rv;
-    pick_schedule(h, std::min(nframes, chunk), &cv, &rv);
-    for (int f0 = 0; f0 < nframes; f0 += chunk) {
-        const int nf = std::min(chunk, nframes - f0);
-        ocn_fields s{out->height + f0 * nn,
-                     out->dx + f0 * nn,
-                     out->dz + f0 * nn,
-                     out->normals ? out->normals + f0 * nn * 4 : nullptr,
-                     out->jacobian ? out->jacobian + f0 * nn : nullptr,
-                     out->displacement ? out->displacement + f0 * nn * 4 : nullptr,
-                     out->render_normals ? out->render_normals + f0 * nn * 4 : nullptr};
-        if ((rc = run_frames(h, t_host + f0, nf, &s, nn, stage, cv, rv, (hipStream_t)stream))) return rc;
-    }
-    return OCN_OK;
+    return frames_in_chunks(h, h->z1, h->z2, t_host, nframes, out, stage, chunk, (hipStream_t)stream);
+}
+
+int ocn_update_frames_noalloc(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream)
+{
+    int rc = check_frames(h, t_host, nframes, out);
+    if (rc) return rc;
+    const size_t nn        = (size_t)h->p.N * h->p.N;
+    const size_t per_frame = nn * (sizeof(float2) + sizeof(float2) / 2);  // Z1 + Z2: 12 bytes per texel
+    if (per_frame > ocn::kStaticScratchBytes)
+        return fail(OCN_EINVAL,
+                    "ocn_update_frames_noalloc: one N=%d frame needs %zu bytes of intermediates, more than the "
+                    "%zu-byte static scratch (N <= 2048 fit)",
+                    h->p.N, per_frame, ocn::kStaticScratchBytes);
+    if (nframes == 0) return OCN_OK;
+    if ((rc = set_device(h))) return rc;
+    char* base = static_cast<char*>(ocn::static_scratch_ptr(h->device));
+    if (!base) return fail(OCN_ESTATE, "the static scratch of device %d was not resolved at ocn_create", h->device);
+    const int chunk = (int)std::min<size_t>(ocn::kMaxFrames, ocn::kStaticScratchBytes / per_frame);
+    float2*   z1    = reinterpret_cast<float2*>(base);
+    float2*   z2    = reinterpret_cast<float2*>(base + (size_t)chunk * nn * sizeof(float2));
+    return frames_in_chunks(h, z1, z2, t_host, nframes, out, OCN_STAGE_ALL, chunk, (hipStream_t)stream);
 }
 
 int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)

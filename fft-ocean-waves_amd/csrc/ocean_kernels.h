@@ -62,6 +62,17 @@ struct FrameArgs {
     float         inv2d;      // N / (2 L)
 };
 
+// Static intermediates of ocn_update_frames_noalloc: one file-scope device
+// array per GPU (ocean_kernels.hip), 48 MiB = Z1 + Z2 of one N=2048 frame,
+// four N=1024 frames, sixteen N=512 frames.
+constexpr size_t kStaticScratchBytes = (size_t)48 << 20;
+constexpr int    kMaxScratchDevices  = 64;
+// Looks up the array's address on `device` (which must be current) once and
+// keeps it; a setup call (it may load the code object on that device).
+hipError_t static_scratch_resolve(int device);
+// The address static_scratch_resolve kept for `device`, or null.
+void*      static_scratch_ptr(int device);
+
 hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s);
 hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, float2* hz, const SpecConst& sc,
                              double t_rev, hipStream_t s);

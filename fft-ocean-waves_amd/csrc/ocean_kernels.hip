@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "fft_core.hpp"
@@ -640,6 +641,42 @@ hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const
 #define CALL(n) launch_ifft2_n<n>(src, tmp, out, tw, s)
     OCN_DISPATCH_N(N, CALL, hipErrorInvalidValue)
 #undef CALL
+}
+
+// ------------------------------------------------------------------ static scratch
+
+// Z1 and Z2 of ocn_update_frames_noalloc, for callers that cannot allocate
+// device memory.  Each GPU the library runs on holds its own copy of this
+// array until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a host-side stand-in, not a device
+// address): static_scratch_resolve looks up the device address once per
+// device with hipGetSymbolAddress, and the frame launches take that pointer.
+// Nothing in it needs clearing: the column pass writes every element of Z1
+// and Z2 before the row pass of the same launch reads them.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void*      g_scratch_addr[kMaxScratchDevices] = {};
+}  // namespace
+
+hipError_t static_scratch_resolve(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_addr[device]) return hipSuccess;
+    void*      p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+    return hipSuccess;
+}
+
+void* static_scratch_ptr(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_addr[device];
 }
 
 hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s)

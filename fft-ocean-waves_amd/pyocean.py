@@ -29,7 +29,7 @@ EXPORTS = [
     "ocn_tilde_h0_k", "ocn_tilde_h0_k_from_noise", "ocn_generate_twiddle_factors",
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
-    "ocn_reserve_frames", "ocn_frames_reserved",
+    "ocn_reserve_frames", "ocn_frames_reserved", "ocn_update_frames_noalloc",
     "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
@@ -115,6 +115,7 @@ def lib():
             "ocn_update_stage": [vp, ctypes.POINTER(ctypes.c_double), i32, F, i32, vp],
             "ocn_update_frames": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_reserve_frames": [vp, i32],
+            "ocn_update_frames_noalloc": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_frames_reserved": [vp, ctypes.POINTER(i32)],
             "ocn_autotune": [vp, i32, i32, vp],
             "ocn_set_schedule": [vp, i32, i32],

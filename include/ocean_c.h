@@ -228,6 +228,20 @@ int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn
  * never allocates, so reserve once before the first multi-frame update. */
 int ocn_update_frames(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream);
 
+/* ocn_update_frames with no device allocation of its own, for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No reference counterpart.  The frames'
+ * intermediates live in a static 48 MiB device array that each GPU holds
+ * until the process ends (its address is looked up by ocn_create, never
+ * here), not in the handle's buffers, so no reservation is needed: launches
+ * hold min(32, 48 MiB / (12 N^2)) frames (N=2048: 1, N=1024: 4, N=512: 16,
+ * N <= 256: 32).  N=4096, whose single frame needs 192 MiB, is rejected
+ * with OCN_EINVAL.  Outputs and their layout are those of ocn_update_frames.
+ * Every call on one GPU shares the array: calls that overlap in time on the
+ * same GPU (from any handle, stream or thread) race and must be ordered by
+ * the caller. */
+int ocn_update_frames_noalloc(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream);
+
 /* Sizes the handle's intermediate buffers for launches of up to nframes frames
  * (1..32; 12 bytes per texel and frame): a setup call, which allocates and,
  * when it grows the buffers, waits for the device and frees the old ones.
