@@ -61,6 +61,14 @@ MEASURED_TRAFFIC = {
 }
 
 
+def kernel_names(sched):
+    """Kernel names of a (column, row) schedule as ocn_get_schedule reports it."""
+    cv, rv = sched["cols_variant"], sched["rows_variant"]
+    cols = "k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_lds"
+    rows = "k_rows" if sched["rows_band"] == 0 else "k_rows_nrm_par" if rv >= 6 else "k_rows_nrm"
+    return cols, rows
+
+
 def workload_params():
     import pyocean
     th = math.radians(30.0)
@@ -532,9 +540,8 @@ def main():
     reps = 40
     sched = oc.schedule(lf)
     fused = sched["rows_band"] > 0  # normal map + Jacobian inside the row kernel
-    cols_name = "k_cols" if sched["cols_variant"] < 7 else "k_cols_lds" if sched["cols_variant"] < 10 else "k_cols_lds2"
-    stages = [(cols_name, pyocean.OCN_STAGE_COLUMNS),
-              ("k_rows_nrm" if fused else "k_rows", pyocean.OCN_STAGE_ROWS)]
+    cols_name, rows_name = kernel_names(sched)
+    stages = [(cols_name, pyocean.OCN_STAGE_COLUMNS), (rows_name, pyocean.OCN_STAGE_ROWS)]
     if not fused:
         stages.append(("k_normals", pyocean.OCN_STAGE_NORMALS))
     for name, stage in stages:
@@ -546,7 +553,8 @@ def main():
         hip.record(e1, stream)
         stage_ms[name] = hip.elapsed_ms(e0, e1) / reps
     dominant = max(stage_ms, key=stage_ms.get)
-    dom_bytes = CANON_STAGE["k_cols" if dominant.startswith("k_cols") else dominant] * nn * lf
+    dom_bytes = CANON_STAGE["k_cols" if dominant.startswith("k_cols") else
+                            "k_rows_nrm" if dominant.startswith("k_rows_nrm") else dominant] * nn * lf
     achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
     tb = MEASURED_TRAFFIC.get((dominant, sched["cols_variant"], sched["rows_variant"]))

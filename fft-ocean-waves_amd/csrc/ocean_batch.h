@@ -54,7 +54,8 @@ __host__ __device__ constexpr int batch_cols_per_block(int N)
 {
     return kBatchColsWidth / N < N / 2 ? kBatchColsWidth / N : N / 2;
 }
-inline int batch_cols_items_per_member(int N) { return (N / 2) / batch_cols_per_block(N) + 1; }
+// strips of cols_lds_strip: the self-mirror columns 0 and N/2 share one slot
+inline int batch_cols_items_per_member(int N) { return (N / 2) / batch_cols_per_block(N); }
 // groups per row block: one band per block in the wide launch
 inline int batch_rows_groups_per_block(int N)
 {

@@ -130,19 +130,25 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 // autotuner may replace them.  Column variants: 0 (16,1) 1 (16,2) 2 (16,4)
 // 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block) of k_cols,
 // 7..9 k_cols_lds with 1, 2, 4 columns per block, 10..12 k_cols_lds2 with 2, 4,
-// 8 columns per block.  Row variants: 0 radix 16,
+// 8 columns per block, 13..14 k_cols_lds with 1, 2 columns and twice the
+// threads.  Row variants: 0 radix 16,
 // 1 radix 8 (each followed by the normal-map kernel), 2..5 the fused
-// row + normals kernel with bands of 4, 8, 16, 32 rows.
+// row + normals walk with bands of 4, 8, 16, 32 rows, 6..7 the fused
+// parallel bands of 4 and 2 rows, 8 the parallel bands of 2 rows with twice
+// the threads.  Single frames at N = 256..1024 take the parallel bands after
+// the column strips with twice the threads (graph-timed sweep of every pair
+// in one box: N=1024 31.7 -> 27.4 us, N=512 19.9 -> 12.7 us, N=256 11.9 ->
+// 8.8 us against the earlier single-frame defaults).
 std::pair<int, int> default_schedule(int N, int nframes)
 {
     const bool batch = nframes > 1;
     int cv, rv;
     switch (N) {
-    case 64: cv = 9; rv = 1; break;
-    case 128: cv = batch ? 9 : 8; rv = batch ? 2 : 1; break;
-    case 256: cv = batch ? 9 : 7; rv = 1; break;
-    case 512: cv = batch ? 5 : 8; rv = batch ? 4 : 1; break;
-    case 1024: cv = 9; rv = batch ? 5 : 1; break;
+    case 64: cv = batch ? 9 : 14; rv = batch ? 1 : 8; break;
+    case 128: cv = batch ? 9 : 14; rv = batch ? 2 : 8; break;
+    case 256: cv = batch ? 9 : 13; rv = batch ? 1 : 8; break;
+    case 512: cv = batch ? 5 : 14; rv = batch ? 4 : 8; break;
+    case 1024: cv = batch ? 9 : 14; rv = batch ? 5 : 6; break;
     case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 2; break;  // single: fused bands of 4 rows (98.3 -> 95.7 us)
     // 4096: k_cols radix 16 with 2 columns per block and fused bands (32 rows
     // for frame batches, 16 for single frames): per frame 419 -> 377 us and

@@ -242,9 +242,18 @@ __host__ __device__ constexpr int strip_group()
 
 // ------------------------------------------------------------------ column pass: one strip
 
-// LDS-staged column transforms of one strip: C half-plane columns starting at
-// tx = strip*C (tx <= N/2), block of 3*C*T threads (T = N/16), frame time
-// t_rev; z1/z2 point at the frame's intermediate, lds holds 3*C*lds_len(N).
+// Strips of cols_lds_strip per frame: the N/2 + 1 half-plane columns
+// 0 .. N/2 in strips of C, with the two self-mirror columns 0 and N/2 sharing
+// one transform (below), i.e. N/2 column slots.
+template <int N, int C>
+__host__ __device__ constexpr int lds_strips()
+{
+    return (N / 2) / C;
+}
+
+// LDS-staged column transforms of one strip: the C half-plane columns
+// tx = strip*C + c, block of 3*C*T threads (T = N/16), frame time t_rev;
+// z1/z2 point at the frame's intermediate, lds holds 3*C*lds_len(N).
 // Phase A: every thread evaluates Hermitian parts (consecutive threads take
 // consecutive columns, then rows) into per-(field, column) LDS buffers.
 // Phase B: each of the 3C transforms runs inside one T-thread group (Sync:
@@ -252,32 +261,58 @@ __host__ __device__ constexpr int strip_group()
 // result in natural order in the same buffer.  Phase C: every thread stores
 // Z1 = G_h + i G_dx (columns tx and N-tx) and Z2 = G_dz(2p) + i G_dz(2p+1),
 // both in two-row tiles (zidx).
+// The self-mirror columns 0 and N/2 (their own mirrors) have Hermitian folded
+// spectra, A(-ty) = conj A(ty), so their column transforms are real: column
+// slot 0 of strip 0 transforms A_0 + i A_{N/2} and its result holds column 0 in
+// the real parts and column N/2 in the imaginary parts.  That leaves N/2
+// column slots, a whole number of strips (at N = 1024 and C = 2, exactly one
+// block per CU for a single frame instead of 257 blocks on 256 CUs).
 template <int N, int C, class Sync, int NB = 3 * C * (N / 16)>
 __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
                                                float2* __restrict__ z2, const float2* __restrict__ tw,
                                                const SpecConst& sc, double t_rev, int strip, float2* lds)
 {
-    // NB = threads of the block; threads from NT on (a larger block shared with
-    // other strip shapes) only take part in the barriers
+    // NB = threads of the block: all of them fold (phase A) and store (phase
+    // C); the 3C transforms take the first NT.  A block wider than NT (a
+    // single frame, whose few blocks leave the SIMDs short of waves, or the
+    // batch's block shared by every N) splits the folds and stores finer.
     constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
-    constexpr int ITERS = (C * N + NT - 1) / NT;
+    constexpr int ITERS = (C * N + NB - 1) / NB;
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
-    const bool active = NB == NT || (int)threadIdx.x < NT;
+    static_assert((N / 2) % C == 0, "strips of C columns tile the half plane");
+    const bool active = NB == NT || (int)threadIdx.x < NT;  // takes part in the transforms
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
-        const int k = threadIdx.x + it * NT;
-        if (active && k < C * N) {
+        const int k = threadIdx.x + it * NB;
+        if (k < C * N) {
             const int c   = k % C, ty = k / C;
-            const int txc = min(strip * C + c, N / 2);
-            const int mxc = (N - txc) & (N - 1);
+            const int tx  = strip * C + c;  // < N/2
+            const int mxc = (N - tx) & (N - 1);
             const int my  = (N - ty) & (N - 1);
             float2 ay, ax, az;
-            pair_fold(sc, h0[(size_t)ty * N + txc], h0[(size_t)my * N + mxc], txc - N / 2, ty - N / 2, mxc - N / 2,
+            pair_fold(sc, h0[(size_t)ty * N + tx], h0[(size_t)my * N + mxc], tx - N / 2, ty - N / 2, mxc - N / 2,
                       my - N / 2, t_rev, ay, ax, az);
             lds[(0 * C + c) * LEN + lds_pad(ty)] = ay;
             lds[(1 * C + c) * LEN + lds_pad(ty)] = ax;
             lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
+        }
+    }
+    if (strip == 0) {
+        // slot 0 += i * (the Nyquist column's fold), its N rows spread over the
+        // whole block (strip is the same for every thread of the block)
+        __syncthreads();
+        for (int ty = threadIdx.x; ty < N; ty += NB) {
+            const int my = (N - ty) & (N - 1);
+            float2 by, bx, bz;
+            pair_fold(sc, h0[(size_t)ty * N + N / 2], h0[(size_t)my * N + N / 2], 0, ty - N / 2, 0, my - N / 2, t_rev,
+                      by, bx, bz);
+            float2& ay = lds[0 * C * LEN + lds_pad(ty)];
+            float2& ax = lds[1 * C * LEN + lds_pad(ty)];
+            float2& az = lds[2 * C * LEN + lds_pad(ty)];
+            ay = make_float2(ay.x - by.y, ay.y + by.x);
+            ax = make_float2(ax.x - bx.y, ax.y + bx.x);
+            az = make_float2(az.x - bz.y, az.y + bz.x);
         }
     }
     __syncthreads();
@@ -295,30 +330,43 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     __syncthreads();
     // thread k stores Z1 tile m = k / C (rows 2m, 2m+1) and, for m < N/4, Z2
     // tile m (Z2 rows 2m, 2m+1, i.e. G_dz rows 4m..4m+3) of column c = k % C
-    constexpr int ITZ = (C * N / 2 + NT - 1) / NT;
+    constexpr int ITZ = (C * N / 2 + NB - 1) / NB;
 #pragma unroll
     for (int it = 0; it < ITZ; ++it) {
-        const int k = threadIdx.x + it * NT;
-        if (active && k < C * N / 2) {
-            const int c = k % C, m = k / C;
-            const int tx = strip * C + c;
-            if (tx <= N / 2) {
-                const int    mxc = (N - tx) & (N - 1);
-                const float2 y0  = lds[(0 * C + c) * LEN + lds_pad(2 * m)];
-                const float2 x0  = lds[(1 * C + c) * LEN + lds_pad(2 * m)];
-                const float2 y1  = lds[(0 * C + c) * LEN + lds_pad(2 * m + 1)];
-                const float2 x1  = lds[(1 * C + c) * LEN + lds_pad(2 * m + 1)];
-                float4*      t1  = ztile<N>(z1, m);
-                t1[tx] = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
-                if (mxc != tx) t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
-                if (m < N / 4) {  // Z2 row q pairs G_dz rows 2q and 2q+1
-                    const float2 a = lds[(2 * C + c) * LEN + lds_pad(4 * m)];
-                    const float2 b = lds[(2 * C + c) * LEN + lds_pad(4 * m + 1)];
-                    const float2 d = lds[(2 * C + c) * LEN + lds_pad(4 * m + 2)];
-                    const float2 e = lds[(2 * C + c) * LEN + lds_pad(4 * m + 3)];
-                    float4*      t2 = ztile<N>(z2, m);
-                    t2[tx] = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
-                    if (mxc != tx) t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
+        const int k = threadIdx.x + it * NB;
+        if (k < C * N / 2) {
+            const int    c   = k % C, m = k / C;
+            const int    tx  = strip * C + c;
+            const float2 y0  = lds[(0 * C + c) * LEN + lds_pad(2 * m)];
+            const float2 x0  = lds[(1 * C + c) * LEN + lds_pad(2 * m)];
+            const float2 y1  = lds[(0 * C + c) * LEN + lds_pad(2 * m + 1)];
+            const float2 x1  = lds[(1 * C + c) * LEN + lds_pad(2 * m + 1)];
+            float4*      t1  = ztile<N>(z1, m);
+            const bool   z2t = m < N / 4;  // Z2 row q pairs G_dz rows 2q and 2q+1
+            float2       a{}, b{}, d{}, e{};
+            if (z2t) {
+                a = lds[(2 * C + c) * LEN + lds_pad(4 * m)];
+                b = lds[(2 * C + c) * LEN + lds_pad(4 * m + 1)];
+                d = lds[(2 * C + c) * LEN + lds_pad(4 * m + 2)];
+                e = lds[(2 * C + c) * LEN + lds_pad(4 * m + 3)];
+            }
+            float4* t2 = ztile<N>(z2, m);
+            if (tx == 0) {
+                // packed self-mirror columns: real parts are column 0's (real)
+                // transforms, imaginary parts column N/2's
+                t1[0]     = make_float4(y0.x, x0.x, y1.x, x1.x);
+                t1[N / 2] = make_float4(y0.y, x0.y, y1.y, x1.y);
+                if (z2t) {
+                    t2[0]     = make_float4(a.x, b.x, d.x, e.x);
+                    t2[N / 2] = make_float4(a.y, b.y, d.y, e.y);
+                }
+            } else {
+                const int mxc = N - tx;
+                t1[tx]        = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
+                t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
+                if (z2t) {
+                    t2[tx]        = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
+                    t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
                 }
             }
         }

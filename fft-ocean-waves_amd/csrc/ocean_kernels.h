@@ -9,8 +9,8 @@
 namespace ocn {
 
 constexpr int kMaxFrames        = 32;  // frames per launch of the fused path
-constexpr int kNumColsVariants  = 13;  // column-pass (kernel, radix, columns/block) schedules
-constexpr int kNumRowsVariants  = 6;   // row-pass schedules (radix, fused band)
+constexpr int kNumColsVariants  = 15;  // column-pass (kernel, radix, columns/block) schedules
+constexpr int kNumRowsVariants  = 9;   // row-pass schedules (radix, fused band, parallel bands)
 
 // Per-ocean spectrum constants, FP32 for the kernels plus the FP64 pieces of
 // the phase.  Filled by the host from ocn_params.
@@ -86,5 +86,6 @@ bool       schedule_valid(int N, int cols_variant, int rows_variant);
 void       cols_variant_shape(int v, int* E, int* C);
 int        rows_variant_radix(int v);
 int        rows_variant_band(int v);  // 0: separate normal-map kernel; R: fused bands of R rows
+bool       rows_variant_parallel(int v);  // fused bands whose rows are transformed at once
 
 }  // namespace ocn

@@ -161,12 +161,13 @@ __global__ __launch_bounds__(C*(N / E)) void k_cols(const float4* __restrict__ h
 // block order of k_cols, with each strip done by cols_lds_strip (every thread
 // folds the spectrum into LDS, each transform runs inside one wave, every
 // thread stores).  Block = 3C*T threads.
-template <int N, int C>
-__global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __restrict__ h0,
-                                                               float2* __restrict__ z1, float2* __restrict__ z2,
-                                                               const float2* __restrict__ tw, SpecConst sc,
-                                                               FrameBatch fb)
+template <int N, int C, int W = 1>
+__global__ __launch_bounds__(W * 3 * C * (N / 16)) void k_cols_lds(const float4* __restrict__ h0,
+                                                                   float2* __restrict__ z1, float2* __restrict__ z2,
+                                                                   const float2* __restrict__ tw, SpecConst sc,
+                                                                   FrameBatch fb)
 {
+    // W = 2: twice the threads, the extra ones folding and storing only
     constexpr int T = N / 16;
     constexpr int G = strip_group<N, C>();
     static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
@@ -175,9 +176,9 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_lds(const float4* __r
     const int rest  = L / G;
     const int f     = rest % fb.nframes;
     const int strip = (rest / fb.nframes) * G + L % G;
-    if (strip > (N / 2) / C) return;  // padding of the last group (whole block)
-    cols_lds_strip<N, C, WaveSync>(h0, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, tw, sc,
-                                   fb.t_rev[f], strip, lds);
+    if (strip >= lds_strips<N, C>()) return;  // padding of the last group (whole block)
+    cols_lds_strip<N, C, WaveSync, W * 3 * C * T>(h0, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride,
+                                                  tw, sc, fb.t_rev[f], strip, lds);
 }
 
 // Two-pass variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
@@ -290,6 +291,135 @@ __global__ __launch_bounds__(N / E >= 64 ? N / E : 64, 2) void k_rows_nrm(
         z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, height + fo, dx + fo, dz + fo,
         normals ? normals + fo : nullptr, jac ? jac + fo : nullptr, tw, lds + g * lds_len(N), j, band, R, lambda,
         inv2d, disp ? disp + fo : nullptr, rnorm ? rnorm + fo : nullptr, dscale);
+}
+
+// ------------------------------------------------------------------ fused row pass, parallel bands
+
+// Row pass fused with the normal map (nearest taps) and the Jacobian for
+// launches of few frames: where the band walk of k_rows_nrm runs its rows one
+// after another (long bands, few groups), here one block owns a band of R
+// output rows [y0, y0 + R) and transforms all the rows its stencils need at
+// once: G1 = R + 2 Z1 rows (y0 - 1 .. y0 + R) and G2 = R/2 + 2 Z2 rows
+// (y0/2 - 1 .. y0/2 + R/2), one T-thread group (radix E) each.  The results
+// stay in the groups' LDS buffers in natural order; after one block barrier
+// every thread takes texels of the band, applies (-1)^(x+y) / N^2
+// (inversion_cs.glsl:35-41) to each value it reads and computes the Sobel
+// normal (normal_map_cs.glsl:37-53) and the Jacobian of the render
+// displacement (grid_tes.glsl:60-62) with the band walk's arithmetic.  1-D
+// grid of (N/R) x frames blocks; after the XCD remap neighbouring bands,
+// which read the same halo tiles, run on one XCD.
+template <int N, int E, int R>
+constexpr int par_groups()
+{
+    return (R + 2) + (R / 2 + 2);
+}
+
+template <int N, int E, int R, int W = 1>
+__global__ __launch_bounds__((W * par_groups<N, E, R>() * (N / E))) void k_rows_nrm_par(
+    const float2* __restrict__ z1, const float2* __restrict__ z2, float* __restrict__ height,
+    float* __restrict__ dx, float* __restrict__ dz, float4* __restrict__ normals, float* __restrict__ jac,
+    const float2* __restrict__ tw, FrameBatch fb, float lambda, float inv2d)
+{
+    // W = 2: twice the threads, the extra ones (after the G transform groups)
+    // taking part in the stencils only
+    constexpr int T = N / E, G1 = R + 2, G = par_groups<N, E, R>(), LEN = lds_len(N), NT = W * G * T;
+    static_assert(R % 2 == 0 && N % R == 0, "bands of an even number of rows");
+    static_assert(T % 64 == 0 || 64 % T == 0, "transform groups of whole waves or within one wave");
+    static_assert(W == 1 || T <= 64, "extra threads only beside wave-local transforms");
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    constexpr float scale = 1.0f / ((float)N * (float)N);
+    __shared__ float2 lds[G * LEN];
+    const int L    = xcd_logical(blockIdx.x, gridDim.x);
+    const int band = L % (N / R);
+    const int f    = L / (N / R);
+    const int y0   = band * R;
+    z1 += (size_t)f * fb.z1_stride;
+    z2 += (size_t)f * fb.z2_stride;
+    const size_t fo = (size_t)f * fb.out_stride;
+    height += fo;
+    dx += fo;
+    dz += fo;
+    if (normals) normals += fo;
+    if (jac) jac += fo;
+
+    if (W == 1 || (int)threadIdx.x < G * T) {
+        const int g = threadIdx.x / T, j = threadIdx.x % T;
+        // group g < G1: Z1 row y0 - 1 + g; else Z2 row y0/2 - 1 + (g - G1) (periodic)
+        const bool    is1 = g < G1;
+        const float2* src = is1 ? z1 : z2;
+        const int     row = is1 ? (y0 - 1 + g) & (N - 1) : ((y0 >> 1) - 1 + (g - G1)) & (N / 2 - 1);
+        float2        v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = src[zidx<N>(row, zpos_elem<N, E>(j, r))];
+        float2* buf = lds + g * LEN;
+        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+        Sync{}();
+#pragma unroll
+        for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+    }
+    __syncthreads();
+
+    // Stencils: each thread walks whole columns x of the band, keeping the
+    // signed values of x-1, x, x+1 for all band and halo rows in registers
+    // (each LDS value is read once per column).  The sign (-1)^(x+y) / N^2 of
+    // texel (x, y): rows y0 - 1 + g alternate starting from that of
+    // (x, y0 - 1) (y0 is even); columns x +- 1 have the opposite sign of x.
+    constexpr int G2 = G - G1;
+    for (int x = threadIdx.x; x < N; x += NT) {
+        const int   pl = lds_pad((x - 1) & (N - 1)), pm = lds_pad(x), pr = lds_pad((x + 1) & (N - 1));
+        const float s0 = ((x + 1) & 1) ? -scale : scale;  // (x, y0 - 1)
+        float2 hl[G1], hm[G1], hr[G1];  // (height, Dx) of rows y0 - 1 .. y0 + R at x - 1, x, x + 1
+#pragma unroll
+        for (int g = 0; g < G1; ++g) {
+            const float  sm = (g & 1) ? -s0 : s0;
+            const float2 l = lds[g * LEN + pl], m = lds[g * LEN + pm], r = lds[g * LEN + pr];
+            hl[g] = make_float2(-sm * l.x, -sm * l.y);
+            hm[g] = make_float2(sm * m.x, sm * m.y);
+            hr[g] = make_float2(-sm * r.x, -sm * r.y);
+        }
+        // Dz of rows y0 - 1 .. y0 + R: Z2 row y/2 (buffer G1 + (y+2)/2 - y0/2), part y & 1
+        float zl[G1], zm[G1], zr[G1];
+#pragma unroll
+        for (int q = 0; q < G2; ++q) {
+            const float2 l = lds[(G1 + q) * LEN + pl], m = lds[(G1 + q) * LEN + pm], r = lds[(G1 + q) * LEN + pr];
+            // Z2 row y0/2 - 1 + q holds rows y0 - 2 + 2q (re) and y0 - 1 + 2q (im), i.e. g = 2q - 1, 2q
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+                const int g = 2 * q - 1 + part;
+                if (g < 0 || g >= G1) continue;
+                const float sm = (g & 1) ? -s0 : s0;
+                zl[g] = -sm * (part ? l.y : l.x);
+                zm[g] = sm * (part ? m.y : m.x);
+                zr[g] = -sm * (part ? r.y : r.x);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int    g = k + 1;  // grid row y0 + k
+            const size_t o = (size_t)(y0 + k) * N + x;
+            if (normals) {
+                // v_s = h(y-1) + 2 h(y) + h(y+1), d_s = h(y-1) - h(y+1) at x-1, x, x+1;
+                // nx = v_s(x-1) - v_s(x+1), nz = d_s(x-1) + 2 d_s(x) + d_s(x+1)
+                const float vl  = hl[g - 1].x + 2.0f * hl[g].x + hl[g + 1].x;
+                const float vr  = hr[g - 1].x + 2.0f * hr[g].x + hr[g + 1].x;
+                const float nx  = vl - vr;
+                const float nz  = (hl[g - 1].x - hl[g + 1].x) + 2.0f * (hm[g - 1].x - hm[g + 1].x) +
+                                  (hr[g - 1].x - hr[g + 1].x);
+                const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
+                normals[o] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+            }
+            if (jac) {
+                const float dxdx = (hr[g].y - hl[g].y) * inv2d;
+                const float dzdx = (zr[g] - zl[g]) * inv2d;
+                const float dxdz = (hm[g + 1].y - hm[g - 1].y) * inv2d;
+                const float dzdz = (zm[g + 1] - zm[g - 1]) * inv2d;
+                jac[o] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+            }
+            height[o] = hm[g].x;
+            dx[o]     = hm[g].y;
+            dz[o]     = zm[g];
+        }
+    }
 }
 
 // ------------------------------------------------------------------ A7 normals (+ Jacobian)
@@ -432,39 +562,50 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // per N.  Row pass: radix E, three transforms per block.
 // kind 0: k_cols (register-resident, block-synchronised transforms);
 // kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only);
-// kind 2: k_cols_lds2 (the same in two passes through half the LDS).
+// kind 2: k_cols_lds2 (the same in two passes through half the LDS);
+// kind 3: k_cols_lds with twice the threads (folds and stores split finer).
 struct ColsVariant {
     int E, C, kind;
 };
 constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
                                                          {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1},
-                                                         {16, 2, 2}, {16, 4, 2}, {16, 8, 2}};
+                                                         {16, 2, 2}, {16, 4, 2}, {16, 8, 2}, {16, 1, 3}, {16, 2, 3}};
 // Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
-// band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused.
+// band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused;
+// par = k_rows_nrm_par (bands of R rows whose rows are transformed at once).
 struct RowsVariant {
-    int E, band;
+    int  E, band;
+    bool par;
+    int  W;  // parallel bands: threads per transform-group thread
 };
-constexpr RowsVariant kRowsVariants[kNumRowsVariants] = {{16, 0}, {8, 0}, {8, 4}, {8, 8}, {8, 16}, {8, 32}};
+constexpr RowsVariant kRowsVariants[kNumRowsVariants] = {{16, 0, false, 1}, {8, 0, false, 1},  {8, 4, false, 1},
+                                                         {8, 8, false, 1},  {8, 16, false, 1}, {8, 32, false, 1},
+                                                         {16, 4, true, 1},  {16, 2, true, 1},  {16, 2, true, 2}};
 
 template <int N, int E, int C, int K>
 constexpr bool cols_fits()
 {
     if constexpr (K == 0)
         return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
-    else if constexpr (K == 1)
-        return E == 16 && N / 16 <= 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024 &&
-               C <= N / 2;
+    else if constexpr (K == 1 || K == 3)
+        return E == 16 && N / 16 <= 64 && (K == 3 ? 2 : 1) * 3 * C * (N / 16) <= 1024 &&
+               3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
     else
         return E == 16 && N >= 64 && N / 16 <= 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024 &&
                C <= N / 2 && (C * N) % (2 * C * (N / 16)) == 0;
 }
-template <int N, int E, int B>
+template <int N, int E, int B, bool PAR = false, int W = 1>
 constexpr bool rows_fits()
 {
-    if constexpr (B == 0)
+    if constexpr (PAR) {
+        constexpr int T = N / E, G = par_groups<N, E, B>();
+        return N >= 2 * E && N % B == 0 && (T % 64 == 0 || 64 % T == 0) && (W == 1 || T <= 64) &&
+               W * G * T <= 1024 && G * lds_len(N) * 8 <= 160 * 1024;
+    } else if constexpr (B == 0) {
         return N >= 2 * E && 3 * (N / E) <= 1024;
-    else
+    } else {
         return N >= 2 * E && N % B == 0 && N / E <= 1024;
+    }
 }
 
 constexpr int radix_slot(int E) { return E == 16 ? 2 : E == 8 ? 1 : 0; }
@@ -475,12 +616,13 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
     constexpr int E = kColsVariants[V].E, C = kColsVariants[V].C, K = kColsVariants[V].kind;
     if constexpr (!cols_fits<N, E, C, K>()) {
         return hipErrorInvalidValue;
-    } else if constexpr (K == 1) {
+    } else if constexpr (K == 1 || K == 3) {
+        constexpr int W      = K == 3 ? 2 : 1;
         constexpr int G      = strip_group<N, C>();
-        constexpr int strips = (N / 2) / C + 1;
+        constexpr int strips = lds_strips<N, C>();
         constexpr int groups = (strips + G - 1) / G;
-        hipLaunchKernelGGL((k_cols_lds<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
-                           a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        hipLaunchKernelGGL((k_cols_lds<N, C, W>), dim3(groups * G * a.nframes), dim3(W * 3 * C * (N / 16)), 0, s,
+                           a.h0, a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 2) {
         constexpr int G      = strip_group<N, C>();
@@ -502,9 +644,16 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
 template <int N, int V>
 static hipError_t launch_rows_v(const FrameArgs& a, hipStream_t s)
 {
-    constexpr int E = kRowsVariants[V].E, B = kRowsVariants[V].band;
-    if constexpr (!rows_fits<N, E, B>()) {
+    constexpr int  E = kRowsVariants[V].E, B = kRowsVariants[V].band, W = kRowsVariants[V].W;
+    constexpr bool P = kRowsVariants[V].par;
+    if constexpr (!rows_fits<N, E, B, P, W>()) {
         return hipErrorInvalidValue;
+    } else if constexpr (P) {
+        constexpr int T = N / E;
+        hipLaunchKernelGGL((k_rows_nrm_par<N, E, B, W>), dim3((N / B) * a.nframes),
+                           dim3(W * par_groups<N, E, B>() * T), 0, s, a.z1, a.z2, a.height, a.dx, a.dz, a.normals,
+                           a.jacobian, a.tw[radix_slot(E)], a.fb, a.lambda, a.inv2d);
+        return hipGetLastError();
     } else if constexpr (B == 0) {
         hipLaunchKernelGGL((k_rows<N, E>), dim3(N / 2, a.nframes), dim3(3 * (N / E)), 0, s, a.z1, a.z2, a.height,
                            a.dx, a.dz, a.tw[radix_slot(E)], a.fb);
@@ -543,14 +692,19 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 10: e = launch_cols_v<N, 10>(a, s); break;
         case 11: e = launch_cols_v<N, 11>(a, s); break;
         case 12: e = launch_cols_v<N, 12>(a, s); break;
+        case 13: e = launch_cols_v<N, 13>(a, s); break;
+        case 14: e = launch_cols_v<N, 14>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
     }
     // The fused row variants cover the normal map and Jacobian (nearest taps
-    // only); with bilinear normals the unfused radix-16 row pass is used.
+    // only); with bilinear normals the unfused radix-16 row pass is used.  The
+    // parallel-band variants write no render export: with one requested the
+    // unfused radix-8 pass and the normal-map kernel write it.
     int rv = a.rows_variant;
     if (rv >= 0 && rv < kNumRowsVariants && kRowsVariants[rv].band > 0 && a.normal_mode != 0) rv = 0;
+    if (rv >= 0 && rv < kNumRowsVariants && kRowsVariants[rv].par && (a.disp || a.rnorm)) rv = 1;
     const bool fused = rv >= 0 && rv < kNumRowsVariants && kRowsVariants[rv].band > 0;
     if (stages & 2) {
         switch (rv) {
@@ -560,6 +714,9 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 3: e = launch_rows_v<N, 3>(a, s); break;
         case 4: e = launch_rows_v<N, 4>(a, s); break;
         case 5: e = launch_rows_v<N, 5>(a, s); break;
+        case 6: e = launch_rows_v<N, 6>(a, s); break;
+        case 7: e = launch_rows_v<N, 7>(a, s); break;
+        case 8: e = launch_rows_v<N, 8>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -580,11 +737,12 @@ static bool variant_ok_n(int cols_v, int rows_v)
 #define OCN_FITS(v) cols_fits<N, kColsVariants[v].E, kColsVariants[v].C, kColsVariants[v].kind>()
         OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
         OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9),
-        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12)};
+        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14)};
 #undef OCN_FITS
-#define OCN_RFITS(v) rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band>()
-    constexpr bool r[kNumRowsVariants] = {OCN_RFITS(0), OCN_RFITS(1), OCN_RFITS(2),
-                                          OCN_RFITS(3), OCN_RFITS(4), OCN_RFITS(5)};
+#define OCN_RFITS(v) \
+    rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band, kRowsVariants[v].par, kRowsVariants[v].W>()
+    constexpr bool r[kNumRowsVariants] = {OCN_RFITS(0), OCN_RFITS(1), OCN_RFITS(2), OCN_RFITS(3), OCN_RFITS(4),
+                                          OCN_RFITS(5), OCN_RFITS(6), OCN_RFITS(7), OCN_RFITS(8)};
 #undef OCN_RFITS
     return cols_v >= 0 && cols_v < kNumColsVariants && rows_v >= 0 && rows_v < kNumRowsVariants && c[cols_v] &&
            r[rows_v];
@@ -635,6 +793,8 @@ void cols_variant_shape(int v, int* E, int* C)
 int rows_variant_radix(int v) { return kRowsVariants[v].E; }
 
 int rows_variant_band(int v) { return kRowsVariants[v].band; }
+
+bool rows_variant_parallel(int v) { return kRowsVariants[v].par; }
 
 hipError_t launch_ifft2(int N, const float2* src, float2* tmp, float* out, const float2* const* tw, hipStream_t s)
 {
