@@ -47,7 +47,13 @@ N = 1024
 # normals and the Jacobian.
 CANON_FRAME = 96
 CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20, "k_rows_nrm": 56}
-LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
+# Frames per kernel launch of the sequence (ocn_reserve_frames): 8.  The
+# intermediates of 8 N=1024 frames (96 MB) stay in the 256 MB Infinity Cache
+# between the column and row passes.  Over the 600-frame sequence 8-frame
+# launches (bands of 8 rows) and 32-frame launches (bands of 32 rows) are
+# within 1% of each other (15.0 and 15.1 us per frame); a 20-frame run takes
+# 16.7 us per frame in 8-frame launches against 17.5 in one 32-row launch.
+LAUNCH_FRAMES = 8
 
 # HBM bytes per texel-frame measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (each scaled by what the counter reads for a device-to-device copy of known
@@ -56,15 +62,18 @@ LAUNCH_FRAMES = 32  # frames per kernel launch of the batched path
 # profiles/k_rows_nrm_k_cols_lds_n1024_batched_pmc_traffic.md.  Reported only
 # when the schedule in use matches, otherwise null.
 MEASURED_TRAFFIC = {
-    ("k_rows_nrm", 9, 5): 12.9 + 32.2,
-    ("k_cols_lds", 9, 5): 1.4 + 12.7,
+    ("k_rows_nrm", 9, 3, 8): 15.8 + 32.0,
+    ("k_cols_lds", 9, 3, 8): 2.9 + 12.9,
+    ("k_rows_nrm", 9, 5, 32): 12.9 + 32.0,
+    ("k_cols_lds", 9, 5, 32): 1.4 + 12.6,
 }
 
 
 def kernel_names(sched):
     """Kernel names of a (column, row) schedule as ocn_get_schedule reports it."""
     cv, rv = sched["cols_variant"], sched["rows_variant"]
-    cols = "k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_lds"
+    cols = ("k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_ldsb" if cv >= 15
+            else "k_cols_lds")
     rows = "k_rows" if sched["rows_band"] == 0 else "k_rows_nrm_par" if rv >= 6 else "k_rows_nrm"
     return cols, rows
 
@@ -449,6 +458,7 @@ def run_copy(hip):
 
 
 def main():
+    global N, LAUNCH_FRAMES
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
@@ -459,12 +469,25 @@ def main():
     ap.add_argument("--cascade-frames", type=int, default=60,
                     help="frames of the batched-cascade comparison (0 skips it)")
     ap.add_argument("--fit", type=int, default=1, help="1: time the differentiable fit too (0 skips it)")
+    ap.add_argument("--single", type=int, default=1, help="1: time single-frame updates too (0 skips them)")
+    ap.add_argument("--launch-frames", type=int, default=LAUNCH_FRAMES,
+                    help="frames per kernel launch of the sequence (ocn_reserve_frames), 1..32")
+    ap.add_argument("--size", type=int, default=N,
+                    help="grid size N of the frame sequence (profiling other sizes; the headline is N=1024)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the ocean path runs on a single MI355X); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    if args.size not in (64, 128, 256, 512, 1024, 2048, 4096):
+        print("bench.py: --size must be a power of two in [64, 4096]", file=sys.stderr)
+        sys.exit(2)
+    N = args.size
+    if not 1 <= args.launch_frames <= 32:
+        print("bench.py: --launch-frames must be in [1, 32]", file=sys.stderr)
+        sys.exit(2)
+    LAUNCH_FRAMES = args.launch_frames
     import pyocean
     hip = pyocean.Hip()
     if hip.device_count() < 1:
@@ -490,7 +513,8 @@ def main():
         return
 
     nn = N * N
-    slots = min(FRAMES, max(1, args.steps))
+    # output slices: one per frame up to the 600 of the sequence, within 32 GiB
+    slots = min(FRAMES, max(1, args.steps), max(LAUNCH_FRAMES, (32 << 30) // (28 * nn)))
     sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
     bufs = {k: hip.malloc(slots * nn * b) for k, b in sizes.items()}
     stream = hip.stream()
@@ -557,7 +581,7 @@ def main():
                             "k_rows_nrm" if dominant.startswith("k_rows_nrm") else dominant] * nn * lf
     achieved = dom_bytes / (stage_ms[dominant] * 1e-3) / 1e9
     frame_gbps = CANON_FRAME * nn / (ms * 1e-3 / args.steps) / 1e9
-    tb = MEASURED_TRAFFIC.get((dominant, sched["cols_variant"], sched["rows_variant"]))
+    tb = MEASURED_TRAFFIC.get((dominant, sched["cols_variant"], sched["rows_variant"], lf))
     traffic = int(round(tb * nn * lf)) if tb is not None else None
 
     oc.close()
@@ -565,7 +589,7 @@ def main():
         hip.free(v)
     hip.stream_destroy(stream)
 
-    single = single_frame_bench(hip)
+    single = single_frame_bench(hip) if args.single else None
     casc = cascades_bench(hip, args.cascade_frames, 10) if args.cascade_frames > 0 else None
     fit = fit_bench(hip) if args.fit else None
     bfit = batched_fit_bench(hip) if args.fit else None
@@ -584,8 +608,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (Philox4x32-10 seed 42 + Box-Muller h0, generated on the GPU)",
         "config": {
-            "workload": "plain single ocean N=1024x1024, L=2000 m, wind 25 m/s at 30 deg, A=4, lambda=1.3, "
-                        "l=1 m, frames t=f/60 s; height/Dx/Dz/normals/Jacobian per frame into %d slices" % slots,
+            "workload": "plain single ocean N=%dx%d, L=2000 m, wind 25 m/s at 30 deg, A=4, lambda=1.3, "
+                        "l=1 m, frames t=f/60 s; height/Dx/Dz/normals/Jacobian per frame into %d slices"
+                        % (N, N, slots),
             "N": N, "frames_in_sequence": FRAMES, "output_slices": slots,
         },
         "frame": {

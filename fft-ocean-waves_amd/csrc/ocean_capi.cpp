@@ -131,7 +131,8 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 // 3 (16,8) 4 (8,2) 5 (8,4) 6 (8,8) as (radix, columns per block) of k_cols,
 // 7..9 k_cols_lds with 1, 2, 4 columns per block, 10..12 k_cols_lds2 with 2, 4,
 // 8 columns per block, 13..14 k_cols_lds with 1, 2 columns and twice the
-// threads.  Row variants: 0 radix 16,
+// threads, 15..16 k_cols_ldsb (N >= 2048) with 1, 2 columns.  Row variants:
+// 0 radix 16,
 // 1 radix 8 (each followed by the normal-map kernel), 2..5 the fused
 // row + normals walk with bands of 4, 8, 16, 32 rows, 6..7 the fused
 // parallel bands of 4 and 2 rows, 8 the parallel bands of 2 rows with twice
@@ -142,18 +143,29 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 std::pair<int, int> default_schedule(int N, int nframes)
 {
     const bool batch = nframes > 1;
+    // Multi-frame launches at N = 512 / 1024: the row pass's band height
+    // follows the frames per launch (sweep of every height at 2..32 frames):
+    // a few frames need short (parallel) bands to fill the GPU, 32 frames the
+    // longest walk.  Launches of up to 8 N=1024 frames also keep their
+    // intermediates (96 MB) in the 256 MB Infinity Cache between the passes.
     int cv, rv;
     switch (N) {
     case 64: cv = batch ? 9 : 14; rv = batch ? 1 : 8; break;
     case 128: cv = batch ? 9 : 14; rv = batch ? 2 : 8; break;
     case 256: cv = batch ? 9 : 13; rv = batch ? 1 : 8; break;
-    case 512: cv = batch ? 5 : 14; rv = batch ? 4 : 8; break;
-    case 1024: cv = batch ? 9 : 14; rv = batch ? 5 : 6; break;
-    case 2048: cv = batch ? 2 : 4; rv = batch ? 5 : 2; break;  // single: fused bands of 4 rows (98.3 -> 95.7 us)
-    // 4096: k_cols radix 16 with 2 columns per block and fused bands (32 rows
-    // for frame batches, 16 for single frames): per frame 419 -> 377 us and
-    // 562 -> 444 us against radix-8 columns with k_rows + k_normals
-    default: cv = 1; rv = batch ? 5 : 4; break;
+    case 512: cv = batch ? 5 : 14; rv = !batch ? 8 : nframes < 32 ? 7 : 4; break;
+    case 1024:
+        cv = batch ? 9 : 14;
+        rv = !batch ? 6 : nframes <= 4 ? 7 : nframes <= 20 ? 3 : 5;
+        break;
+    // N=2048: LDS-staged block-synchronised columns for single frames and
+    // launches of a few frames (single 96.6 -> 79.4 us, 2 frames 90.4 -> 83.7
+    // us per frame), k_cols radix 16 with 4 columns for 32
+    case 2048: cv = nframes < 32 ? 16 : 2; rv = !batch ? 2 : nframes <= 2 ? 3 : 5; break;
+    // 4096: k_cols_ldsb (one 4096-point transform over four waves, block
+    // barriers at the stage exchanges): 8-frame launches 394 -> 353 us per
+    // frame, single frames 448 -> 406 us, against k_cols radix 16
+    default: cv = 15; rv = batch ? 5 : 4; break;
     }
     if (!ocn::schedule_valid(N, cv, rv)) {
         cv = 0;

@@ -181,6 +181,27 @@ __global__ __launch_bounds__(W * 3 * C * (N / 16)) void k_cols_lds(const float4*
                                                   tw, sc, fb.t_rev[f], strip, lds);
 }
 
+// k_cols_lds for N >= 2048, whose N/16-thread transform groups span several
+// waves: the same strip (cols_lds_strip) with block barriers between the
+// transform stages.  Block = 3C*T threads.
+template <int N, int C>
+__global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __restrict__ h0,
+                                                                float2* __restrict__ z1, float2* __restrict__ z2,
+                                                                const float2* __restrict__ tw, SpecConst sc,
+                                                                FrameBatch fb)
+{
+    constexpr int G = strip_group<N, C>();
+    static_assert(N / 16 > 64 && (N / 16) % 64 == 0, "transform groups of whole waves");
+    __shared__ float2 lds[3 * C * lds_len(N)];
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int rest  = L / G;
+    const int f     = rest % fb.nframes;
+    const int strip = (rest / fb.nframes) * G + L % G;
+    if (strip >= lds_strips<N, C>()) return;  // padding of the last group (whole block)
+    cols_lds_strip<N, C, BlockSync>(h0, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, tw, sc,
+                                    fb.t_rev[f], strip, lds);
+}
+
 // Two-pass variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
 // LDS, same block order.
 template <int N, int C>
@@ -563,13 +584,15 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // kind 0: k_cols (register-resident, block-synchronised transforms);
 // kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only);
 // kind 2: k_cols_lds2 (the same in two passes through half the LDS);
-// kind 3: k_cols_lds with twice the threads (folds and stores split finer).
+// kind 3: k_cols_lds with twice the threads (folds and stores split finer);
+// kind 4: k_cols_ldsb (LDS-staged, block-synchronised transforms, N >= 2048).
 struct ColsVariant {
     int E, C, kind;
 };
 constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
                                                          {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1},
-                                                         {16, 2, 2}, {16, 4, 2}, {16, 8, 2}, {16, 1, 3}, {16, 2, 3}};
+                                                         {16, 2, 2}, {16, 4, 2}, {16, 8, 2}, {16, 1, 3}, {16, 2, 3},
+                                                         {16, 1, 4}, {16, 2, 4}};
 // Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
 // band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused;
 // par = k_rows_nrm_par (bands of R rows whose rows are transformed at once).
@@ -590,6 +613,8 @@ constexpr bool cols_fits()
     else if constexpr (K == 1 || K == 3)
         return E == 16 && N / 16 <= 64 && (K == 3 ? 2 : 1) * 3 * C * (N / 16) <= 1024 &&
                3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+    else if constexpr (K == 4)
+        return E == 16 && N / 16 > 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024;
     else
         return E == 16 && N >= 64 && N / 16 <= 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024 &&
                C <= N / 2 && (C * N) % (2 * C * (N / 16)) == 0;
@@ -623,6 +648,13 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
         constexpr int groups = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols_lds<N, C, W>), dim3(groups * G * a.nframes), dim3(W * 3 * C * (N / 16)), 0, s,
                            a.h0, a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        return hipGetLastError();
+    } else if constexpr (K == 4) {
+        constexpr int G      = strip_group<N, C>();
+        constexpr int strips = lds_strips<N, C>();
+        constexpr int groups = (strips + G - 1) / G;
+        hipLaunchKernelGGL((k_cols_ldsb<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
+                           a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 2) {
         constexpr int G      = strip_group<N, C>();
@@ -694,6 +726,8 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 12: e = launch_cols_v<N, 12>(a, s); break;
         case 13: e = launch_cols_v<N, 13>(a, s); break;
         case 14: e = launch_cols_v<N, 14>(a, s); break;
+        case 15: e = launch_cols_v<N, 15>(a, s); break;
+        case 16: e = launch_cols_v<N, 16>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -737,7 +771,7 @@ static bool variant_ok_n(int cols_v, int rows_v)
 #define OCN_FITS(v) cols_fits<N, kColsVariants[v].E, kColsVariants[v].C, kColsVariants[v].kind>()
         OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
         OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9),
-        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14)};
+        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14), OCN_FITS(15), OCN_FITS(16)};
 #undef OCN_FITS
 #define OCN_RFITS(v) \
     rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band, kRowsVariants[v].par, kRowsVariants[v].W>()

@@ -287,15 +287,17 @@ def test_updates_captured_in_a_graph_replay_like_eager_launches(hip):
     hip.stream_destroy(stream)
 
 
-def test_c2_sequence_as_benched_matches_oracle(hip):
+@pytest.mark.parametrize("launch", [8, 32], ids=["8_frame_launches", "32_frame_launches"])
+def test_c2_sequence_as_benched_matches_oracle(hip, launch):
     """The headline workload exactly as bench.py launches it: N=1024 with the
-    c2 parameters, 32 frames reserved, frames t = f/60 of the 600-frame
-    sequence through ocn_update_frames in 32-frame launches with the built-in
-    multi-frame schedule (column variant 9: k_cols_lds with 4 columns; row
-    variant 5: the fused row + normal-map + Jacobian walk with bands of 32
-    rows, odd bands walking upwards).  Frames 0..63 in one call (two
-    launches) and frames 568..599 in a second call; frames 0, 31, 32, 33, 63
-    and 599 against the oracle at the c2 tolerance."""
+    c2 parameters, `launch` frames reserved (bench.py reserves 8), frames
+    t = f/60 of the 600-frame sequence through ocn_update_frames with the
+    built-in multi-frame schedule for that launch size: 8 frames run column
+    variant 9 (k_cols_lds with 4 columns) and row variant 3 (the fused row +
+    normal-map + Jacobian walk with bands of 8 rows); 32 frames variant 9 and
+    variant 5 (bands of 32 rows, odd bands walking upwards).  Frames 0..63 in
+    one call and frames 568..599 in a second call; frames 0, 7, 8, 31, 32, 33,
+    63 and 599 against the oracle at the c2 tolerance."""
     lp, op = both_params(**c2_kwargs())
     N, nn = 1024, 1024 * 1024
     sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
@@ -303,22 +305,22 @@ def test_c2_sequence_as_benched_matches_oracle(hip):
     bufs = {k: hip.malloc(F * nn * b) for k, b in sizes.items()}
     fl = pyocean.OcnFields(*[bufs[k] for k in sizes])
     got = {}
-    _say("c2 as benched: N=1024, 32-frame launches")
+    _say("c2 as benched: N=1024, %d-frame launches" % launch)
     try:
         with pyocean.Ocean(lp) as oc:
-            oc.reserve_frames(32)
-            assert oc.frames_reserved() == 32
+            oc.reserve_frames(launch)
+            assert oc.frames_reserved() == launch
             oc.tilde_h0_k()
-            sch = oc.schedule(32)
-            assert (sch["cols_variant"], sch["rows_variant"]) == (9, 5), sch
-            assert sch["rows_band"] == 32 and sch["cols_per_block"] == 4, sch
+            sch = oc.schedule(launch)
+            assert (sch["cols_variant"], sch["rows_variant"]) == ((9, 3) if launch == 8 else (9, 5)), sch
+            assert sch["rows_band"] == (8 if launch == 8 else 32) and sch["cols_per_block"] == 4, sch
             oc.update_frames(np.arange(F, dtype=np.float64) / 60.0, fl)
             hip.sync()
 
             def take(slot):
                 return {k: hip.download(bufs[k] + slot * nn * b, (N, N, 4) if k == "normals" else (N, N))
                         for k, b in sizes.items()}
-            for f in (0, 31, 32, 33, 63):
+            for f in (0, 7, 8, 31, 32, 33, 63):
                 got[f] = take(f)
             oc.update_frames(np.arange(568, 600, dtype=np.float64) / 60.0, fl)
             hip.sync()
@@ -492,8 +494,9 @@ def test_full_size_point_samples_and_parseval(hip, N):
 @pytest.mark.parametrize("N", [2048, 4096])
 def test_full_size_multiframe_launch(hip, N):
     """Two frames in one launch take the multi-frame schedule, at these N the
-    fused row + normal-map + Jacobian pass with bands of 32 rows that reads Z2
-    rows one at a time from their two-row tiles.  Height, Dx and Dz of both
+    fused row + normal-map + Jacobian walk (bands of 8 rows at N=2048, 32 at
+    N=4096) that reads Z2 rows one at a time from their two-row tiles, after
+    the LDS-staged block-synchronised column pass.  Height, Dx and Dz of both
     frames against direct-DFT point samples; normals and Jacobian against the
     oracle's stencils applied to the GPU's own height, Dx and Dz over the
     whole field, which covers every band seam (the halo rows of each band)."""
@@ -511,7 +514,7 @@ def test_full_size_multiframe_launch(hip, N):
             oc.tilde_h0_k()
             oc.reserve_frames(F)
             sch = oc.schedule(F)
-            assert sch["rows_band"] == 32, sch
+            assert sch["rows_band"] == (8 if N == 2048 else 32) and sch["cols_variant"] in (15, 16), sch
             oc.update_frames(times, pyocean.OcnFields(*[bufs[k] for k in names]))
             hip.sync()
             got = [{k: hip.download(bufs[k] + f * nn * sizes[k], (N, N, 4) if k == "normals" else (N, N))
@@ -545,7 +548,7 @@ def test_every_schedule_variant_matches_oracle(hip, N):
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
         tried = 0
-        for cv in range(15):
+        for cv in range(17):
             fused = {}  # first fused row variant's frame per row radix under this column variant
             for rv in range(9):
                 try:
