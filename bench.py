@@ -178,12 +178,14 @@ def cascades_bench(hip, frames, warmup):
         for i in range(warmup):
             b.update(i / 60.0)
         out["batch_eager_ms"] = timed(lambda i: b.update(i / 60.0), frames)
-        # the batch's own autotuner over the row pass's band height per N
+        # the batch's own autotuner over the column strip width and the row
+        # pass's band height per N
         b.autotune(reps=5)
         firsts = {}
         for i, m in enumerate(members):
             firsts.setdefault(m.N, i)
-        out["autotuned_rows_band"] = {str(N): b.schedule(i) for N, i in sorted(firsts.items())}
+        out["autotuned"] = {str(N): {"cols_per_block": b.column_strips(i), "rows_band": b.schedule(i)}
+                            for N, i in sorted(firsts.items())}
         for i in range(warmup):
             b.update_graph(i / 60.0)
         out["batch_graph_autotuned_ms"] = timed(lambda i: b.update_graph(i / 60.0), frames)
@@ -294,7 +296,18 @@ def batched_fit_bench(hip, steps=200, n_oceans=256):
     (17.5 m/s, 0 rad, 3.5); `steps` Adam steps of all oceans, each step one
     column, one row, one adjoint-column and one reduction launch for all
     members.  Reports steps/s and the final parameter errors (the angle modulo
-    pi: the Phillips spectrum depends on (k.w)^2, so w and -w are the same ocean)."""
+    pi: the Phillips spectrum depends on (k.w)^2, so w and -w are the same ocean).
+
+    Convergence is judged against each ocean's noise floor, the loss of its
+    noisy targets at the hidden parameters: a fit whose loss reaches it
+    explains the data as well as the truth does, whatever its parameters.
+    Low-wind oceans have heights of a few cm under 5 cm of noise, and there
+    (V, A) trade off along a flat valley of the loss, so their parameter
+    errors say little; errors are also reported over the oceans whose clean
+    heights have an RMS of at least 2 sigma.  Remedy for the oceans caught in
+    a local minimum: members whose loss exceeds 1.2 x the batch median (all
+    oceans share the noise level) are fitted again for `steps` steps from the
+    start with their angle rotated by 90 degrees, and the lower loss is kept."""
     import pyocean
     times = [0.5, 1.0, 1.5, 2.0]
     hidden, starts = pyocean.fit_batch_params(n_oceans)
@@ -302,36 +315,66 @@ def batched_fit_bench(hip, steps=200, n_oceans=256):
     nn = N * N
     stream = hip.stream()
     rng = np.random.Generator(np.random.Philox(key=777))
-    with pyocean.Fit(starts, times, stream=stream) as f:
-        with pyocean.Batch(hidden, stream=stream) as b:
-            b.tilde_h0_k()
-            for k, t in enumerate(times):
-                b.update(t)
-                hip.stream_sync(stream)
-                for o in range(n_oceans):
-                    hip.ok(hip.rt.hipMemcpy(f.targets_ptr(o) + k * nn * 4, b.fields(o).height, nn * 4, hip.D2D),
-                           "hipMemcpy D2D")
+    tgs = np.empty((n_oceans, len(times), N, N), dtype=np.float32)
+    with pyocean.Batch(hidden, stream=stream) as b:
+        b.tilde_h0_k()
+        for k, t in enumerate(times):
+            b.update(t)
+            hip.stream_sync(stream)
+            for o in range(n_oceans):
+                tgs[o, k] = hip.download(b.fields(o).height, (N, N))
+    signal_rms = np.sqrt((tgs.astype(np.float64) ** 2).mean(axis=(1, 2, 3)))
+    tgs += (0.05 * rng.standard_normal(tgs.shape)).astype(np.float32)
+    cfg = pyocean.adam()
+    e0, e1 = hip.event(), hip.event()
+
+    def run(members, owners):
+        """`steps` Adam steps of a fit of `members` against their owners' targets."""
+        with pyocean.Fit(members, times, stream=stream) as f:
+            for i, o in enumerate(owners):
+                f.upload_targets(i, tgs[o])
+            hip.record(e0, stream)
+            for _ in range(steps):
+                f.step(cfg)
+            hip.record(e1, stream)
+            ms = hip.elapsed_ms(e0, e1)
+            return [f.get(i) for i in range(len(members))], ms
+
+    def with_angle(p, theta):
+        q = pyocean.make_params(N=p.N, L=p.L, wind_speed=p.wind_speed, wind_dir=(math.cos(theta), math.sin(theta)),
+                                amplitude=p.amplitude, seed=p.seed, ocean_id=p.ocean_id)
+        return q
+
+    res, ms = run(starts, range(n_oceans))
+    # the noise floor: the loss of each ocean's targets at its hidden parameters
+    with pyocean.Fit(hidden, times, stream=stream) as f:
         for o in range(n_oceans):
-            tg = hip.download(f.targets_ptr(o), (len(times), N, N))
-            f.upload_targets(o, tg + (0.05 * rng.standard_normal(tg.shape)).astype(np.float32))
-        cfg = pyocean.adam()
-        f.step(cfg)
-        for o in range(n_oceans):
-            f.set_vars(o, starts[o].wind_speed, math.atan2(starts[o].wind_dir_y, starts[o].wind_dir_x),
-                       starts[o].amplitude)
-        e0, e1 = hip.event(), hip.event()
-        hip.record(e0, stream)
-        for _ in range(steps):
-            f.step(cfg)
-        hip.record(e1, stream)
-        ms = hip.elapsed_ms(e0, e1)
-        res = [f.get(o) for o in range(n_oceans)]
-    hip.stream_destroy(stream)
-    dv = np.array([abs(r[0][0] - h.wind_speed) for r, h in zip(res, hidden)])
-    dth = np.array([abs((r[0][1] - math.atan2(h.wind_dir_y, h.wind_dir_x) + math.pi / 2) % math.pi - math.pi / 2)
-                    for r, h in zip(res, hidden)])
-    da = np.array([abs(r[0][2] / h.amplitude - 1.0) for r, h in zip(res, hidden)])
+            f.upload_targets(o, tgs[o])
+        f.forward()
+        floor = np.array([f.get(o)[2] for o in range(n_oceans)])
     loss = np.array([r[2] for r in res])
+    stuck = [o for o in range(n_oceans) if loss[o] > 1.2 * np.median(loss)]
+    final = list(res)
+    ms_restart = 0.0
+    if stuck:
+        rs, ms_restart = run([with_angle(starts[o], res[o][0][1] + math.pi / 2) for o in stuck], stuck)
+        for o, r in zip(stuck, rs):
+            if r[2] < final[o][2]:
+                final[o] = r
+    hip.stream_destroy(stream)
+
+    def errors(rr, idx):
+        dv = np.array([abs(rr[o][0][0] - hidden[o].wind_speed) for o in idx])
+        dth = np.array([abs((rr[o][0][1] - math.atan2(hidden[o].wind_dir_y, hidden[o].wind_dir_x) + math.pi / 2)
+                            % math.pi - math.pi / 2) for o in idx])
+        da = np.array([abs(rr[o][0][2] / hidden[o].amplitude - 1.0) for o in idx])
+        st = lambda a, f=1.0: {"median": round(f * float(np.median(a)), 4), "p90": round(f * float(np.percentile(a, 90)), 4),
+                               "max": round(f * float(a.max()), 4)}
+        return {"wind_speed_abs": st(dv), "angle_deg_mod_180": st(dth, 180.0 / math.pi), "amplitude_rel": st(da)}
+
+    allo = list(range(n_oceans))
+    ident = [o for o in allo if signal_rms[o] >= 0.1]
+    loss_f = np.array([r[2] for r in final])
     return {
         "workload": "%d oceans, N=%d, L=500 m, K=4 (t=0.5..2.0 s), hidden V~U(5,30), angle~U(0,2pi), A~U(2,5), "
                     "target noise N(0, 0.05^2), start (17.5 m/s, 0 rad, 3.5)" % (n_oceans, N),
@@ -339,14 +382,16 @@ def batched_fit_bench(hip, steps=200, n_oceans=256):
         "ms_per_step": round(ms / steps, 4),
         "fit_steps_per_s": round(steps / (ms * 1e-3), 1),
         "ocean_steps_per_s": round(n_oceans * steps / (ms * 1e-3), 1),
-        "final_abs_error_wind_speed": {"median": round(float(np.median(dv)), 4), "p90": round(float(np.percentile(dv, 90)), 4),
-                                       "max": round(float(dv.max()), 4)},
-        "final_abs_error_angle_deg_mod_180": {"median": round(math.degrees(float(np.median(dth))), 4),
-                                              "p90": round(math.degrees(float(np.percentile(dth, 90))), 4),
-                                              "max": round(math.degrees(float(dth.max())), 4)},
-        "final_rel_error_amplitude": {"median": round(float(np.median(da)), 4), "p90": round(float(np.percentile(da, 90)), 4),
-                                      "max": round(float(da.max()), 4)},
-        "final_loss": {"median": float(np.median(loss)), "max": float(loss.max())},
+        "at_noise_floor_after_steps": int((loss <= 1.01 * floor).sum()),
+        "errors_after_steps": errors(res, allo),
+        "restart": {"rule": "loss > 1.2 x batch median: %d more steps from the angle + 90 deg" % steps,
+                    "members": len(stuck), "ms_per_step": round(ms_restart / steps, 4) if stuck else 0.0},
+        "at_noise_floor": int((loss_f <= 1.01 * floor).sum()),
+        "final_errors": errors(final, allo),
+        "identifiable_oceans": len(ident),
+        "final_errors_identifiable": errors(final, ident),
+        "final_loss_over_floor": {"median": round(float(np.median(loss_f / floor)), 4),
+                                  "max": round(float((loss_f / floor).max()), 4)},
     }
 
 

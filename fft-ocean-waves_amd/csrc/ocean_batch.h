@@ -46,6 +46,7 @@ struct BatchMember {
     float4*       rnorm;
     float         dscale;  // displacement_scale
     int           band;    // rows per band of the row pass
+    int           cols;    // columns per strip of the column pass
 };
 
 // Work items: int2 (member, strip) for the column pass, (member, first band)
@@ -54,8 +55,18 @@ __host__ __device__ constexpr int batch_cols_per_block(int N)
 {
     return kBatchColsWidth / N < N / 2 ? kBatchColsWidth / N : N / 2;
 }
+// Column strip widths a member of size N may use: the default C = 4096/N (N/2
+// at N = 64), whose 3C transform groups fill the 768-thread block, or half of
+// it up to N = 1024 (the transforms then take half the block, every thread
+// still folds and stores).  N = 2048 / 4096 transforms synchronise the whole
+// block, so they keep the default.
+inline bool batch_cols_valid(int N, int C)
+{
+    const int C0 = batch_cols_per_block(N);
+    return C == C0 || (N <= 1024 && C0 >= 2 && C == C0 / 2);
+}
 // strips of cols_lds_strip: the self-mirror columns 0 and N/2 share one slot
-inline int batch_cols_items_per_member(int N) { return (N / 2) / batch_cols_per_block(N); }
+inline int batch_cols_items_per_member(int N, int C) { return (N / 2) / C; }
 // groups per row block: one band per block in the wide launch
 inline int batch_rows_groups_per_block(int N)
 {

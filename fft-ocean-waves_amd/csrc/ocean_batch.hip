@@ -5,7 +5,7 @@
 // A batch member is one ocean cascade (its own N, parameters, h0, intermediate
 // and outputs).  A frame is two launches over device tables of work items:
 //   k_batch_cols: item (member, strip)   -> cols_lds_strip<N, C>, 768 threads,
-//                 C = 4096/N columns (N/2 at N = 64, whose strips use 384 threads)
+//                 C = 4096/N columns (N/2 at N = 64) or, per member, half of it
 //   k_batch_rows: item (member, band set) -> rows_nrm_band<N, 8>,        256 threads
 //                 (bands of the member's R rows)
 //                 (k_batch_rows_render: the same with each member's render export;
@@ -52,8 +52,15 @@ constexpr int kRowsLds = rows_lds_need();  // (256 / T) * lds_len(N), T = N/8 (N
 template <int N>
 __device__ __forceinline__ void batch_cols_item(const BatchMember& m, int strip, double t_rev, float2* lds)
 {
+    // m.cols is C or (N <= 1024) C/2 (batch_cols_valid, checked on the host)
     constexpr int C = batch_cols_per_block(N);
     using Sync = typename std::conditional<(N / 16 <= 64), WaveSync, BlockSync>::type;
+    if constexpr (N <= 1024 && C >= 2) {
+        if (m.cols != C) {
+            cols_lds_strip<N, C / 2, Sync, kBatchColsThreads>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
+            return;
+        }
+    }
     cols_lds_strip<N, C, Sync, kBatchColsThreads>(m.h0, m.z1, m.z2, m.tw16, m.sc, t_rev, strip, lds);
 }
 

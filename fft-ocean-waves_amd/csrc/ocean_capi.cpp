@@ -511,6 +511,29 @@ int batch_build_rows(ocn_batch* b)
     return rc;
 }
 
+// Column work items from every member's strip width: (member, strip),
+// member-major (a member's strips share Z lines).  Replaces the device table
+// and drops a built graph; no launch may be reading the old table.
+int batch_build_cols(ocn_batch* b)
+{
+    std::vector<int2> cols;
+    for (size_t i = 0; i < b->members.size(); ++i)
+        for (int k = 0; k < ocn::batch_cols_items_per_member(b->params[i].N, b->members[i].cols); ++k)
+            cols.push_back(make_int2((int)i, k));
+    if (b->d_cols) {
+        (void)hipFree(b->d_cols);
+        b->d_cols = nullptr;
+    }
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        (void)hipGraphDestroy(b->graph);
+        b->exec  = nullptr;
+        b->graph = nullptr;
+    }
+    b->n_cols = (int)cols.size();
+    return upload_vector(&b->d_cols, cols, "batch column items");
+}
+
 int check_batch(const ocn_batch* b)
 {
     if (!b) return fail(OCN_EINVAL, "batch is NULL");
@@ -1086,12 +1109,10 @@ int ocn_batch_create(const ocn_params* params, int count, ocn_batch** out)
             m.inv2d   = (float)((double)p.N / (2.0 * p.L));
             m.dscale  = (float)p.displacement_scale;
             m.band    = ocn::batch_rows_band(p.N);
+            m.cols    = ocn::batch_cols_per_block(p.N);
             b->members.push_back(m);
-            for (int k = 0; k < ocn::batch_cols_items_per_member(p.N); ++k) cols.push_back(make_int2(i, k));
         }
-        b->n_cols = (int)cols.size();
-        if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) &&
-            !(rc = upload_vector(&b->d_cols, cols, "batch column items")))
+        if (!(rc = upload_vector(&b->d_members, b->members, "batch members")) && !(rc = batch_build_cols(b)))
             rc = batch_build_rows(b);
     }
     if (rc) {
@@ -1274,6 +1295,38 @@ int ocn_batch_set_schedule(ocn_batch* b, int N, int rows_band)
     return batch_build_rows(b);
 }
 
+// Columns per strip of the column pass for every member of size N.
+int ocn_batch_set_column_strips(ocn_batch* b, int N, int cols_per_block)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    bool present = false;
+    for (const ocn_params& p : b->params) present |= p.N == N;
+    if (!present) return fail(OCN_EINVAL, "no member has N=%d", N);
+    if (!ocn::batch_cols_valid(N, cols_per_block))
+        return fail(OCN_EINVAL, "cols_per_block %d is not a valid strip width at N=%d", cols_per_block, N);
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();  // no launch may read the tables while they change
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    for (size_t i = 0; i < b->members.size(); ++i)
+        if (b->params[i].N == N) b->members[i].cols = cols_per_block;
+    e = hipMemcpy(b->d_members, b->members.data(), b->members.size() * sizeof(ocn::BatchMember),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(batch members)");
+    return batch_build_cols(b);
+}
+
+int ocn_batch_get_column_strips(const ocn_batch* b, int member, int* cols_per_block)
+{
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (!cols_per_block) return fail(OCN_EINVAL, "cols_per_block is NULL");
+    if (member < 0 || member >= (int)b->members.size())
+        return fail(OCN_EINVAL, "member %d out of range [0, %zu)", member, b->members.size());
+    *cols_per_block = b->members[member].cols;
+    return OCN_OK;
+}
+
 int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band)
 {
     int rc = check_batch(b);
@@ -1285,9 +1338,9 @@ int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band)
     return OCN_OK;
 }
 
-// Per N present, every valid band height timed over `reps` whole frames of
-// the batch (HIP events on `stream`, one warm-up frame first); the fastest
-// is kept, N by N.
+// Per N present, every valid (column strip width, band height) pair timed
+// over `reps` whole frames of the batch (HIP events on `stream`, one warm-up
+// frame first); the fastest pair is kept, N by N.
 int ocn_batch_autotune(ocn_batch* b, int reps, void* stream)
 {
     int rc = batch_ready(b, 0.0);
@@ -1301,11 +1354,13 @@ int ocn_batch_autotune(ocn_batch* b, int reps, void* stream)
     hipError_t  e  = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     for (int N : sizes) {
-        int   best_band = 0;
+        int   best_band = 0, best_cols = 0;
         float best_ms   = 0.0f;
+        const int C0 = ocn::batch_cols_per_block(N);
+        for (int C : {C0, C0 / 2})
         for (int R = 2; R <= 32 && e == hipSuccess && !rc; R *= 2) {
-            if (!ocn::batch_band_valid(N, R)) continue;
-            if ((rc = ocn_batch_set_schedule(b, N, R))) break;
+            if (!ocn::batch_band_valid(N, R) || !ocn::batch_cols_valid(N, C)) continue;
+            if ((rc = ocn_batch_set_schedule(b, N, R)) || (rc = ocn_batch_set_column_strips(b, N, C))) break;
             auto frame = [&](int i) {
                 return ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows, b->d_wide,
                                                b->n_wide, 0.25 * i, b->render, s);
@@ -1319,11 +1374,13 @@ int ocn_batch_autotune(ocn_batch* b, int reps, void* stream)
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
             if (e == hipSuccess && (best_band == 0 || ms < best_ms)) {
                 best_band = R;
+                best_cols = C;
                 best_ms   = ms;
             }
         }
         if (e != hipSuccess || rc) break;
-        if ((rc = ocn_batch_set_schedule(b, N, best_band))) break;
+        if ((rc = ocn_batch_set_schedule(b, N, best_band)) || (rc = ocn_batch_set_column_strips(b, N, best_cols)))
+            break;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

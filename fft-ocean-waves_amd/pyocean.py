@@ -36,6 +36,7 @@ EXPORTS = [
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
     "ocn_batch_get_fields", "ocn_batch_get_h0", "ocn_batch_copy_fields_to_host", "ocn_batch_set_render_export",
     "ocn_batch_set_member_fields", "ocn_batch_set_schedule", "ocn_batch_get_schedule", "ocn_batch_autotune",
+    "ocn_batch_set_column_strips", "ocn_batch_get_column_strips",
     "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
     "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
@@ -140,6 +141,8 @@ def lib():
             "ocn_batch_set_schedule": [vp, i32, i32],
             "ocn_batch_get_schedule": [vp, i32, ctypes.POINTER(i32)],
             "ocn_batch_autotune": [vp, i32, vp],
+            "ocn_batch_set_column_strips": [vp, i32, i32],
+            "ocn_batch_get_column_strips": [vp, i32, ctypes.POINTER(i32)],
             "ocn_fit_upload_targets": [vp, i32, vp, vp],
             "ocn_fit_create": [P, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(vp)],
             "ocn_fit_destroy": [vp],
@@ -607,6 +610,14 @@ class Batch:
 
     def autotune(self, reps=10):
         check(lib().ocn_batch_autotune(self.h, int(reps), self.stream))
+
+    def set_column_strips(self, N, cols_per_block):
+        check(lib().ocn_batch_set_column_strips(self.h, int(N), int(cols_per_block)))
+
+    def column_strips(self, member):
+        v = ctypes.c_int()
+        check(lib().ocn_batch_get_column_strips(self.h, int(member), ctypes.byref(v)))
+        return v.value
 
     def set_member_fields(self, member, fields):
         """Outputs of one member written at caller-provided device addresses."""

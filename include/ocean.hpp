@@ -171,6 +171,13 @@ public:
         check(ocn_batch_get_schedule(b_, member, &r));
         return r;
     }
+    void set_column_strips(int N, int cols_per_block) { check(ocn_batch_set_column_strips(b_, N, cols_per_block)); }
+    int column_strips(int member) const
+    {
+        int c = 0;
+        check(ocn_batch_get_column_strips(b_, member, &c));
+        return c;
+    }
     void autotune(int reps = 10) { check(ocn_batch_autotune(b_, reps, stream_)); }
     // One member's outputs at caller-provided device addresses (e.g. ExternalMemory).
     void set_member_fields(int member, const ocn_fields& f) { check(ocn_batch_set_member_fields(b_, member, &f)); }

@@ -334,10 +334,16 @@ int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
  * for the device to be idle. */
 int ocn_batch_set_schedule(ocn_batch* b, int N, int rows_band);
 int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band);
-/* For each N in the batch, times every valid band height over `reps` whole
- * frames on `stream` (after one warm-up frame) and keeps the fastest.  The
- * outputs afterwards hold a frame of the tuning runs.  Needs the spectra
- * (ocn_batch_tilde_h0_k). */
+/* Columns per block of the column pass for the members of size N: the
+ * default 4096/N (N/2 at N = 64), whose three transforms per column fill a
+ * 768-thread block, or half of it up to N = 1024 (more, smaller blocks).  The
+ * outputs do not depend on it.  Waits for the device to be idle. */
+int ocn_batch_set_column_strips(ocn_batch* b, int N, int cols_per_block);
+int ocn_batch_get_column_strips(const ocn_batch* b, int member, int* cols_per_block);
+/* For each N in the batch, times every valid (column strip width, band
+ * height) pair over `reps` whole frames on `stream` (after one warm-up frame)
+ * and keeps the fastest.  The outputs afterwards hold a frame of the tuning
+ * runs.  Needs the spectra (ocn_batch_tilde_h0_k). */
 int ocn_batch_autotune(ocn_batch* b, int reps, void* stream);
 
 /* Device pointers of one member's outputs (valid until ocn_batch_destroy;

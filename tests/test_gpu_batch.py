@@ -237,10 +237,11 @@ def test_batch_with_n4096_members(hip):
 
 
 def test_batch_band_heights_give_identical_frames_and_autotune(hip):
-    """The band height of the row pass only regroups the rows: every height
-    at every N gives bit-identical frames, invalid heights are refused, and
-    the autotuner leaves a valid height per N with the graph (rebuilt for the
-    new row items) reproducing the frame."""
+    """The band height of the row pass and the strip width of the column pass
+    only regroup rows and columns: every height and width at every N gives
+    bit-identical frames, invalid ones are refused, and the autotuner leaves a
+    valid pair per N with the graph (rebuilt for the new items) reproducing
+    the frame."""
     members = [pyocean.make_params(**kw) for kw in MIXED]
     sizes = sorted({m.N for m in members})
     t = 1.9
@@ -265,13 +266,37 @@ def test_batch_band_heights_give_identical_frames_and_autotune(hip):
         for N, R in ((256, 3), (256, 64), (256, 1), (100, 8)):
             with pytest.raises(pyocean.OceanError):
                 b.set_schedule(N, R)
+        # column strip widths regroup the columns only: C = 4096/N (N/2 at
+        # N = 64) and, up to N = 1024, half of it give bit-identical frames
+        cdef = {m.N: b.column_strips(i) for i, m in enumerate(members)}
+        for N in sizes:
+            C0 = cdef[N]
+            assert C0 == min(4096 // N, N // 2)
+            for C in ((C0, C0 // 2) if N <= 1024 and C0 >= 2 else (C0,)):
+                b.set_column_strips(N, C)
+                b.update(t)
+                _say("  N=%d column strips of %d" % (N, C))
+                for i, m in enumerate(members):
+                    assert b.column_strips(i) == (C if m.N == N else cdef[m.N])
+                    if m.N != N:
+                        continue
+                    got = b.download_fields(i)
+                    for k in FIELDS:
+                        np.testing.assert_array_equal(got[k], ref[i][k], err_msg="N=%d C=%d %s" % (N, C, k))
+            b.set_column_strips(N, C0)
+            for C in (3, 2 * C0, 0):
+                with pytest.raises(pyocean.OceanError):
+                    b.set_column_strips(N, C)
         b.autotune(reps=3)
         tuned = [b.schedule(i) for i in range(len(members))]
-        _say("  autotuned bands %s" % dict(zip([m.N for m in members], tuned)))
+        tuned_cols = [b.column_strips(i) for i in range(len(members))]
+        _say("  autotuned bands %s, column strips %s" % (dict(zip([m.N for m in members], tuned)),
+                                                      dict(zip([m.N for m in members], tuned_cols))))
         b.update_graph(t + 1.0)
         b.update_graph(t)
         after = [b.download_fields(i) for i in range(len(members))]
     for i, m in enumerate(members):
         assert tuned[i] in (2, 4, 8, 16, 32) and m.N % tuned[i] == 0
+        assert tuned_cols[i] in (cdef[m.N], cdef[m.N] // 2)
         for k in FIELDS:
             np.testing.assert_array_equal(after[i][k], ref[i][k], err_msg="member %d %s after autotune" % (i, k))
