@@ -47,13 +47,18 @@ N = 1024
 # normals and the Jacobian.
 CANON_FRAME = 96
 CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20, "k_rows_nrm": 56}
-# Frames per kernel launch of the sequence (ocn_reserve_frames): 8.  The
-# intermediates of 8 N=1024 frames (96 MB) stay in the 256 MB Infinity Cache
-# between the column and row passes.  Over the 600-frame sequence 8-frame
-# launches (bands of 8 rows) and 32-frame launches (bands of 32 rows) are
-# within 1% of each other (15.0 and 15.1 us per frame); a 20-frame run takes
-# 16.7 us per frame in 8-frame launches against 17.5 in one 32-row launch.
-LAUNCH_FRAMES = 8
+# Frames per kernel launch of the sequence (ocn_reserve_frames), by N.  The
+# row pass writes its outputs with non-temporal stores, so the intermediates a
+# launch's column pass wrote stay in the 256 MB Infinity Cache for its row
+# pass as long as they fit there beside h0: 16 N=1024 frames (192 MB) ran at
+# 12.4 us per frame over the 600-frame sequence, against 13.8 in 8-frame and
+# 15.0 in 32-frame launches (384 MB, read back from HBM).  N=2048 and 4096
+# run fastest in 4-frame launches, smaller N in 32-frame ones.
+def launch_frames_for(n):
+    return 16 if n == 1024 else 4 if n >= 2048 else 32
+
+
+LAUNCH_FRAMES = launch_frames_for(1024)
 
 # HBM bytes per texel-frame measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (each scaled by what the counter reads for a device-to-device copy of known
@@ -62,6 +67,8 @@ LAUNCH_FRAMES = 8
 # profiles/k_rows_nrm_k_cols_lds_n1024_batched_pmc_traffic.md.  Reported only
 # when the schedule in use matches, otherwise null.
 MEASURED_TRAFFIC = {
+    ("k_rows_nrm", 9, 4, 16): 13.5 + 32.1,
+    ("k_cols_lds", 9, 4, 16): 1.9 + 12.6,
     ("k_rows_nrm", 9, 3, 8): 15.8 + 32.0,
     ("k_cols_lds", 9, 3, 8): 2.9 + 12.9,
     ("k_rows_nrm", 9, 5, 32): 12.9 + 32.0,
@@ -515,8 +522,9 @@ def main():
                     help="frames of the batched-cascade comparison (0 skips it)")
     ap.add_argument("--fit", type=int, default=1, help="1: time the differentiable fit too (0 skips it)")
     ap.add_argument("--single", type=int, default=1, help="1: time single-frame updates too (0 skips them)")
-    ap.add_argument("--launch-frames", type=int, default=LAUNCH_FRAMES,
-                    help="frames per kernel launch of the sequence (ocn_reserve_frames), 1..32")
+    ap.add_argument("--launch-frames", type=int, default=0,
+                    help="frames per kernel launch of the sequence (ocn_reserve_frames), 1..32; "
+                         "0: by N (16 at N=1024)")
     ap.add_argument("--size", type=int, default=N,
                     help="grid size N of the frame sequence (profiling other sizes; the headline is N=1024)")
     args = ap.parse_args()
@@ -529,10 +537,10 @@ def main():
         print("bench.py: --size must be a power of two in [64, 4096]", file=sys.stderr)
         sys.exit(2)
     N = args.size
-    if not 1 <= args.launch_frames <= 32:
-        print("bench.py: --launch-frames must be in [1, 32]", file=sys.stderr)
+    if not 0 <= args.launch_frames <= 32:
+        print("bench.py: --launch-frames must be in [1, 32] (0: by N)", file=sys.stderr)
         sys.exit(2)
-    LAUNCH_FRAMES = args.launch_frames
+    LAUNCH_FRAMES = args.launch_frames or launch_frames_for(N)
     import pyocean
     hip = pyocean.Hip()
     if hip.device_count() < 1:

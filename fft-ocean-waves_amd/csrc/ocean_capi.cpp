@@ -123,7 +123,10 @@ ocn::SpecConst spec_const(const ocn_params& p)
 std::mutex                                               g_sched_mu;
 std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_sched;
 
-int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn::kMaxFrames; }
+int frames_class(int nframes)
+{
+    return nframes <= 1 ? 1 : nframes <= 4 ? 4 : nframes <= 8 ? 8 : nframes <= 16 ? 16 : ocn::kMaxFrames;
+}
 
 // Built-in schedules: the fastest (column variant, row variant) measured on
 // MI355X per N for one frame per launch and for batches of frames; the
@@ -143,25 +146,30 @@ int frames_class(int nframes) { return nframes <= 1 ? 1 : nframes <= 8 ? 8 : ocn
 std::pair<int, int> default_schedule(int N, int nframes)
 {
     const bool batch = nframes > 1;
-    // Multi-frame launches at N = 512 / 1024: the row pass's band height
-    // follows the frames per launch (sweep of every height at 2..32 frames):
-    // a few frames need short (parallel) bands to fill the GPU, 32 frames the
-    // longest walk.  Launches of up to 8 N=1024 frames also keep their
-    // intermediates (96 MB) in the 256 MB Infinity Cache between the passes.
+    // Multi-frame launches: the row pass's band height follows the frames per
+    // launch (sweep of every pair at 2..32 frames per launch over a 600-frame
+    // sequence).  The walks of one launch fill the GPU best at about four per
+    // CU: a few frames need short (parallel) bands, 16-32 frames the long
+    // walks.  Launches whose intermediates fit the 256 MB Infinity Cache
+    // beside h0 (up to 20 N=1024 frames, 4 N=2048 frames) are the fastest per
+    // frame, since the row pass then reads them from there.
     int cv, rv;
     switch (N) {
     case 64: cv = batch ? 9 : 14; rv = batch ? 1 : 8; break;
     case 128: cv = batch ? 9 : 14; rv = batch ? 2 : 8; break;
-    case 256: cv = batch ? 9 : 13; rv = batch ? 1 : 8; break;
-    case 512: cv = batch ? 5 : 14; rv = !batch ? 8 : nframes < 32 ? 7 : 4; break;
+    case 256: cv = batch ? 9 : 13; rv = 8; break;
+    case 512: cv = !batch ? 14 : (nframes <= 4 || nframes >= 32) ? 9 : 5; rv = batch ? 7 : 8; break;
     case 1024:
         cv = batch ? 9 : 14;
-        rv = !batch ? 6 : nframes <= 4 ? 7 : nframes <= 20 ? 3 : 5;
+        rv = !batch ? 6 : nframes <= 6 ? 7 : nframes <= 9 ? 3 : nframes <= 16 ? 4 : 5;
         break;
     // N=2048: LDS-staged block-synchronised columns for single frames and
     // launches of a few frames (single 96.6 -> 79.4 us, 2 frames 90.4 -> 83.7
     // us per frame), k_cols radix 16 with 4 columns for 32
-    case 2048: cv = nframes < 32 ? 16 : 2; rv = !batch ? 2 : nframes <= 2 ? 3 : 5; break;
+    case 2048:
+        cv = nframes < 32 ? 16 : 2;
+        rv = !batch ? 2 : nframes <= 2 ? 3 : nframes <= 4 ? 4 : 5;
+        break;
     // 4096: k_cols_ldsb (one 4096-point transform over four waves, block
     // barriers at the stage exchanges): 8-frame launches 394 -> 353 us per
     // frame, single frames 448 -> 406 us, against k_cols radix 16
@@ -366,14 +374,16 @@ int run_frames(ocn_handle* h, const double* t, int nframes, const ocn_fields* ou
 
 // Frames t[0..nframes) into consecutive output slices in launches of at
 // most `chunk` frames, intermediates at z1 / z2 (room for `chunk` frames).
+// Each launch takes the schedule for its own frame count, so a short last
+// launch gets the short bands that fill the GPU.
 int frames_in_chunks(ocn_handle* h, float2* z1, float2* z2, const double* t_host, int nframes,
                      const ocn_fields* out, int stage, int chunk, hipStream_t s)
 {
     const size_t nn = (size_t)h->p.N * h->p.N;
     int cv, rv, rc;
-    pick_schedule(h, std::min(nframes, chunk), &cv, &rv);
     for (int f0 = 0; f0 < nframes; f0 += chunk) {
         const int nf = std::min(chunk, nframes - f0);
+        pick_schedule(h, nf, &cv, &rv);
         ocn_fields sl{out->height + f0 * nn,
                       out->dx + f0 * nn,
                       out->dz + f0 * nn,

@@ -15,6 +15,19 @@
 
 namespace ocn {
 
+// ------------------------------------------------------------------ output stores
+
+// The frame's outputs are written once and never read back by these kernels:
+// non-temporal stores keep them from displacing the intermediates (still to be
+// read by other blocks of the row pass) in L2 and the Infinity Cache.
+__device__ __forceinline__ void st_out(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_out(float4* p, float4 v)
+{
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    const f32x4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f32x4*>(p));
+}
+
 // ------------------------------------------------------------------ noise
 
 // Philox4x32-10 (Salmon et al., SC'11).
@@ -642,7 +655,7 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 const float  nx = lf.x - rt.x;
                 const float  nz = lf.y + 2.0f * md.y + rt.y;
                 const float  inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
-                normals[row + x] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+                st_out(&normals[row + x], make_float4(nx * inv, inv, nz * inv, 1.0f));
             }
         }
         if (jac || (RENDER && (disp || rnorm))) {
@@ -672,9 +685,9 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                 const float  dxdz = (sc * v[r].y - xA[r]) * vinv;
                 const float  dzdz = (zC[r] - zA[r]) * vinv;
                 const float  J = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
-                if (jac) jac[row + x] = J;
+                if (jac) st_out(&jac[row + x], J);
                 if (!RENDER) continue;
-                if (disp) disp[row + x] = make_float4(-lambda * xB[r], dscale * hB[r], -lambda * zB[r], J);
+                if (disp) st_out(&disp[row + x], make_float4(-lambda * xB[r], dscale * hB[r], -lambda * zB[r], J));
                 if (rnorm) {
                     // n = dP/dv x dP/du with dP/du = (1 - l dDx/dx, s dh/dx, -l dDz/dx),
                     // dP/dv = (-l dDx/dz, s dh/dz, 1 - l dDz/dz); its y component is J
@@ -682,16 +695,16 @@ __device__ __forceinline__ void rows_nrm_band(const float2* __restrict__ z1, con
                     const float nx = -dscale * (lambda * dhdz * dzdx + (1.0f - lambda * dzdz) * dhdx[r]);
                     const float nz = -dscale * (lambda * dxdz * dhdx[r] + dhdz * (1.0f - lambda * dxdx));
                     const float inv = rsqrtf(fmaf(nx, nx, fmaf(J, J, nz * nz)));
-                    rnorm[row + x] = make_float4(nx * inv, J * inv, nz * inv, 1.0f);
+                    st_out(&rnorm[row + x], make_float4(nx * inv, J * inv, nz * inv, 1.0f));
                 }
             }
         }
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const int x = j + r * T;
-            height[row + x] = hB[r];
-            dx[row + x]     = xB[r];
-            dz[row + x]     = zB[r];
+            st_out(&height[row + x], hB[r]);
+            st_out(&dx[row + x], xB[r]);
+            st_out(&dz[row + x], zB[r]);
             hA[r] = hB[r];
             xA[r] = xB[r];
             zA[r] = zB[r];

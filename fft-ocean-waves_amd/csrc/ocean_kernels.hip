@@ -427,18 +427,18 @@ __global__ __launch_bounds__((W * par_groups<N, E, R>() * (N / E))) void k_rows_
                 const float nz  = (hl[g - 1].x - hl[g + 1].x) + 2.0f * (hm[g - 1].x - hm[g + 1].x) +
                                   (hr[g - 1].x - hr[g + 1].x);
                 const float inv = rsqrtf(fmaf(nx, nx, fmaf(nz, nz, 1.0f)));
-                normals[o] = make_float4(nx * inv, inv, nz * inv, 1.0f);
+                st_out(&normals[o], make_float4(nx * inv, inv, nz * inv, 1.0f));
             }
             if (jac) {
                 const float dxdx = (hr[g].y - hl[g].y) * inv2d;
                 const float dzdx = (zr[g] - zl[g]) * inv2d;
                 const float dxdz = (hm[g + 1].y - hm[g - 1].y) * inv2d;
                 const float dzdz = (zm[g + 1] - zm[g - 1]) * inv2d;
-                jac[o] = (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx;
+                st_out(&jac[o], (1.0f - lambda * dxdx) * (1.0f - lambda * dzdz) - lambda * lambda * dxdz * dzdx);
             }
-            height[o] = hm[g].x;
-            dx[o]     = hm[g].y;
-            dz[o]     = zm[g];
+            st_out(&height[o], hm[g].x);
+            st_out(&dx[o], hm[g].y);
+            st_out(&dz[o], zm[g]);
         }
     }
 }

@@ -287,17 +287,18 @@ def test_updates_captured_in_a_graph_replay_like_eager_launches(hip):
     hip.stream_destroy(stream)
 
 
-@pytest.mark.parametrize("launch", [8, 32], ids=["8_frame_launches", "32_frame_launches"])
+@pytest.mark.parametrize("launch", [16, 8, 32], ids=["16_frame_launches", "8_frame_launches", "32_frame_launches"])
 def test_c2_sequence_as_benched_matches_oracle(hip, launch):
     """The headline workload exactly as bench.py launches it: N=1024 with the
-    c2 parameters, `launch` frames reserved (bench.py reserves 8), frames
+    c2 parameters, `launch` frames reserved (bench.py reserves 16), frames
     t = f/60 of the 600-frame sequence through ocn_update_frames with the
-    built-in multi-frame schedule for that launch size: 8 frames run column
-    variant 9 (k_cols_lds with 4 columns) and row variant 3 (the fused row +
-    normal-map + Jacobian walk with bands of 8 rows); 32 frames variant 9 and
-    variant 5 (bands of 32 rows, odd bands walking upwards).  Frames 0..63 in
-    one call and frames 568..599 in a second call; frames 0, 7, 8, 31, 32, 33,
-    63 and 599 against the oracle at the c2 tolerance."""
+    built-in multi-frame schedule for that launch size.  All run column
+    variant 9 (k_cols_lds with 4 columns) and the fused row + normal-map +
+    Jacobian walk: 16 frames with bands of 16 rows (row variant 4), 8 frames
+    bands of 8 (variant 3), 32 frames bands of 32 (variant 5); odd bands walk
+    upwards.  Frames 0..63 in one call and frames 568..599 in a second call;
+    frames 0, 7, 8, 15, 16, 31, 32, 33, 63 and 599 against the oracle at the
+    c2 tolerance."""
     lp, op = both_params(**c2_kwargs())
     N, nn = 1024, 1024 * 1024
     sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
@@ -312,15 +313,15 @@ def test_c2_sequence_as_benched_matches_oracle(hip, launch):
             assert oc.frames_reserved() == launch
             oc.tilde_h0_k()
             sch = oc.schedule(launch)
-            assert (sch["cols_variant"], sch["rows_variant"]) == ((9, 3) if launch == 8 else (9, 5)), sch
-            assert sch["rows_band"] == (8 if launch == 8 else 32) and sch["cols_per_block"] == 4, sch
+            assert (sch["cols_variant"], sch["rows_variant"]) == (9, {8: 3, 16: 4, 32: 5}[launch]), sch
+            assert sch["rows_band"] == launch and sch["cols_per_block"] == 4, sch
             oc.update_frames(np.arange(F, dtype=np.float64) / 60.0, fl)
             hip.sync()
 
             def take(slot):
                 return {k: hip.download(bufs[k] + slot * nn * b, (N, N, 4) if k == "normals" else (N, N))
                         for k, b in sizes.items()}
-            for f in (0, 7, 8, 31, 32, 33, 63):
+            for f in (0, 7, 8, 15, 16, 31, 32, 33, 63):
                 got[f] = take(f)
             oc.update_frames(np.arange(568, 600, dtype=np.float64) / 60.0, fl)
             hip.sync()
