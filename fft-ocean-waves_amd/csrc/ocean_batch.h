@@ -25,7 +25,7 @@ constexpr int kBatchWideRowsThreads = kBatchWideN / 8;
 // The band height of each member can be changed (ocn_batch_set_schedule,
 // ocn_batch_autotune) among the heights batch_band_valid accepts.
 __host__ __device__ constexpr int batch_rows_radix(int) { return 8; }
-__host__ __device__ constexpr int batch_rows_band(int N) { return N <= 64 ? 2 : N <= 256 ? 8 : N <= 512 ? 16 : 32; }
+__host__ __device__ constexpr int batch_rows_band(int N) { return N <= 64 ? 2 : N <= 512 ? 8 : N <= 1024 ? 16 : 32; }
 
 // Device-side descriptor of one batch member (one ocean cascade, one frame).
 struct BatchMember {
