@@ -326,7 +326,7 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream);
 int ocn_batch_update_graph(ocn_batch* b, double t, void* stream);
 
 /* The batch's schedule: rows per band of the row pass for the members of
- * size N (default 2 at N = 64, 8 up to 256, 16 at 512, 32 above).  A height
+ * size N (default 2 at N = 64, 8 up to 512, 16 at 1024, 32 above).  A height
  * is valid when it is a power of two in [2, 32] dividing N and, at N = 1024
  * and 2048 (row groups wider than a wave), leaves every row block whole
  * bands.  Short bands give more row blocks for batches of few members, long
