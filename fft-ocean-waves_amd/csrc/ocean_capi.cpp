@@ -1507,13 +1507,15 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     int        rc = OCN_OK;
     hipError_t e  = hipGetDevice(&f->device);
     if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
-    // Per member, 256-byte aligned: xi 16, Z and Y 8 P each, targets 4 K, dL/dh0 16 bytes per texel, partials.
+    // Per member, 256-byte aligned: xi 16, Z and Y 8 P each, targets 4 K, dL/dh0 16 bytes per texel, partials,
+    // and with equally spaced times the phase table (16 bytes per texel pair of the half plane).
     auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
+    auto   phase_bytes = [&](int N) { return uniform_times ? al(sizeof(float4) * ocn::fit_phase_entries(N)) : 0; };
     auto   bytes = [&](int N) {
         const size_t nn = (size_t)N * N;
         return al(nn * 16) + 2 * al(nn * 8 * P) + al(nn * 4 * K) + al(nn * 16) +
                al(sizeof(float) * ocn::fit_rows_items_per_member(N, P)) +
-               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N));
+               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N)) + phase_bytes(N);
     };
     size_t total = 0;
     for (const ocn_params& p : f->params) total += bytes(p.N);
@@ -1557,6 +1559,7 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.n_grad_part = ocn::fit_cols_items_per_member(p.N);
             m.loss_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * m.n_loss_part);
             m.grad_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * 3 * m.n_grad_part);
+            m.phase     = uniform_times ? reinterpret_cast<const float4*>(q) : nullptr, q += phase_bytes(p.N);
             m.tw8       = f->tw[p.N][0];
             m.tw16      = f->tw[p.N][1];
             m.vars      = f->d_vars + i;
@@ -1577,8 +1580,11 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             e = hipMemcpy(f->d_vars, vars.data(), sizeof(ocn::FitVars) * vars.size(), hipMemcpyHostToDevice);
             if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(fit variables)");
         }
-        for (int i = 0; !rc && i < count; ++i)
+        for (int i = 0; !rc && i < count; ++i) {
             if ((e = ocn::launch_fit_xi(f->members[i], nullptr)) != hipSuccess) rc = hip_fail(e, "fit noise launch");
+            else if (uniform_times && (e = ocn::launch_fit_phase(f->members[i], nullptr)) != hipSuccess)
+                rc = hip_fail(e, "fit phase launch");
+        }
         if (!rc && (e = hipDeviceSynchronize()) != hipSuccess) rc = hip_fail(e, "hipDeviceSynchronize");
     }
     if (rc) {

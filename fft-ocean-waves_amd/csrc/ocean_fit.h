@@ -47,6 +47,11 @@ struct FitMember {
     int     uniform;        // 1: the times are equally spaced (phases by recurrence)
     int     pad0_;
     const float4* xi;       // unit-amplitude noise, N*N
+    // Equally spaced times: per column item and texel pair of its strip, the
+    // dispersion phase e^{i w t_0} and its step e^{i w dt} as (cos, sin,
+    // cos, sin), written once at creation (launch_fit_phase); entry
+    // (strip * N + ty) * C + c.  Null for other time sets.
+    const float4* phase;
     const float2* tw8;      // Stockham twiddles (N, 8) and (N, 16)
     const float2* tw16;
     float2* z;              // P * N*N: column-transformed spectra (two frames per complex field)
@@ -61,7 +66,9 @@ struct FitMember {
 };
 
 __host__ __device__ constexpr int fit_cols_per_block(int N) { return kFitColsWidth / N; }
-inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
+__host__ __device__ inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
+// float4 entries of a member's phase table
+__host__ __device__ inline size_t fit_phase_entries(int N) { return (size_t)fit_cols_items_per_member(N) * N * fit_cols_per_block(N); }
 inline int fit_rows_groups_per_block(int N) { return kFitRowsThreads / (N / 8); }
 inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_per_block(N) * P; }
 
@@ -69,6 +76,8 @@ inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_
 enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2 };
 
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
+// Fills m.phase (a member with equally spaced times) from its constants.
+hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
 // cols_items: int2 (member, strip); rows_items: int2 (member, local row item)
 hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
                       const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, hipStream_t s);

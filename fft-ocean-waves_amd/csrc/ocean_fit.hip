@@ -93,6 +93,12 @@ struct PhaseWalk {
         sincos_turns(turns(w, dt_rev), s, c);
         step = make_float2(c, s);
     }
+    // from a phase-table entry (cos, sin of w t_0; cos, sin of w dt)
+    __device__ __forceinline__ void load(float4 q)
+    {
+        cur  = make_float2(q.x, q.y);
+        step = make_float2(q.z, q.w);
+    }
     // e^{i phi} of the current frame, then advance to the next
     __device__ __forceinline__ float2 next()
     {
@@ -177,24 +183,30 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
     float4    hp[IT], hq[IT];
     double    w[UNIFORM ? 1 : IT];    // per-frame phases: omega of each texel pair
     PhaseWalk walk[UNIFORM ? IT : 1]; // equally spaced frames: the phase recurrence
+    const float4* ph = UNIFORM ? m.phase + (size_t)strip * (N * C) : nullptr;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int k = threadIdx.x + it * NT;
         const int c = k % C, ty = k / C;
         const int txc = min(strip * C + c, N / 2);
         const int mxc = (N - txc) & (N - 1), my = (N - ty) & (N - 1);
-        // The mirror's amplitude is evaluated on its own: on the Nyquist row or
-        // column its centred index is (-nx, ny) or (nx, -ny), not -n, so the
-        // wind factor d^2 differs from the texel's.
-        const float  s = phillips_amp(sc, txc, ty), sm = phillips_amp(sc, mxc, my);
+        // The amplitude is even in k, and the mirror's centred index is the
+        // texel's negated (bit for bit the same amplitude) except on the Nyquist
+        // row or column (ty = 0 or txc = 0), where it is (nx, -N/2) or
+        // (-N/2, ny): there the wind factor d^2 differs and it is evaluated on
+        // its own.
+        const float  s  = phillips_amp(sc, txc, ty);
+        const float  sm = (txc != 0 && ty != 0) ? s : phillips_amp(sc, mxc, my);
         const float4 a = m.xi[(size_t)ty * N + txc], b = m.xi[(size_t)my * N + mxc];
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
-        const int nx = txc - N / 2, ny = ty - N / 2;
-        float        kinv;
-        const double om = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
-        if constexpr (UNIFORM) walk[it].start(om, m.t_rev[0], m.dt_rev);
-        else w[it] = om;
+        if constexpr (UNIFORM) {
+            walk[it].load(ph[k]);
+        } else {
+            const int nx = txc - N / 2, ny = ty - N / 2;
+            float     kinv;
+            w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
+        }
     }
     for (int p0 = 0; p0 < m.P; p0 += 2) {
         const int np = min(2, m.P - p0);
@@ -274,22 +286,27 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
     const float2* src = m.z + (size_t)p * nn + (size_t)y * N;
 #pragma unroll
     for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
+    const int    k1 = 2 * p, k2 = 2 * p + 1;
+    const bool   two = k2 < m.K;
+    const float* t1 = m.targets + (size_t)k1 * nn + (size_t)y * N;
+    const float* t2 = m.targets + (size_t)(two ? k2 : k1) * nn + (size_t)y * N;
+    float        ta[E], tb[E];  // the targets, loaded before the transform so their reads overlap it
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        ta[r] = t1[j + r * T];
+        tb[r] = t2[j + r * T];
+    }
     fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
 
-    const int   k1 = 2 * p, k2 = 2 * p + 1;
-    const bool  two = k2 < m.K;
     // (-1)^(x+y) / N^2; x = j + r*T with T even
     const float sg = ((j + y) & 1) ? -1.0f / ((float)N * (float)N) : 1.0f / ((float)N * (float)N);
     // adjoint input (-1)^(x+y) r / N^2 with r = 2 (h - target) / (K N^2)
     const float  qs = sg * 2.0f * m.inv_knn;
-    const float* t1 = m.targets + (size_t)k1 * nn + (size_t)y * N;
-    const float* t2 = m.targets + (size_t)(two ? k2 : k1) * nn + (size_t)y * N;
     float lsum = 0.0f;
 #pragma unroll
     for (int r = 0; r < E; ++r) {
-        const int   x  = j + r * T;
-        const float d1 = sg * v[r].x - t1[x];
-        const float d2 = two ? sg * v[r].y - t2[x] : 0.0f;
+        const float d1 = sg * v[r].x - ta[r];
+        const float d2 = two ? sg * v[r].y - tb[r] : 0.0f;
         lsum = fmaf(d1, d1, fmaf(d2, d2, lsum));
         v[r] = make_float2(qs * d1, -(qs * d2));  // conj of q1 + i q2: the forward DFT as conj(IDFT(conj))
     }
@@ -305,30 +322,50 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
 
 // ---------------------------------------------------------------- adjoint columns + parameter gradient
 
-// One texel's share of the parameter gradient: dL/ds * ds/d(V, theta, A)
-// with s = sqrt(P/2) unclamped; nothing where the clamp binds or s = 0.
-// s = c |d| with d = k_hat . w and c free of d, so ds/dtheta = c sgn(d) dd/dtheta
-// needs no division by d (a d recomputed apart from s could round to 0 where s
-// does not).  At |d| < 1e-6 the angle term is dropped, as in the oracle.
-__device__ __forceinline__ void param_terms(const SpecConst& sc, const float4* __restrict__ xi_tex, int N, int tx,
-                                            int ty, float2 ga, float2 gb, float dv_scale, float da_scale, float& dV,
-                                            float& dT, float& dA)
+// The parameter-gradient factors of one texel: dL/d(V, theta, A) gets
+// dL/ds * ds/d(V, theta, A) with s = sqrt(P/2) unclamped, nothing where the
+// clamp binds or s = 0.  s = c |d| with d = k_hat . w and c free of d, so
+// ds/dtheta = c sgn(d) dd/dtheta needs no division by d (a d recomputed apart
+// from s could round to 0 where s does not); at |d| < 1e-6 the angle term is
+// dropped, as in the oracle.  ds/dV = s * dv_scale / |k|^2, ds/dA = s * da_scale.
+// Under k -> -k (a texel and its mirror), |k|, c and s are unchanged and d,
+// dd/dtheta change sign, so every factor here is shared by the pair.
+struct ParamGeo {
+    float s;      // the amplitude
+    float rk2;    // 1 / |k|^2
+    float dterm;  // c sgn(d) dd/dtheta, or 0
+    bool  live;
+};
+
+__device__ __forceinline__ ParamGeo param_geo(const SpecConst& sc, float iLp2, int N, int tx, int ty)
 {
     const float kx = sc.kscale * (float)(tx - N / 2), ky = sc.kscale * (float)(ty - N / 2);
     const float kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
-    const float k2   = kmag * kmag;
-    const float d    = (kx * sc.wx + ky * sc.wy) / kmag;
-    const float dp   = (ky * sc.wx - kx * sc.wy) / kmag;
-    const float c    = sqrtf(sc.A * expf(-1.0f / (k2 * sc.Lp2) - k2 * sc.l2) / (k2 * k2)) * 0.70710678118654752f;
-    const float s    = c * fabsf(d);
-    if (!(s > 0.0f) || s > 4000.0f) return;
-    const float4 xi   = xi_tex[(size_t)ty * N + tx];
-    const float  dLds = ga.x * xi.x + ga.y * xi.y + gb.x * xi.z + gb.y * xi.w;
-    const float  w    = dLds * s;
-    dV = fmaf(w, dv_scale / k2, dV);
-    // c sgn(d) dd/dtheta; |d| < 1e-6 (k perpendicular to the wind, up to rounding)
-    // is the kink of |d|, where the symmetric derivative has no term
-    if (fabsf(d) >= 1e-6f) dT = fmaf(dLds * c, d < 0.0f ? -dp : dp, dT);
+    const float rk   = 1.0f / kmag;
+    const float rk2  = rk * rk;
+    const float d    = (kx * sc.wx + ky * sc.wy) * rk;
+    const float dp   = (ky * sc.wx - kx * sc.wy) * rk;
+    const float c    = sqrtf(sc.A * expf(-rk2 * iLp2 - kmag * kmag * sc.l2) * (rk2 * rk2)) * 0.70710678118654752f;
+    ParamGeo    g;
+    g.s     = c * fabsf(d);
+    g.rk2   = rk2;
+    g.dterm = fabsf(d) >= 1e-6f ? c * (d < 0.0f ? -dp : dp) : 0.0f;
+    g.live  = g.s > 0.0f && g.s <= 4000.0f;
+    return g;
+}
+
+// dL/ds of a texel: its dL/dh0 (ga for h0k, gb for h0mk) against the unit noise.
+__device__ __forceinline__ float dl_ds(float2 ga, float2 gb, float4 xi)
+{
+    return ga.x * xi.x + ga.y * xi.y + gb.x * xi.z + gb.y * xi.w;
+}
+
+__device__ __forceinline__ void param_add(const ParamGeo& g, float dl, float dv_scale, float da_scale, float& dV,
+                                          float& dT, float& dA)
+{
+    const float w = dl * g.s;
+    dV = fmaf(w, dv_scale * g.rk2, dV);
+    dT = fmaf(dl, g.dterm, dT);
     dA = fmaf(w, da_scale, dA);
 }
 
@@ -353,17 +390,20 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     float2    ga[IT], gb[IT], gam[IT], gbm[IT];  // u, then -u
     double    w[UNIFORM ? 1 : IT];
     PhaseWalk walk[UNIFORM ? IT : 1];
+    const float4* ph = UNIFORM ? m.phase + (size_t)strip * (N * C) : nullptr;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         ga[it] = gb[it] = gam[it] = gbm[it] = make_float2(0.0f, 0.0f);
         const int k = threadIdx.x + it * NT;
-        const int c = k % C, ty = k / C;
-        const int tx = min(strip * C + c, N / 2);
-        const int nx = tx - N / 2, ny = ty - N / 2;
-        float        kinv;
-        const double om = omega_fp64(sc, nx * nx + ny * ny, kinv);
-        if constexpr (UNIFORM) walk[it].start(om, m.t_rev[0], m.dt_rev);
-        else w[it] = om;
+        if constexpr (UNIFORM) {
+            walk[it].load(ph[k]);
+        } else {
+            const int c = k % C, ty = k / C;
+            const int tx = min(strip * C + c, N / 2);
+            const int nx = tx - N / 2, ny = ty - N / 2;
+            float     kinv;
+            w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);
+        }
     }
 
     for (int p0 = 0; p0 < m.P; p0 += 2) {
@@ -430,6 +470,7 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     const double V = vars.V, A = vars.A;
     const float  dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
     const float  da_scale = (float)(0.5 / A);
+    const float  iLp2     = 1.0f / sc.Lp2;
     float dV = 0.0f, dT = 0.0f, dA = 0.0f;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -437,12 +478,25 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
         const int c = k % C, ty = k / C;
         const int tx = strip * C + c;
         if (tx > N / 2) continue;  // past the half plane (the last strip)
-        if (store_grad_h0) m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
-        param_terms(sc, m.xi, N, tx, ty, ga[it], gb[it], dv_scale, da_scale, dV, dT, dA);
-        if (tx == 0 || tx == N / 2) continue;  // -u lies in the same column: a u of its own
-        const int mx = N - tx, my = (N - ty) & (N - 1);
-        if (store_grad_h0) m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
-        param_terms(sc, m.xi, N, mx, my, gam[it], gbm[it], dv_scale, da_scale, dV, dT, dA);
+        // -u lies in the same column for tx = 0, N/2 (a u of its own); off the
+        // Nyquist row (ty != 0) its centred index is u's negated and shares u's
+        // factors, on it (my = 0) it has factors of its own
+        const bool self = tx == 0 || tx == N / 2;
+        const int  mx = N - tx, my = (N - ty) & (N - 1);
+        if (store_grad_h0) {
+            m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
+            if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
+        }
+        const bool     paired = !self && ty != 0;
+        const ParamGeo gu     = param_geo(sc, iLp2, N, tx, ty);
+        if (gu.live) {
+            float dl = dl_ds(ga[it], gb[it], m.xi[(size_t)ty * N + tx]);
+            if (paired) dl += dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]);
+            param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
+        }
+        if (self || paired) continue;
+        const ParamGeo gm = param_geo(sc, iLp2, N, mx, my);
+        if (gm.live) param_add(gm, dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]), dv_scale, da_scale, dV, dT, dA);
     }
     dV = block_sum(dV, red);
     dT = block_sum(dT, red);
@@ -555,6 +609,37 @@ __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMembe
     }
     v.V = fmax(v.V, 1e-3);
     v.A = fmax(v.A, 1e-6);
+}
+
+// The phase table of a member with equally spaced times: per column item and
+// texel pair the start and step of the recurrence the column kernels take
+// (the same omega_fp64 and PhaseWalk::start they would evaluate).
+__global__ __launch_bounds__(256) void k_fit_phase(FitMember m)
+{
+    const int    C = fit_cols_per_block(m.N), N = m.N;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)fit_cols_items_per_member(N) * N * C) return;
+    const int strip = (int)(e / ((size_t)N * C)), k = (int)(e % ((size_t)N * C));
+    const int c = k % C, ty = k / C;
+    const int tx = min(strip * C + c, N / 2);
+    const int nx = tx - N / 2, ny = ty - N / 2;
+    SpecConst sc{};
+    sc.kscale = m.kscale;
+    sc.g      = m.g;
+    sc.k2_d   = m.k2_d;
+    sc.g2_d   = m.g2_d;
+    float     kinv;
+    PhaseWalk walk;
+    walk.start(omega_fp64(sc, nx * nx + ny * ny, kinv), m.t_rev[0], m.dt_rev);
+    const_cast<float4*>(m.phase)[e] = make_float4(walk.cur.x, walk.cur.y, walk.step.x, walk.step.y);
+}
+
+hipError_t launch_fit_phase(const FitMember& m, hipStream_t s)
+{
+    if (!m.uniform || !m.phase) return hipErrorInvalidValue;
+    const size_t n = fit_phase_entries(m.N);
+    hipLaunchKernelGGL(k_fit_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m);
+    return hipGetLastError();
 }
 
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s)
