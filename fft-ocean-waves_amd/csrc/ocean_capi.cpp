@@ -608,13 +608,13 @@ ocn::FitVars initial_vars(const ocn_params& p)
     return v;
 }
 
-int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, void* stream)
+int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, const ocn::QnCfg& qn, void* stream)
 {
     if (!f) return fail(OCN_EINVAL, "fit is NULL");
     hipError_t e = hipSetDevice(f->device);
     if (e == hipSuccess)
         e = ocn::launch_fit(f->d_members, (int)f->members.size(), f->d_cols, f->n_cols, f->d_rows, f->n_rows, mode,
-                            adam, (hipStream_t)stream);
+                            adam, qn, (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "fit launch");
 }
 
@@ -1656,9 +1656,15 @@ int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3])
     return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpy(fit variables)");
 }
 
-int ocn_fit_forward(ocn_fit* f, void* stream) { return fit_run(f, ocn::kFitLoss, ocn::AdamCfg{}, stream); }
+int ocn_fit_forward(ocn_fit* f, void* stream)
+{
+    return fit_run(f, ocn::kFitLoss, ocn::AdamCfg{}, ocn::QnCfg{}, stream);
+}
 
-int ocn_fit_backward(ocn_fit* f, void* stream) { return fit_run(f, ocn::kFitGrad, ocn::AdamCfg{}, stream); }
+int ocn_fit_backward(ocn_fit* f, void* stream)
+{
+    return fit_run(f, ocn::kFitGrad, ocn::AdamCfg{}, ocn::QnCfg{}, stream);
+}
 
 int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream)
 {
@@ -1672,7 +1678,22 @@ int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream)
     c.beta1 = adam->beta1;
     c.beta2 = adam->beta2;
     c.eps   = adam->eps;
-    return fit_run(f, ocn::kFitAdam, c, stream);
+    return fit_run(f, ocn::kFitAdam, c, ocn::QnCfg{}, stream);
+}
+
+int ocn_fit_step_qn(ocn_fit* f, const ocn_qn* qn, void* stream)
+{
+    if (!qn) return fail(OCN_EINVAL, "qn is NULL");
+    for (int i = 0; i < 3; ++i)
+        if (!(std::isfinite(qn->scale[i]) && qn->scale[i] > 0.0)) return fail(OCN_EINVAL, "scale[%d] must be > 0", i);
+    if (!(std::isfinite(qn->radius) && qn->radius > 0.0 && std::isfinite(qn->radius_max) &&
+          qn->radius_max >= qn->radius))
+        return fail(OCN_EINVAL, "the quasi-Newton step needs 0 < radius <= radius_max, both finite");
+    ocn::QnCfg c{};
+    for (int i = 0; i < 3; ++i) c.scale[i] = qn->scale[i];
+    c.radius     = qn->radius;
+    c.radius_max = qn->radius_max;
+    return fit_run(f, ocn::kFitQn, ocn::AdamCfg{}, c, stream);
 }
 
 int ocn_fit_get(ocn_fit* f, int member, double vars[3], double grad[3], double* loss, void* stream)

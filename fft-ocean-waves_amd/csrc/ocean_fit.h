@@ -24,12 +24,25 @@ struct FitVars {
     double loss;           // last loss
     double grad[3];        // last dL/d(V, theta, A)
     int    step;           // Adam steps taken
+    int    qn;             // quasi-Newton evaluations taken
+    // quasi-Newton state, in scaled coordinates z_i = x_i / scale_i: the anchor
+    // (the best point accepted so far), its loss and gradient, the inverse
+    // Hessian approximation (zz: xx, xy, xz, yy, yz, zz), the trust radius
+    double qz[3], qg[3], qL, qH[6], qr;
+    int    qk;             // BFGS updates taken
     int    pad_;
 };
 
 struct AdamCfg {
     double lr[3];
     double beta1, beta2, eps;
+};
+
+// Quasi-Newton (BFGS with a trust region) settings: the scales of the
+// variables and the initial and largest trust radius in scaled units.
+struct QnCfg {
+    double scale[3];
+    double radius, radius_max;
 };
 
 // Device-side descriptor of one fit member (one ocean, K frames).
@@ -73,13 +86,14 @@ inline int fit_rows_groups_per_block(int N) { return kFitRowsThreads / (N / 8); 
 inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_per_block(N) * P; }
 
 // Modes of the fit launch sequence.
-enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2 };
+enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2, kFitQn = 3 };
 
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
 // Fills m.phase (a member with equally spaced times) from its constants.
 hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
 // cols_items: int2 (member, strip); rows_items: int2 (member, local row item)
 hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
-                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, hipStream_t s);
+                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, const QnCfg& qn,
+                      hipStream_t s);
 
 }  // namespace ocn

@@ -572,11 +572,96 @@ __global__ __launch_bounds__(kFitColsThreads, 3) void k_fit_cols_bwd(const FitMe
 #undef OCN_CALL
 }
 
+// One quasi-Newton step of one member (a single thread, FP64), after the loss
+// L and gradient g at the current variables x were reduced.  BFGS on the
+// inverse Hessian with a trust region (Nocedal & Wright, Numerical
+// Optimization, 2nd ed., eq. 6.17 and 6.20), in scaled coordinates
+// z = x / scale:
+//   - first evaluation: x becomes the anchor, H = I, radius = cfg.radius;
+//   - L above the anchor's loss (or not finite): the trial is rejected and the
+//     radius shrinks to a quarter of the failed step;
+//   - otherwise the trial is accepted: BFGS update with s = z - z_anchor,
+//     y = g - g_anchor when s.y > 0 (H scaled by s.y / y.y before the first
+//     update), the radius doubles (up to cfg.radius_max) after a step that
+//     reached it, and the trial becomes the anchor;
+// then the next trial is z_anchor + p, p = -H g_anchor cut to the radius.
+// Wind speed and amplitude stay positive (floors 1e-3 and 1e-6).
+__device__ void qn_step(FitVars& v, double L, const double g[3], const QnCfg& cfg)
+{
+    double z[3], gz[3];
+    z[0] = v.V / cfg.scale[0];
+    z[1] = v.theta / cfg.scale[1];
+    z[2] = v.A / cfg.scale[2];
+    for (int i = 0; i < 3; ++i) gz[i] = g[i] * cfg.scale[i];
+    auto H = [&](int i, int j) -> double& {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        return v.qH[a == 0 ? b : a == 1 ? 2 + b : 5];
+    };
+    bool anchor = false;
+    if (v.qn == 0) {
+        for (int i = 0; i < 6; ++i) v.qH[i] = (i == 0 || i == 3 || i == 5) ? 1.0 : 0.0;
+        v.qr   = cfg.radius;
+        v.qk   = 0;
+        anchor = true;
+    } else if (!(L <= v.qL)) {
+        double d2 = 0.0;
+        for (int i = 0; i < 3; ++i) d2 += (z[i] - v.qz[i]) * (z[i] - v.qz[i]);
+        v.qr = 0.25 * sqrt(d2);
+    } else {
+        double s[3], y[3], sy = 0.0, ss = 0.0, yy = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            s[i] = z[i] - v.qz[i];
+            y[i] = gz[i] - v.qg[i];
+            sy += s[i] * y[i];
+            ss += s[i] * s[i];
+            yy += y[i] * y[i];
+        }
+        if (sy > 1e-12 * sqrt(ss * yy)) {
+            if (v.qk == 0)
+                for (int i = 0; i < 6; ++i) v.qH[i] *= sy / yy;
+            // H <- (I - rho s y^T) H (I - rho y s^T) + rho s s^T
+            const double rho = 1.0 / sy;
+            double       Hy[3], yHy = 0.0;
+            for (int i = 0; i < 3; ++i) {
+                Hy[i] = 0.0;
+                for (int j = 0; j < 3; ++j) Hy[i] += H(i, j) * y[j];
+                yHy += y[i] * Hy[i];
+            }
+            double nh[6];
+            for (int i = 0, k = 0; i < 3; ++i)
+                for (int j = i; j < 3; ++j, ++k)
+                    nh[k] = H(i, j) - rho * (s[i] * Hy[j] + Hy[i] * s[j]) + (rho * rho * yHy + rho) * s[i] * s[j];
+            for (int k = 0; k < 6; ++k) v.qH[k] = nh[k];
+            v.qk += 1;
+        }
+        if (sqrt(ss) >= 0.99 * v.qr) v.qr = fmin(2.0 * v.qr, cfg.radius_max);
+        anchor = true;
+    }
+    if (anchor) {
+        for (int i = 0; i < 3; ++i) {
+            v.qz[i] = z[i];
+            v.qg[i] = gz[i];
+        }
+        v.qL = L;
+    }
+    double p[3], pp = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        p[i] = 0.0;
+        for (int j = 0; j < 3; ++j) p[i] -= H(i, j) * v.qg[j];
+        pp += p[i] * p[i];
+    }
+    const double f = pp > v.qr * v.qr ? v.qr / sqrt(pp) : 1.0;
+    v.V     = fmax((v.qz[0] + f * p[0]) * cfg.scale[0], 1e-3);
+    v.theta = (v.qz[1] + f * p[1]) * cfg.scale[1];
+    v.A     = fmax((v.qz[2] + f * p[2]) * cfg.scale[2], 1e-6);
+    v.qn += 1;
+}
+
 // One block per member: loss and gradient from the partial sums (FP64), then
 // optionally one Adam step on (V, theta, A) (Kingma & Ba 2015, bias-corrected).
 // V and A stay positive (floors 1e-3 and 1e-6).
 __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMember* __restrict__ members, int mode,
-                                                                  AdamCfg cfg)
+                                                                  AdamCfg cfg, QnCfg qcfg)
 {
     __shared__ double red[kFitReduceThreads / 64];
     const FitMember&  m = members[blockIdx.x];
@@ -598,6 +683,10 @@ __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMembe
     if (mode == kFitLoss) return;
     const double g[3] = {g0, g1, g2};
     for (int i = 0; i < 3; ++i) v.grad[i] = g[i];
+    if (mode == kFitQn) {
+        qn_step(v, l / ((double)m.K * (double)m.N * (double)m.N), g, qcfg);
+        return;
+    }
     if (mode != kFitAdam) return;
     v.step += 1;
     const double b1t = 1.0 - pow(cfg.beta1, (double)v.step), b2t = 1.0 - pow(cfg.beta2, (double)v.step);
@@ -651,18 +740,19 @@ hipError_t launch_fit_xi(const FitMember& m, hipStream_t s)
 }
 
 hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
-                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, hipStream_t s)
+                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, const QnCfg& qn,
+                      hipStream_t s)
 {
     if (n_members < 1 || n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
     const int adjoint = mode != kFitLoss;
     hipLaunchKernelGGL(k_fit_cols_fwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
     hipLaunchKernelGGL(k_fit_rows, dim3(n_rows), dim3(kFitRowsThreads), 0, s, members, rows_items, adjoint);
-    // an Adam step needs the parameter gradient only; dL/dh0 (16 bytes per
-    // texel) is written by ocn_fit_backward
+    // an optimizer step needs the parameter gradient only; dL/dh0 (16 bytes
+    // per texel) is written by ocn_fit_backward
     if (adjoint)
         hipLaunchKernelGGL(k_fit_cols_bwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
                            mode == kFitGrad ? 1 : 0);
-    hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam);
+    hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam, qn);
     return hipGetLastError();
 }
 

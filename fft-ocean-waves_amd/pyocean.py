@@ -39,7 +39,7 @@ EXPORTS = [
     "ocn_batch_set_column_strips", "ocn_batch_get_column_strips",
     "ocn_fit_upload_targets",
     "ocn_fit_create", "ocn_fit_destroy", "ocn_fit_targets", "ocn_fit_set_vars",
-    "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_get", "ocn_fit_grad_h0",
+    "ocn_fit_forward", "ocn_fit_backward", "ocn_fit_step", "ocn_fit_step_qn", "ocn_fit_get", "ocn_fit_grad_h0",
     "ocn_external_import", "ocn_external_release",
     "ocn_bit_reversed_indices", "ocn_twiddle_factors",
 ]
@@ -75,6 +75,10 @@ class OcnFields(ctypes.Structure):
 class OcnAdam(ctypes.Structure):
     _fields_ = [("lr", ctypes.c_double * 3), ("beta1", ctypes.c_double), ("beta2", ctypes.c_double),
                 ("eps", ctypes.c_double)]
+
+
+class OcnQn(ctypes.Structure):
+    _fields_ = [("scale", ctypes.c_double * 3), ("radius", ctypes.c_double), ("radius_max", ctypes.c_double)]
 
 
 class OceanError(RuntimeError):
@@ -151,6 +155,7 @@ def lib():
             "ocn_fit_forward": [vp, vp],
             "ocn_fit_backward": [vp, vp],
             "ocn_fit_step": [vp, ctypes.POINTER(OcnAdam), vp],
+            "ocn_fit_step_qn": [vp, ctypes.POINTER(OcnQn), vp],
             "ocn_fit_get": [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl), vp],
             "ocn_fit_grad_h0": [vp, i32, ctypes.POINTER(vp)],
             "ocn_external_import": [i32, i32, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(vp),
@@ -645,6 +650,16 @@ def adam(lr=(0.2, 0.02, 0.05), beta1=0.9, beta2=0.999, eps=1e-8):
     return a
 
 
+def qn(scale=(1.0, 0.05, 0.1), radius=1.0, radius_max=20.0):
+    """Quasi-Newton settings: scales of (wind speed m/s, angle rad, amplitude),
+    the units of the trust region, and its initial and largest radius."""
+    q = OcnQn()
+    for i in range(3):
+        q.scale[i] = float(scale[i])
+    q.radius, q.radius_max = float(radius), float(radius_max)
+    return q
+
+
 def fit_batch_params(n_oceans=256, N=256, L=500.0, seed=42, start=(17.5, 0.0, 3.5)):
     """Hidden and starting members of a batched fit of independent oceans.
     Ocean o draws its hidden wind speed ~ U(5, 30) m/s, angle ~ U(0, 2 pi) and
@@ -719,6 +734,9 @@ class Fit:
 
     def step(self, adam_cfg):
         check(lib().ocn_fit_step(self.h, ctypes.byref(adam_cfg), self.stream))
+
+    def step_qn(self, qn_cfg):
+        check(lib().ocn_fit_step_qn(self.h, ctypes.byref(qn_cfg), self.stream))
 
     def get(self, member):
         """(vars (V, theta, A), grad, loss) of one member; synchronises."""

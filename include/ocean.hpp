@@ -213,8 +213,8 @@ private:
 
 // The differentiable fit of (wind speed, wind angle, amplitude) to target
 // heights: a forward operator (loss) and a backward operator (loss and
-// gradients, the spectrum recomputed), plus a device-side Adam step.  No
-// reference counterpart.
+// gradients, the spectrum recomputed), plus device-side Adam and
+// quasi-Newton steps.  No reference counterpart.
 struct FitResult {
     double vars[3];  // wind speed, wind angle (rad), amplitude
     double grad[3];  // dL/d(vars)
@@ -250,6 +250,7 @@ public:
     void forward() { check(ocn_fit_forward(f_, stream_)); }
     void backward() { check(ocn_fit_backward(f_, stream_)); }
     void step(const ocn_adam& a) { check(ocn_fit_step(f_, &a, stream_)); }
+    void step_qn(const ocn_qn& q) { check(ocn_fit_step_qn(f_, &q, stream_)); }
     void set_vars(int member, double V, double theta, double A)
     {
         const double v[3] = {V, theta, A};

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define OCN_ABI_VERSION 3
+#define OCN_ABI_VERSION 4
 
 /* status codes */
 #define OCN_OK        0
@@ -137,6 +137,14 @@ typedef struct ocn_adam {
     double lr[3];
     double beta1, beta2, eps;
 } ocn_adam;
+
+/* Quasi-Newton settings of ocn_fit_step_qn: the scales of (wind speed, wind
+ * angle, amplitude), which are the units of its trust region, and the initial
+ * and largest trust radius in those units. */
+typedef struct ocn_qn {
+    double scale[3];
+    double radius, radius_max;
+} ocn_qn;
 
 /* ---------------------------------------------------------------- lifecycle */
 
@@ -401,7 +409,8 @@ int ocn_fit_targets(const ocn_fit* f, int member, float** targets);
  * (synchronises it). */
 int ocn_fit_upload_targets(ocn_fit* f, int member, const float* targets, void* stream);
 
-/* Sets (V, theta, A) of one member and resets its Adam state (synchronises). */
+/* Sets (V, theta, A) of one member and resets its optimizer state, Adam and
+ * quasi-Newton (synchronises). */
 int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3]);
 
 /* Forward operator: the loss of every member at its current variables. */
@@ -414,6 +423,17 @@ int ocn_fit_backward(ocn_fit* f, void* stream);
  * steps can be queued at once).  It does not write dL/dh0, which only
  * ocn_fit_backward produces. */
 int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream);
+/* The same evaluation followed by one quasi-Newton step per member on the
+ * device: BFGS on the inverse Hessian of the loss in (V, theta, A) divided by
+ * qn->scale, inside a trust region.  Each call evaluates the loss at the
+ * member's current variables (the trial point); a trial that does not lower
+ * the loss below the best point so far (the anchor) is rejected and the next
+ * trial lies a quarter of the way along the same direction; an accepted one
+ * updates the inverse Hessian and becomes the anchor.  The variables after the
+ * call are the next trial point; the anchor and its loss are the best of the
+ * evaluations so far.  Its state starts at the first call after creation or
+ * ocn_fit_set_vars; Adam and quasi-Newton steps keep separate states. */
+int ocn_fit_step_qn(ocn_fit* f, const ocn_qn* qn, void* stream);
 
 /* After the work on `stream` has finished (the call synchronises it): one
  * member's variables, last gradient and last loss (any pointer may be NULL). */

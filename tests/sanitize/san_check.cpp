@@ -204,6 +204,12 @@ static void abi_checks()
     expect_status(ocn_fit_forward(nullptr, nullptr), OCN_EINVAL, "fit_forward(NULL)");
     expect_status(ocn_fit_backward(nullptr, nullptr), OCN_EINVAL, "fit_backward(NULL)");
     expect_status(ocn_fit_step(nullptr, nullptr, nullptr), OCN_EINVAL, "fit_step(NULL)");
+    expect_status(ocn_fit_step_qn(nullptr, nullptr, nullptr), OCN_EINVAL, "fit_step_qn(NULL)");
+    ocn_qn qn{{1.0, 0.05, 0.1}, 2.0, 1.0};
+    expect_status(ocn_fit_step_qn(nullptr, &qn, nullptr), OCN_EINVAL, "fit_step_qn(radius > radius_max)");
+    qn.radius = 1.0;
+    qn.scale[1] = std::numeric_limits<double>::quiet_NaN();
+    expect_status(ocn_fit_step_qn(nullptr, &qn, nullptr), OCN_EINVAL, "fit_step_qn(NaN scale)");
     double vars[3], g3[3], l1;
     expect_status(ocn_fit_get(nullptr, 0, vars, g3, &l1, nullptr), OCN_EINVAL, "fit_get(NULL)");
 

@@ -33,7 +33,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     dyn = subprocess.run(["nm", "-D", "--defined-only", pyocean.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r"\bT (ocn_\w+)", dyn))
     assert set(names) <= exported
-    assert L.ocn_abi_version() == 3
+    assert L.ocn_abi_version() == 4
 
 
 def test_params_default_matches_reference_members():
@@ -171,6 +171,10 @@ def test_fit_null_pointers_rejected():
     assert L.ocn_fit_forward(None, None) == pyocean.OCN_EINVAL
     assert L.ocn_fit_backward(None, None) == pyocean.OCN_EINVAL
     assert L.ocn_fit_step(None, None, None) == pyocean.OCN_EINVAL
+    assert L.ocn_fit_step_qn(None, None, None) == pyocean.OCN_EINVAL
+    for bad in (dict(scale=(1.0, 0.0, 0.1)), dict(radius=-1.0), dict(radius=5.0, radius_max=1.0),
+                dict(radius=float("nan"))):
+        assert L.ocn_fit_step_qn(None, ctypes.byref(pyocean.qn(**bad)), None) == pyocean.OCN_EINVAL, bad
     assert L.ocn_fit_get(None, 0, None, None, None, None) == pyocean.OCN_EINVAL
     assert L.ocn_fit_targets(None, 0, None) == pyocean.OCN_EINVAL
     assert L.ocn_fit_destroy(None) == pyocean.OCN_OK

@@ -2,7 +2,8 @@
 and dL/dh0 from the FP32 adjoint kernels against the FP64 oracle adjoint
 (itself pinned by FP64 central differences in test_oracle_adjoint.py), with
 relative error <= 1e-3 on the gradients; batched members against single fits;
-and the 200-step Adam fit of the N=512 workload."""
+the 200-step Adam fit of the N=512 workload; the quasi-Newton step against a
+host restatement of its update rule."""
 import math
 import os
 import sys
@@ -242,3 +243,101 @@ def test_fit_rejects_bad_arguments(hip):
             f.set_vars(0, -1.0, 0.0, 1.0)
         with pytest.raises(pyocean.OceanError):
             f.step(pyocean.adam(beta1=1.0))
+        with pytest.raises(pyocean.OceanError):
+            f.step_qn(pyocean.qn(radius=0.0))
+
+
+def _qn_replay(x_eval, grads, losses, cfg):
+    """Host restatement of the quasi-Newton step (ocn_fit_step_qn): BFGS on the
+    inverse Hessian with a trust region (Nocedal & Wright, Numerical
+    Optimization, eq. 6.17 and 6.20) in scaled coordinates x / scale, fed with
+    the losses and gradients the device evaluated at each trial point x_eval[k];
+    returns the next trial point after every evaluation."""
+    D = np.array(cfg.scale[:], dtype=np.float64)
+    r0, rmax = cfg.radius, cfg.radius_max
+    out, state = [], None
+    for x, g, L in zip(x_eval, grads, losses):
+        z, gz = np.asarray(x) / D, np.asarray(g) * D
+        if state is None:
+            state = dict(z=z, g=gz, L=L, H=np.eye(3), r=r0, k=0)
+        elif not L <= state["L"]:
+            state["r"] = 0.25 * np.linalg.norm(z - state["z"])
+        else:
+            s, y = z - state["z"], gz - state["g"]
+            sy = float(s @ y)
+            if sy > 1e-12 * np.linalg.norm(s) * np.linalg.norm(y):
+                H = state["H"] * (sy / float(y @ y)) if state["k"] == 0 else state["H"]
+                rho = 1.0 / sy
+                I = np.eye(3)
+                state["H"] = (I - rho * np.outer(s, y)) @ H @ (I - rho * np.outer(y, s)) + rho * np.outer(s, s)
+                state["k"] += 1
+            if np.linalg.norm(s) >= 0.99 * state["r"]:
+                state["r"] = min(2.0 * state["r"], rmax)
+            state.update(z=z, g=gz, L=L)
+        p = -state["H"] @ state["g"]
+        n = np.linalg.norm(p)
+        if n > state["r"]:
+            p *= state["r"] / n
+        nxt = (state["z"] + p) * D
+        nxt[0], nxt[2] = max(nxt[0], 1e-3), max(nxt[2], 1e-6)
+        out.append(nxt)
+    return out
+
+
+def test_fit_quasi_newton_step_follows_its_update_rule(hip):
+    """Two N=128 oceans, 40 quasi-Newton steps from offset starts: after every
+    step the device's next trial point equals the host restatement of the
+    BFGS / trust-region rule fed with the losses and gradients the device
+    reported (relative 1e-9), the run takes accepted and rejected trials and
+    BFGS updates, and the loss ends below its start."""
+    N, L = 128, 250.0
+    times = [0.5, 1.0, 1.5, 2.0]
+    tg = _targets(N, L, times)
+    starts = [(18.0, math.radians(24.0), 2.8), (25.0, math.radians(40.0), 4.5)]
+    cfg = pyocean.qn()
+    with pyocean.Fit([pyocean.make_params(**_kw(N, L, *s)) for s in starts], times) as f:
+        for o in range(2):
+            f.upload_targets(o, tg)
+        trials = [[np.array(s, dtype=np.float64)] for s in starts]
+        grads, losses = [[], []], [[], []]
+        for _ in range(40):
+            f.step_qn(cfg)
+            for o in range(2):
+                x, g, loss = f.get(o)
+                trials[o].append(x)
+                grads[o].append(g)
+                losses[o].append(loss)
+    for o in range(2):
+        want = _qn_replay(trials[o][:-1], grads[o], losses[o], cfg)
+        for k, (w, got) in enumerate(zip(want, trials[o][1:])):
+            np.testing.assert_allclose(got, w, rtol=1e-9, atol=1e-12, err_msg="member %d step %d" % (o, k))
+        L = np.array(losses[o])
+        best = np.minimum.accumulate(L)
+        rejected = int(np.sum(L[1:] > best[:-1]))
+        _say("qn member %d: loss %.6e -> best %.6e, %d rejected trials" % (o, L[0], best[-1], rejected))
+        assert best[-1] < 0.5 * L[0]
+        assert rejected >= 1 and rejected < 30
+
+
+def test_fit_adam_then_quasi_newton_reaches_the_noise_floor(hip):
+    """The N=512 workload of test_fit_adam_recovers_hidden_variables: 60 Adam
+    steps, then 140 quasi-Newton steps, recover the hidden variables at least
+    as closely as 200 Adam steps (wind speed within 0.1 m/s, angle within
+    0.1 deg, amplitude within 1%), at the noise floor."""
+    N, L = 512, 1000.0
+    tg = _targets(N, L, TIMES8)
+    start = (18.0, math.radians(15.0), 2.5)
+    with pyocean.Fit([pyocean.make_params(**_kw(N, L, *start))], TIMES8) as f:
+        f.upload_targets(0, tg)
+        a, q = pyocean.adam(), pyocean.qn()
+        for _ in range(60):
+            f.step(a)
+        for _ in range(140):
+            f.step_qn(q)
+        f.forward()
+        x, g, loss = f.get(0)
+    _say("adam + qn: V %.4f theta %.4f deg A %.4f loss %.6e" % (x[0], math.degrees(x[1]), x[2], loss))
+    assert loss < 0.0026
+    assert abs(x[0] - HIDDEN[0]) < 0.1
+    assert abs(math.degrees(x[1] - HIDDEN[1])) < 0.1
+    assert abs(x[2] / HIDDEN[2] - 1.0) < 0.01
