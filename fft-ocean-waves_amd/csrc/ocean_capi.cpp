@@ -194,6 +194,7 @@ struct ocn_handle {
     int        rows_variant = 0;
     bool       tuned = false;       // schedule set by ocn_autotune / ocn_set_schedule
     float4*    h0 = nullptr;        // N*N
+    float4*    h0t = nullptr;       // N*N, h0 transposed (N >= 2048: read by the column strips)
     float2*    z1 = nullptr;        // zcap*N*N: intermediate; frame 0 doubles as butterfly scratch
     float2*    z2 = nullptr;        // zcap*N*N/2
     float2*    tw[3] = {nullptr, nullptr, nullptr};  // Stockham twiddles, radix 4 / 8 / 16
@@ -210,7 +211,7 @@ namespace {
 
 void free_all(ocn_handle* h)
 {
-    void* ptrs[] = {h->h0, h->z1, h->z2, h->tw[0], h->tw[1], h->tw[2], h->spec[0], h->spec[1], h->spec[2],
+    void* ptrs[] = {h->h0, h->h0t, h->z1, h->z2, h->tw[0], h->tw[1], h->tw[2], h->spec[0], h->spec[1], h->spec[2],
                     h->noise, h->height, h->dx, h->dz, h->jac, h->normals};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
@@ -314,6 +315,7 @@ void frame_args(const ocn_handle* h, float2* z1, float2* z2, const double* t, in
 {
     const size_t nn = (size_t)h->p.N * h->p.N;
     a->h0 = h->h0;
+    a->h0t = h->h0t;
     a->z1 = z1;
     a->z2 = z2;
     for (int i = 0; i < 3; ++i) a->tw[i] = h->tw[i];
@@ -683,7 +685,8 @@ int ocn_create(const ocn_params* p, ocn_handle** out)
     h->cols_variant = sch.first;
     h->rows_variant = sch.second;
     const size_t nn = (size_t)p->N * p->N;
-    if ((rc = dalloc(&h->h0, nn, "h0")) || (rc = reserve_z(h, 1)) || (rc = dalloc(&h->height, nn, "height")) ||
+    if ((rc = dalloc(&h->h0, nn, "h0")) || (p->N >= 2048 && (rc = dalloc(&h->h0t, nn, "h0 transposed"))) ||
+        (rc = reserve_z(h, 1)) || (rc = dalloc(&h->height, nn, "height")) ||
         (rc = dalloc(&h->dx, nn, "dx")) || (rc = dalloc(&h->dz, nn, "dz")) ||
         (rc = dalloc(&h->jac, nn, "jacobian")) || (rc = dalloc(&h->normals, nn, "normals")) ||
         (rc = upload_twiddles(h))) {
@@ -736,7 +739,7 @@ int ocn_tilde_h0_k(ocn_handle* h, void* stream)
 {
     int rc = check_handle(h);
     if (rc || (rc = set_device(h))) return rc;
-    hipError_t e = ocn::launch_h0(h->p.N, h->h0, nullptr, spec_const(h->p), (hipStream_t)stream);
+    hipError_t e = ocn::launch_h0(h->p.N, h->h0, h->h0t, nullptr, spec_const(h->p), (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "tilde_h0_k launch");
     h->h0_ready = true;
     return OCN_OK;
@@ -753,7 +756,7 @@ int ocn_tilde_h0_k_from_noise(ocn_handle* h, const uint8_t* noise_host, void* st
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(noise)");
     e = hipStreamSynchronize((hipStream_t)stream);  // noise_host may be freed by the caller on return
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    e = ocn::launch_h0(h->p.N, h->h0, h->noise, spec_const(h->p), (hipStream_t)stream);
+    e = ocn::launch_h0(h->p.N, h->h0, h->h0t, h->noise, spec_const(h->p), (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "tilde_h0_k launch");
     h->h0_ready = true;
     return OCN_OK;
@@ -1171,7 +1174,7 @@ int ocn_batch_tilde_h0_k(ocn_batch* b, void* stream)
     hipError_t e = hipSetDevice(b->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     for (size_t i = 0; i < b->members.size(); ++i) {
-        e = ocn::launch_h0(b->params[i].N, const_cast<float4*>(b->members[i].h0), nullptr, b->members[i].sc,
+        e = ocn::launch_h0(b->params[i].N, const_cast<float4*>(b->members[i].h0), nullptr, nullptr, b->members[i].sc,
                            (hipStream_t)stream);
         if (e != hipSuccess) return hip_fail(e, "batch tilde_h0_k launch");
     }

@@ -280,11 +280,15 @@ __host__ __device__ constexpr int lds_strips()
 // the real parts and column N/2 in the imaginary parts.  That leaves N/2
 // column slots, a whole number of strips (at N = 1024 and C = 2, exactly one
 // block per CU for a single frame instead of 257 blocks on 256 CUs).
-template <int N, int C, class Sync, int NB = 3 * C * (N / 16)>
+//
+// HT: h0 is given transposed (h0[x * N + y] holds texel (x, y)), so the strip
+// reads its columns as contiguous runs instead of one 16-byte value per row.
+template <int N, int C, class Sync, int NB = 3 * C * (N / 16), bool HT = false>
 __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, float2* __restrict__ z1,
                                                float2* __restrict__ z2, const float2* __restrict__ tw,
                                                const SpecConst& sc, double t_rev, int strip, float2* lds)
 {
+    auto at = [&](int x, int y) -> float4 { return HT ? h0[(size_t)x * N + y] : h0[(size_t)y * N + x]; };
     // NB = threads of the block: all of them fold (phase A) and store (phase
     // C); the 3C transforms take the first NT.  A block wider than NT (a
     // single frame, whose few blocks leave the SIMDs short of waves, or the
@@ -304,8 +308,8 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
             const int mxc = (N - tx) & (N - 1);
             const int my  = (N - ty) & (N - 1);
             float2 ay, ax, az;
-            pair_fold(sc, h0[(size_t)ty * N + tx], h0[(size_t)my * N + mxc], tx - N / 2, ty - N / 2, mxc - N / 2,
-                      my - N / 2, t_rev, ay, ax, az);
+            pair_fold(sc, at(tx, ty), at(mxc, my), tx - N / 2, ty - N / 2, mxc - N / 2, my - N / 2, t_rev, ay, ax,
+                      az);
             lds[(0 * C + c) * LEN + lds_pad(ty)] = ay;
             lds[(1 * C + c) * LEN + lds_pad(ty)] = ax;
             lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
@@ -318,8 +322,7 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
         for (int ty = threadIdx.x; ty < N; ty += NB) {
             const int my = (N - ty) & (N - 1);
             float2 by, bx, bz;
-            pair_fold(sc, h0[(size_t)ty * N + N / 2], h0[(size_t)my * N + N / 2], 0, ty - N / 2, 0, my - N / 2, t_rev,
-                      by, bx, bz);
+            pair_fold(sc, at(N / 2, ty), at(N / 2, my), 0, ty - N / 2, 0, my - N / 2, t_rev, by, bx, bz);
             float2& ay = lds[0 * C * LEN + lds_pad(ty)];
             float2& ax = lds[1 * C * LEN + lds_pad(ty)];
             float2& az = lds[2 * C * LEN + lds_pad(ty)];

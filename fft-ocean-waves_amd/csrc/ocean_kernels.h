@@ -41,6 +41,7 @@ struct FrameBatch {
 
 struct FrameArgs {
     const float4* h0;
+    const float4* h0t;        // h0 transposed ([x][y]) for the N >= 2048 column strips, else null
     float2*       z1;         // nframes x N x N
     float2*       z2;         // nframes x N/2 x N
     const float2* tw[3];      // Stockham twiddles for radix 4, 8, 16
@@ -73,7 +74,8 @@ hipError_t static_scratch_resolve(int device);
 // The address static_scratch_resolve kept for `device`, or null.
 void*      static_scratch_ptr(int device);
 
-hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s);
+// h0 of the spectrum, and when h0t is given its transpose h0t[x * N + y] = h0[y * N + x]
+hipError_t launch_h0(int N, float4* h0, float4* h0t, const uint8_t* noise, const SpecConst& sc, hipStream_t s);
 hipError_t launch_spectrum_t(int N, const float4* h0, float2* hy, float2* hx, float2* hz, const SpecConst& sc,
                              double t_rev, hipStream_t s);
 // stages: bit 0 = spectrum + column transforms, bit 1 = row transforms +

@@ -47,6 +47,19 @@ __global__ __launch_bounds__(256) void k_tilde_h0_k(float4* __restrict__ h0, Spe
     h0[idx] = make_float4(s * xi.x, s * xi.y, s * xi.z, s * xi.w);
 }
 
+// h0t[x * N + y] = h0[y * N + x] (N a multiple of 32): 32 x 32 tiles through
+// LDS, so both the reads and the writes are whole rows of a tile.  The column
+// strips at N >= 2048 read h0 down their columns from this copy.
+__global__ __launch_bounds__(256) void k_transpose_h0(const float4* __restrict__ h0, float4* __restrict__ h0t, int N)
+{
+    __shared__ float4 tile[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int x = threadIdx.x % 32, y = threadIdx.x / 32;  // 8 tile rows per pass
+    for (int r = y; r < 32; r += 8) tile[r][x] = h0[(size_t)(by + r) * N + bx + x];
+    __syncthreads();
+    for (int r = y; r < 32; r += 8) h0t[(size_t)(bx + r) * N + by + x] = tile[x][r];
+}
+
 // Same with the reference's four 256x256 8-bit noise images sampled NEAREST +
 // CLAMP_TO_EDGE at gid/N (tilde_h0_k_cs.glsl:53-58, main.cpp:1102-1116).
 __global__ __launch_bounds__(256) void k_tilde_h0_k_noise(float4* __restrict__ h0, const uint8_t* __restrict__ noise,
@@ -183,9 +196,12 @@ __global__ __launch_bounds__(W * 3 * C * (N / 16)) void k_cols_lds(const float4*
 
 // k_cols_lds for N >= 2048, whose N/16-thread transform groups span several
 // waves: the same strip (cols_lds_strip) with block barriers between the
-// transform stages.  Block = 3C*T threads.
+// transform stages, reading the transposed h0 (h0t), where a strip's columns
+// are contiguous runs instead of one 16-byte value per 32 or 64 KB row
+// (sequences of frames: N=2048 70 -> 64 us, N=4096 363 -> 350 us per frame).
+// Block = 3C*T threads.
 template <int N, int C>
-__global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __restrict__ h0,
+__global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __restrict__ h0t,
                                                                 float2* __restrict__ z1, float2* __restrict__ z2,
                                                                 const float2* __restrict__ tw, SpecConst sc,
                                                                 FrameBatch fb)
@@ -198,8 +214,9 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __
     const int f     = rest % fb.nframes;
     const int strip = (rest / fb.nframes) * G + L % G;
     if (strip >= lds_strips<N, C>()) return;  // padding of the last group (whole block)
-    cols_lds_strip<N, C, BlockSync>(h0, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, tw, sc,
-                                    fb.t_rev[f], strip, lds);
+    cols_lds_strip<N, C, BlockSync, 3 * C * (N / 16), true>(h0t, z1 + (size_t)f * fb.z1_stride,
+                                                            z2 + (size_t)f * fb.z2_stride, tw, sc, fb.t_rev[f], strip,
+                                                            lds);
 }
 
 // Two-pass variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
@@ -653,7 +670,8 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
         constexpr int G      = strip_group<N, C>();
         constexpr int strips = lds_strips<N, C>();
         constexpr int groups = (strips + G - 1) / G;
-        hipLaunchKernelGGL((k_cols_ldsb<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0,
+        if (!a.h0t) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_cols_ldsb<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0t,
                            a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 2) {
@@ -873,11 +891,12 @@ void* static_scratch_ptr(int device)
     return g_scratch_addr[device];
 }
 
-hipError_t launch_h0(int N, float4* h0, const uint8_t* noise, const SpecConst& sc, hipStream_t s)
+hipError_t launch_h0(int N, float4* h0, float4* h0t, const uint8_t* noise, const SpecConst& sc, hipStream_t s)
 {
     dim3 grid((N + 255) / 256, N);
     if (noise) hipLaunchKernelGGL(k_tilde_h0_k_noise, grid, dim3(256), 0, s, h0, noise, sc);
     else hipLaunchKernelGGL(k_tilde_h0_k, grid, dim3(256), 0, s, h0, sc);
+    if (h0t) hipLaunchKernelGGL(k_transpose_h0, dim3(N / 32, N / 32), dim3(256), 0, s, h0, h0t, N);
     return hipGetLastError();
 }
 

@@ -290,7 +290,9 @@ int ocn_get_fields(const ocn_handle* h, ocn_fields* out);
 
 /* Device pointers of the spectrum textures: h0 (float4[N*N]) and the three
  * tilde_h0_t textures (float2[N*N] each; NULL until ocn_tilde_h0_t ran).
- * Any out-pointer may be NULL. */
+ * Any out-pointer may be NULL.  h0 is for reading: at N >= 2048 the frame
+ * reads a transposed copy that ocn_tilde_h0_k and ocn_tilde_h0_k_from_noise
+ * make along with it. */
 int ocn_get_spectrum(const ocn_handle* h, void** h0, void** hy, void** hx, void** hz);
 
 /* Copies the handle's output textures to host memory after the work queued on
