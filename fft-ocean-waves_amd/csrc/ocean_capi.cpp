@@ -22,9 +22,21 @@
 #include "../../include/ocean_c.h"
 #include "ocean_batch.h"
 #include "ocean_fit.h"
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "ocean_kernels.h"
 
 namespace {
+
+// Host-side trace ranges (roctx) around the ABI's setup, frame, batch, fit and
+// autotune calls, shown by rocprofv3 --marker-trace on the host timeline; a
+// range costs a few tens of nanoseconds when no tool is attached.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&)            = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 thread_local std::string g_err;
 
@@ -737,6 +749,7 @@ int ocn_get_params(const ocn_handle* h, ocn_params* p)
 
 int ocn_tilde_h0_k(ocn_handle* h, void* stream)
 {
+    TraceRange trace("ocn_tilde_h0_k");
     int rc = check_handle(h);
     if (rc || (rc = set_device(h))) return rc;
     hipError_t e = ocn::launch_h0(h->p.N, h->h0, h->h0t, nullptr, spec_const(h->p), (hipStream_t)stream);
@@ -819,6 +832,7 @@ int ocn_generate_normal_map(ocn_handle* h, void* stream)
 int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, int stage,
                      void* stream)
 {
+    TraceRange trace("ocn_update_stage");
     int rc = check_frames(h, t_host, nframes, out);
     if (rc) return rc;
     if (stage < OCN_STAGE_COLUMNS || stage > OCN_STAGE_ALL)
@@ -832,6 +846,7 @@ int ocn_update_stage(ocn_handle* h, const double* t_host, int nframes, const ocn
 
 int ocn_update_frames_noalloc(ocn_handle* h, const double* t_host, int nframes, const ocn_fields* out, void* stream)
 {
+    TraceRange trace("ocn_update_frames_noalloc");
     int rc = check_frames(h, t_host, nframes, out);
     if (rc) return rc;
     const size_t nn        = (size_t)h->p.N * h->p.N;
@@ -853,6 +868,7 @@ int ocn_update_frames_noalloc(ocn_handle* h, const double* t_host, int nframes, 
 
 int ocn_update_into(ocn_handle* h, double t, const ocn_fields* out, void* stream)
 {
+    TraceRange trace("ocn_update_into");
     return ocn_update_stage(h, &t, 1, out, OCN_STAGE_ALL, stream);
 }
 
@@ -920,6 +936,7 @@ int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* r
 
 int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
 {
+    TraceRange trace("ocn_autotune");
     int rc = check_handle(h);
     if (rc) return rc;
     if (nframes < 1 || nframes > ocn::kMaxFrames || reps < 1 || reps > 10000)
@@ -973,6 +990,65 @@ int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream)
         g_sched[{h->device, h->p.N, frames_class(nframes), h->p.normal_mode}] = {bc, br};
     }
     h->tuned = false;  // use the tuned table entry for this frame count class
+    return OCN_OK;
+}
+
+int ocn_autotune_save(const char* path)
+{
+    if (!path) return fail(OCN_EINVAL, "path is NULL");
+    std::vector<std::pair<std::tuple<int, int, int, int>, std::pair<int, int>>> rows;
+    {
+        std::lock_guard<std::mutex> lk(g_sched_mu);
+        rows.assign(g_sched.begin(), g_sched.end());
+    }
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return fail(OCN_EINVAL, "cannot open %s for writing", path);
+    std::fprintf(fp, "ocn-autotune %d\n# device N frames_class normal_mode cols_variant rows_variant\n",
+                 OCN_ABI_VERSION);
+    for (const auto& r : rows)
+        std::fprintf(fp, "%d %d %d %d %d %d\n", std::get<0>(r.first), std::get<1>(r.first), std::get<2>(r.first),
+                     std::get<3>(r.first), r.second.first, r.second.second);
+    const bool ok = std::fclose(fp) == 0;
+    return ok ? OCN_OK : fail(OCN_EINVAL, "writing %s failed", path);
+}
+
+int ocn_autotune_load(const char* path)
+{
+    if (!path) return fail(OCN_EINVAL, "path is NULL");
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return fail(OCN_EINVAL, "cannot open %s", path);
+    std::map<std::tuple<int, int, int, int>, std::pair<int, int>> got;
+    char line[256];
+    int  version = -1, lineno = 0, rc = OCN_OK;
+    while (rc == OCN_OK && std::fgets(line, sizeof line, fp)) {
+        ++lineno;
+        if (lineno == 1) {
+            if (std::sscanf(line, "ocn-autotune %d", &version) != 1 || version != OCN_ABI_VERSION)
+                rc = fail(OCN_EINVAL, "%s: not an autotune file of ABI %d", path, OCN_ABI_VERSION);
+            continue;
+        }
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int  dev, N, cls, mode, cv, rv;
+        char extra;
+        if (std::sscanf(line, "%d %d %d %d %d %d %c", &dev, &N, &cls, &mode, &cv, &rv, &extra) != 6) {
+            rc = fail(OCN_EINVAL, "%s:%d: expected six integers", path, lineno);
+            break;
+        }
+        const bool ok = dev >= 0 && N >= 64 && N <= 4096 && (N & (N - 1)) == 0 && frames_class(cls) == cls &&
+                        (mode == OCN_NORMALS_NEAREST || mode == OCN_NORMALS_BILINEAR) &&
+                        ocn::schedule_valid(N, cv, rv) &&
+                        (mode == OCN_NORMALS_NEAREST || ocn::rows_variant_band(rv) == 0);
+        if (!ok) {
+            rc = fail(OCN_EINVAL, "%s:%d: not a valid (device, N, class, normal mode, schedule)", path, lineno);
+            break;
+        }
+        got[{dev, N, cls, mode}] = {cv, rv};
+    }
+    std::fclose(fp);
+    if (rc == OCN_OK && version < 0) rc = fail(OCN_EINVAL, "%s is empty", path);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_sched_mu);
+    for (const auto& kv : got) g_sched[kv.first] = kv.second;
     return OCN_OK;
 }
 
@@ -1184,6 +1260,7 @@ int ocn_batch_tilde_h0_k(ocn_batch* b, void* stream)
 
 int ocn_batch_update(ocn_batch* b, double t, void* stream)
 {
+    TraceRange trace("ocn_batch_update");
     int rc = batch_ready(b, t);
     if (rc) return rc;
     hipError_t e = ocn::launch_batch_frame(b->d_members, b->d_cols, b->n_cols, b->d_rows, b->n_rows, b->d_wide,
@@ -1193,6 +1270,7 @@ int ocn_batch_update(ocn_batch* b, double t, void* stream)
 
 int ocn_batch_update_graph(ocn_batch* b, double t, void* stream)
 {
+    TraceRange trace("ocn_batch_update_graph");
     int rc = batch_ready(b, t);
     if (rc) return rc;
     // The graph holds kernel nodes only: columns, then the row pass (and the
@@ -1356,6 +1434,7 @@ int ocn_batch_get_schedule(const ocn_batch* b, int member, int* rows_band)
 // frame first); the fastest pair is kept, N by N.
 int ocn_batch_autotune(ocn_batch* b, int reps, void* stream)
 {
+    TraceRange trace("ocn_batch_autotune");
     int rc = batch_ready(b, 0.0);
     if (rc) return rc;
     if (reps < 1 || reps > 1000) return fail(OCN_EINVAL, "reps must be in [1, 1000]");
@@ -1661,16 +1740,19 @@ int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3])
 
 int ocn_fit_forward(ocn_fit* f, void* stream)
 {
+    TraceRange trace("ocn_fit_forward");
     return fit_run(f, ocn::kFitLoss, ocn::AdamCfg{}, ocn::QnCfg{}, stream);
 }
 
 int ocn_fit_backward(ocn_fit* f, void* stream)
 {
+    TraceRange trace("ocn_fit_backward");
     return fit_run(f, ocn::kFitGrad, ocn::AdamCfg{}, ocn::QnCfg{}, stream);
 }
 
 int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream)
 {
+    TraceRange trace("ocn_fit_step");
     if (!adam) return fail(OCN_EINVAL, "adam is NULL");
     for (int i = 0; i < 3; ++i)
         if (!(std::isfinite(adam->lr[i]) && adam->lr[i] >= 0.0)) return fail(OCN_EINVAL, "lr[%d] must be >= 0", i);
@@ -1686,6 +1768,7 @@ int ocn_fit_step(ocn_fit* f, const ocn_adam* adam, void* stream)
 
 int ocn_fit_step_qn(ocn_fit* f, const ocn_qn* qn, void* stream)
 {
+    TraceRange trace("ocn_fit_step_qn");
     if (!qn) return fail(OCN_EINVAL, "qn is NULL");
     for (int i = 0; i < 3; ++i)
         if (!(std::isfinite(qn->scale[i]) && qn->scale[i] > 0.0)) return fail(OCN_EINVAL, "scale[%d] must be > 0", i);

@@ -30,7 +30,7 @@ EXPORTS = [
     "ocn_tilde_h0_t", "ocn_butterfly_fft", "ocn_butterfly_fft_buffers", "ocn_generate_normal_map",
     "ocn_update", "ocn_update_into", "ocn_update_stage", "ocn_update_frames",
     "ocn_reserve_frames", "ocn_frames_reserved", "ocn_update_frames_noalloc",
-    "ocn_autotune", "ocn_set_schedule", "ocn_get_schedule",
+    "ocn_autotune", "ocn_autotune_save", "ocn_autotune_load", "ocn_set_schedule", "ocn_get_schedule",
     "ocn_get_fields", "ocn_get_spectrum", "ocn_copy_fields_to_host",
     "ocn_batch_create", "ocn_batch_destroy", "ocn_batch_size", "ocn_batch_get_params",
     "ocn_batch_tilde_h0_k", "ocn_batch_update", "ocn_batch_update_graph",
@@ -123,6 +123,8 @@ def lib():
             "ocn_update_frames_noalloc": [vp, ctypes.POINTER(ctypes.c_double), i32, F, vp],
             "ocn_frames_reserved": [vp, ctypes.POINTER(i32)],
             "ocn_autotune": [vp, i32, i32, vp],
+            "ocn_autotune_save": [ctypes.c_char_p],
+            "ocn_autotune_load": [ctypes.c_char_p],
             "ocn_set_schedule": [vp, i32, i32],
             "ocn_get_schedule": [vp, i32] + [ctypes.POINTER(i32)] * 6,
             "ocn_get_fields": [vp, F],
@@ -636,6 +638,16 @@ class Batch:
             raise OceanError("OCN_ESTATE", "the render export is off")
         return {"displacement": self.hip.download(f.displacement, (N, N, 4)),
                 "render_normals": self.hip.download(f.render_normals, (N, N, 4))}
+
+
+def autotune_save(path):
+    """Writes the autotuner's table (every ocn_autotune result of this process)."""
+    check(lib().ocn_autotune_save(os.fsencode(path)))
+
+
+def autotune_load(path):
+    """Records the entries of a file autotune_save wrote."""
+    check(lib().ocn_autotune_load(os.fsencode(path)))
 
 
 # ---------------------------------------------------------------- differentiable fits

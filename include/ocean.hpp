@@ -42,6 +42,10 @@ inline void check(int status)
 }
 
 // Parameter block with the reference's defaults (main.cpp:1632-1646).
+// The autotuner's table as a file (ocn_autotune_save / ocn_autotune_load).
+inline void autotune_save(const std::string& path) { check(ocn_autotune_save(path.c_str())); }
+inline void autotune_load(const std::string& path) { check(ocn_autotune_load(path.c_str())); }
+
 inline ocn_params default_params()
 {
     ocn_params p;

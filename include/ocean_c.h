@@ -279,6 +279,18 @@ int ocn_frames_reserved(const ocn_handle* h, int* nframes);
  * rows_band > 0 means the normal map and Jacobian are fused into the row pass,
  * which walks bands of that many rows. */
 int ocn_autotune(ocn_handle* h, int nframes, int reps, void* stream);
+/* The autotuner's table as a text file, so a tuned schedule outlives the
+ * process: ocn_autotune_save writes every entry ocn_autotune recorded in this
+ * process (a version line, then "device N frames_class normal_mode
+ * cols_variant rows_variant" per line; frames_class is 1, 4, 8, 16 or 32, the
+ * launch sizes up to it sharing one entry); ocn_autotune_load reads such a
+ * file and records its entries as if ocn_autotune had found them.  The file
+ * names device ordinals and this library's variant numbers: a file of another
+ * ABI version, or with a line that is not a valid entry, is rejected whole.
+ * Host-only (no device work).  The batch's tuned schedules are not part of it
+ * (ocn_batch_autotune keeps them on the batch). */
+int ocn_autotune_save(const char* path);
+int ocn_autotune_load(const char* path);
 int ocn_set_schedule(ocn_handle* h, int cols_variant, int rows_variant);
 int ocn_get_schedule(const ocn_handle* h, int nframes, int* cols_variant, int* rows_variant, int* cols_radix,
                      int* cols_per_block, int* rows_radix, int* rows_band);

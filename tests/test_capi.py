@@ -255,3 +255,37 @@ def test_cpp_api_rejects_bad_params():
     r = subprocess.run([CPP_EXAMPLE, "--validate"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok ") == 4
+
+
+def test_autotune_file_round_trip_and_rejects_bad_files(tmp_path):
+    """ocn_autotune_load records valid entries (host only) and ocn_autotune_save
+    writes them back; files of another version, with an invalid N, class,
+    normal mode or schedule pair, or with junk, are rejected whole."""
+    L = pyocean.lib()
+    good = tmp_path / "tune.txt"
+    entries = ["0 256 1 0 13 8", "0 1024 16 0 9 4", "1 512 32 1 5 1"]
+    good.write_text("ocn-autotune %d\n# comment\n%s\n" % (L.ocn_abi_version(), "\n".join(entries)))
+    pyocean.autotune_load(good)
+    out = tmp_path / "out.txt"
+    pyocean.autotune_save(out)
+    lines = out.read_text().splitlines()
+    assert lines[0] == "ocn-autotune %d" % L.ocn_abi_version()
+    saved = {l for l in lines[1:] if not l.startswith("#")}
+    assert set(entries) <= saved
+    bad = {
+        "version": "ocn-autotune 1\n0 256 1 0 13 8\n",
+        "N": "ocn-autotune %d\n0 300 1 0 13 8\n",
+        "class": "ocn-autotune %d\n0 256 3 0 13 8\n",
+        "mode": "ocn-autotune %d\n0 256 1 7 13 8\n",
+        "pair": "ocn-autotune %d\n0 4096 1 0 14 8\n",
+        "bilinear_fused": "ocn-autotune %d\n0 256 1 1 9 3\n",
+        "junk": "ocn-autotune %d\n0 256 1 0 13 8 9\n",
+        "empty": "",
+    }
+    for name, text in bad.items():
+        f = tmp_path / ("bad_%s.txt" % name)
+        f.write_text(text % L.ocn_abi_version() if "%d" in text else text)
+        assert L.ocn_autotune_load(os.fsencode(f)) == pyocean.OCN_EINVAL, name
+    assert L.ocn_autotune_load(os.fsencode(tmp_path / "missing.txt")) == pyocean.OCN_EINVAL
+    assert L.ocn_autotune_load(None) == pyocean.OCN_EINVAL
+    assert L.ocn_autotune_save(None) == pyocean.OCN_EINVAL

@@ -603,7 +603,10 @@ def test_multiframe_launch_matches_oracle_per_frame(hip):
             assert_field_close("%s[f=%d]" % (fld, f), got[fld][f], ref[fld])
 
 
-def test_autotune_picks_a_valid_schedule_and_stays_correct(hip):
+def test_autotune_picks_a_valid_schedule_and_stays_correct(hip, tmp_path):
+    """The autotuner's pick at N=256 gives the oracle's frame and is written
+    by ocn_autotune_save; an entry read by ocn_autotune_load is the schedule a
+    new handle takes for that N and launch size."""
     lp, op = both_params(**c1_kwargs())
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
@@ -611,8 +614,23 @@ def test_autotune_picks_a_valid_schedule_and_stays_correct(hip):
         sch = oc.schedule(1)
         _say("autotuned N=256: %s" % sch)
         assert sch["cols_radix"] in (8, 16) and sch["rows_radix"] in (8, 16)
+        f = tmp_path / "tune.txt"
+        pyocean.autotune_save(f)
+        rows = [l.split() for l in f.read_text().splitlines()[1:] if not l.startswith("#")]
+        assert any(r[1:] == ["256", "1", "0", str(sch["cols_variant"]), str(sch["rows_variant"])] for r in rows), rows
         oc.update(1.0)
         got = oc.download_fields()
+    v = pyocean.lib().ocn_abi_version()
+    pin = tmp_path / "pin.txt"
+    pin.write_text("ocn-autotune %d\n%d 128 4 0 7 1\n" % (v, 0))
+    pyocean.autotune_load(pin)
+    try:
+        with pyocean.Ocean(pyocean.make_params(N=128)) as o2:
+            s2 = o2.schedule(4)
+            assert (s2["cols_variant"], s2["rows_variant"]) == (7, 1), s2
+    finally:  # back to the built-in choice for that key
+        pin.write_text("ocn-autotune %d\n%d 128 4 0 9 2\n" % (v, 0))
+        pyocean.autotune_load(pin)
     ref = oracle.frame(op, oracle.h0(op), 1.0, nthreads=NTHREADS)
     for fld in ("height", "dx", "dz", "normals", "jacobian"):
         assert_field_close(fld, got[fld], ref[fld])
