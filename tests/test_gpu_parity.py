@@ -337,6 +337,40 @@ def test_c2_sequence_as_benched_matches_oracle(hip, launch):
         _say("  frame %d: jacobian rel-L2 %.2e" % (f, r))
 
 
+def test_transposed_h0_follows_every_spectrum_change_at_2048(hip):
+    """At N >= 2048 the fused column pass reads a transposed copy of h0 that
+    the spectrum calls make along with it.  After ocn_tilde_h0_k, after
+    ocn_set_params + ocn_tilde_h0_k and after ocn_tilde_h0_k_from_noise, the
+    fused frame equals the reference's pass structure (tilde_h0_t, three
+    butterfly_fft, generate_normal_map), which reads h0 itself."""
+    kw1 = dict(N=2048, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(0.5), math.sin(0.5)), amplitude=4.0,
+               choppiness=1.3)
+    kw2 = dict(kw1, wind_speed=31.0, wind_dir=(0.3, -1.0), amplitude=2.5)
+    _say("transposed h0 at N=2048")
+
+    def both(oc, t):
+        oc.update(t)
+        fused = oc.download_fields()
+        oc.tilde_h0_t(t)
+        for f in (pyocean.OCN_FIELD_DY, pyocean.OCN_FIELD_DX, pyocean.OCN_FIELD_DZ):
+            oc.butterfly_fft(f)
+        oc.generate_normal_map()
+        return fused, oc.download_fields()
+
+    with pyocean.Ocean(pyocean.make_params(**kw1)) as oc:
+        oc.tilde_h0_k()
+        runs = [("philox", both(oc, 3.25))]
+        oc.set_params(pyocean.make_params(**kw2))
+        oc.tilde_h0_k()
+        runs.append(("set_params", both(oc, 3.25)))
+        oc.tilde_h0_k_from_noise(_noise())
+        runs.append(("reference noise", both(oc, 1.5)))
+    for name, (fused, passes) in runs:
+        for fld in ("height", "dx", "dz", "normals", "jacobian"):
+            assert_field_close("%s %s" % (name, fld), fused[fld], passes[fld])
+    assert rel_l2(runs[1][1][0]["height"], runs[0][1][0]["height"]) > 0.1
+
+
 def test_set_params_then_tilde_h0_k_reinitialises_the_ocean(hip):
     """Live re-initialisation (the reference's GUI edits wind speed and
     amplitude, main.cpp:350-353, but never regenerates the spectrum):
