@@ -587,7 +587,7 @@ def main():
     bufs = {k: hip.malloc(slots * nn * b) for k, b in sizes.items()}
     stream = hip.stream()
     oc = pyocean.Ocean(workload_params(), stream=stream)
-    # Setup, before any timing: the intermediates for 32-frame launches.  The
+    # Setup, before any timing: the intermediates for the launch size.  The
     # frame calls themselves never allocate, so warm-up and timed calls of any
     # length run the same kernels on the same buffers.
     oc.reserve_frames(min(LAUNCH_FRAMES, slots))
