@@ -173,7 +173,7 @@ std::pair<int, int> default_schedule(int N, int nframes)
     case 512: cv = !batch ? 14 : (nframes <= 4 || nframes >= 32) ? 9 : 5; rv = batch ? 7 : 8; break;
     case 1024:
         cv = batch ? 9 : 14;
-        rv = !batch ? 6 : nframes <= 6 ? 7 : nframes <= 9 ? 3 : nframes <= 16 ? 4 : 5;
+        rv = !batch ? 7 : nframes <= 6 ? 7 : nframes <= 9 ? 3 : nframes <= 16 ? 4 : 5;
         break;
     // N=2048: LDS-staged block-synchronised columns for single frames and
     // launches of a few frames (single 96.6 -> 79.4 us, 2 frames 90.4 -> 83.7
