@@ -52,10 +52,11 @@ CANON_STAGE = {"k_cols": 40, "k_rows": 36, "k_normals": 20, "k_rows_nrm": 56}
 # launch's column pass wrote stay in the 256 MB Infinity Cache for its row
 # pass as long as they fit there beside h0: 16 N=1024 frames (192 MB) ran at
 # 12.4 us per frame over the 600-frame sequence, against 13.8 in 8-frame and
-# 15.0 in 32-frame launches (384 MB, read back from HBM).  N=2048 and 4096
-# run fastest in 4-frame launches, smaller N in 32-frame ones.
+# 15.0 in 32-frame launches (384 MB, read back from HBM).  N=2048 runs
+# fastest in 4-frame launches, N=4096 in 8-frame ones, smaller N in 32-frame
+# ones.
 def launch_frames_for(n):
-    return 16 if n == 1024 else 4 if n >= 2048 else 32
+    return 16 if n == 1024 else 4 if n == 2048 else 8 if n == 4096 else 32
 
 
 LAUNCH_FRAMES = launch_frames_for(1024)
@@ -79,8 +80,8 @@ MEASURED_TRAFFIC = {
 def kernel_names(sched):
     """Kernel names of a (column, row) schedule as ocn_get_schedule reports it."""
     cv, rv = sched["cols_variant"], sched["rows_variant"]
-    cols = ("k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_ldsb" if cv >= 15
-            else "k_cols_lds")
+    cols = ("k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_lds2b" if cv >= 17
+            else "k_cols_ldsb" if cv >= 15 else "k_cols_lds")
     rows = "k_rows" if sched["rows_band"] == 0 else "k_rows_nrm_par" if rv >= 6 else "k_rows_nrm"
     return cols, rows
 

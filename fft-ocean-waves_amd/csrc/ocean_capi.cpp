@@ -184,8 +184,10 @@ std::pair<int, int> default_schedule(int N, int nframes)
         break;
     // 4096: k_cols_ldsb (one 4096-point transform over four waves, block
     // barriers at the stage exchanges): 8-frame launches 394 -> 353 us per
-    // frame, single frames 448 -> 406 us, against k_cols radix 16
-    default: cv = 15; rv = batch ? 5 : 4; break;
+    // frame, single frames 448 -> 406 us, against k_cols radix 16; from 4
+    // frames per launch its two-pass form with two columns per strip
+    // (k_cols_lds2b): 8-frame launches 356 -> 330 us per frame
+    default: cv = nframes >= 4 ? 17 : 15; rv = batch ? 5 : 4; break;
     }
     if (!ocn::schedule_valid(N, cv, rv)) {
         cv = 0;

@@ -479,6 +479,108 @@ __device__ __forceinline__ void cols_lds2_strip(const float4* __restrict__ h0, f
     }
 }
 
+// cols_lds2_strip for N >= 2048 (transform groups of several waves, block
+// barriers): block = 2*C*T threads, lds holds 2*C*lds_len(N), h0t is h0
+// transposed (the strip's columns contiguous).  Phase A folds all three
+// spectra; height and Dx go to LDS and each thread keeps its Dz values in
+// registers.  The first pass transforms the 2C height/Dx columns at radix 16
+// and stores Z1; the second moves Dz into LDS and transforms its C columns at
+// radix 8 (N/8 threads each), so every thread of the block takes part in both
+// passes and meets every barrier.  With half the LDS per column of
+// cols_lds_strip, a strip holds twice the columns: at N = 4096 two columns, so
+// each two-row tile row gets 32 contiguous bytes from one block instead of 16.
+template <int N, int C>
+__device__ __forceinline__ void cols_lds2b_strip(const float4* __restrict__ h0t, float2* __restrict__ z1,
+                                                 float2* __restrict__ z2, const float2* __restrict__ tw16,
+                                                 const float2* __restrict__ tw8, const SpecConst& sc, double t_rev,
+                                                 int strip, float2* lds)
+{
+    constexpr int T = N / 16, NT = 2 * C * T, LEN = lds_len(N);
+    constexpr int ITERS = C * N / NT;  // = 8
+    static_assert(T % 64 == 0 && NT <= 1024 && C * (N / 8) == NT, "whole waves; both passes use every thread");
+    float2 dzr[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k   = threadIdx.x + it * NT;
+        const int c   = k % C, ty = k / C;
+        const int txc = min(strip * C + c, N / 2);
+        const int mxc = (N - txc) & (N - 1);
+        const int my  = (N - ty) & (N - 1);
+        float2 ay, ax;
+        pair_fold(sc, h0t[(size_t)txc * N + ty], h0t[(size_t)mxc * N + my], txc - N / 2, ty - N / 2, mxc - N / 2,
+                  my - N / 2, t_rev, ay, ax, dzr[it]);
+        lds[c * LEN + lds_pad(ty)]       = ay;
+        lds[(C + c) * LEN + lds_pad(ty)] = ax;
+    }
+    __syncthreads();
+    {
+        const int g = threadIdx.x / T, j = threadIdx.x % T;
+        float2*   buf = lds + g * LEN;
+        float2    v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = buf[lds_pad(j + r * T)];
+        fft_inverse<N, 16>(v, j, buf, tw16, BlockSync{});
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; ++r) buf[lds_pad(j + r * T)] = v[r];
+    }
+    __syncthreads();
+    // Z1 tile m = k / C (rows 2m, 2m+1) of column c = k % C
+#pragma unroll
+    for (int it = 0; it < ITERS / 2; ++it) {
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, m = k / C;
+        const int tx = strip * C + c;
+        if (tx <= N / 2) {
+            const int    mxc = (N - tx) & (N - 1);
+            const float2 y0  = lds[c * LEN + lds_pad(2 * m)];
+            const float2 x0  = lds[(C + c) * LEN + lds_pad(2 * m)];
+            const float2 y1  = lds[c * LEN + lds_pad(2 * m + 1)];
+            const float2 x1  = lds[(C + c) * LEN + lds_pad(2 * m + 1)];
+            float4*      t1  = ztile<N>(z1, m);
+            t1[tx] = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
+            if (mxc != tx) t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int k = threadIdx.x + it * NT;
+        lds[(k % C) * LEN + lds_pad(k / C)] = dzr[it];
+    }
+    __syncthreads();
+    {
+        constexpr int T8 = N / 8;
+        const int     g = threadIdx.x / T8, j = threadIdx.x % T8;
+        float2*       buf = lds + g * LEN;
+        float2        v[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = buf[lds_pad(j + r * T8)];
+        fft_inverse<N, 8>(v, j, buf, tw8, BlockSync{});
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) buf[lds_pad(j + r * T8)] = v[r];
+    }
+    __syncthreads();
+    // Z2 tile m = k / C, m < N/4: Z2 rows 2m, 2m+1, Z2 row q pairing G_dz rows 2q and 2q+1
+#pragma unroll
+    for (int it = 0; it < ITERS / 4; ++it) {
+        const int k = threadIdx.x + it * NT;
+        const int c = k % C, m = k / C;
+        const int tx = strip * C + c;
+        if (tx <= N / 2) {
+            const int    mxc = (N - tx) & (N - 1);
+            const float2 a   = lds[c * LEN + lds_pad(4 * m)];
+            const float2 b   = lds[c * LEN + lds_pad(4 * m + 1)];
+            const float2 d   = lds[c * LEN + lds_pad(4 * m + 2)];
+            const float2 e   = lds[c * LEN + lds_pad(4 * m + 3)];
+            float4*      t2  = ztile<N>(z2, m);
+            t2[tx] = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
+            if (mxc != tx) t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ row pass + normals + Jacobian: one band
 
 // Row transforms fused with the normal map (nearest taps) and the Jacobian

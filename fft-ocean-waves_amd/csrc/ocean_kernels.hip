@@ -219,6 +219,27 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __
                                                             lds);
 }
 
+// Two-pass variant of k_cols_ldsb (cols_lds2b_strip, N >= 2048): 2C*T threads,
+// half the LDS per column, the transposed h0.  Strips of C columns plus the
+// column N/2 (the last strip), in the block order of k_cols.
+template <int N, int C>
+__global__ __launch_bounds__(2 * C * (N / 16)) void k_cols_lds2b(const float4* __restrict__ h0t,
+                                                                 float2* __restrict__ z1, float2* __restrict__ z2,
+                                                                 const float2* __restrict__ tw16,
+                                                                 const float2* __restrict__ tw8, SpecConst sc,
+                                                                 FrameBatch fb)
+{
+    constexpr int G = strip_group<N, C>();
+    __shared__ float2 lds[2 * C * lds_len(N)];
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int rest  = L / G;
+    const int f     = rest % fb.nframes;
+    const int strip = (rest / fb.nframes) * G + L % G;
+    if (strip > (N / 2) / C) return;  // padding of the last group (whole block)
+    cols_lds2b_strip<N, C>(h0t, z1 + (size_t)f * fb.z1_stride, z2 + (size_t)f * fb.z2_stride, tw16, tw8, sc,
+                           fb.t_rev[f], strip, lds);
+}
+
 // Two-pass variant of k_cols_lds (cols_lds2_strip): 2C*T threads and half the
 // LDS, same block order.
 template <int N, int C>
@@ -602,14 +623,15 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // kind 1: k_cols_lds (LDS-staged, one wave per transform, radix 16 only);
 // kind 2: k_cols_lds2 (the same in two passes through half the LDS);
 // kind 3: k_cols_lds with twice the threads (folds and stores split finer);
-// kind 4: k_cols_ldsb (LDS-staged, block-synchronised transforms, N >= 2048).
+// kind 4: k_cols_ldsb (LDS-staged, block-synchronised transforms, N >= 2048);
+// kind 5: k_cols_lds2b (the same in two passes through half the LDS, N >= 2048).
 struct ColsVariant {
     int E, C, kind;
 };
 constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
                                                          {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1},
                                                          {16, 2, 2}, {16, 4, 2}, {16, 8, 2}, {16, 1, 3}, {16, 2, 3},
-                                                         {16, 1, 4}, {16, 2, 4}};
+                                                         {16, 1, 4}, {16, 2, 4}, {16, 2, 5}, {16, 4, 5}};
 // Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
 // band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused;
 // par = k_rows_nrm_par (bands of R rows whose rows are transformed at once).
@@ -632,6 +654,8 @@ constexpr bool cols_fits()
                3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
     else if constexpr (K == 4)
         return E == 16 && N / 16 > 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024;
+    else if constexpr (K == 5)
+        return E == 16 && N / 16 > 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024;
     else
         return E == 16 && N >= 64 && N / 16 <= 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024 &&
                C <= N / 2 && (C * N) % (2 * C * (N / 16)) == 0;
@@ -673,6 +697,14 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
         if (!a.h0t) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_cols_ldsb<N, C>), dim3(groups * G * a.nframes), dim3(3 * C * (N / 16)), 0, s, a.h0t,
                            a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        return hipGetLastError();
+    } else if constexpr (K == 5) {
+        constexpr int G      = strip_group<N, C>();
+        constexpr int strips = (N / 2) / C + 1;
+        constexpr int groups = (strips + G - 1) / G;
+        if (!a.h0t) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_cols_lds2b<N, C>), dim3(groups * G * a.nframes), dim3(2 * C * (N / 16)), 0, s, a.h0t,
+                           a.z1, a.z2, a.tw[radix_slot(16)], a.tw[radix_slot(8)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 2) {
         constexpr int G      = strip_group<N, C>();
@@ -746,6 +778,8 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 14: e = launch_cols_v<N, 14>(a, s); break;
         case 15: e = launch_cols_v<N, 15>(a, s); break;
         case 16: e = launch_cols_v<N, 16>(a, s); break;
+        case 17: e = launch_cols_v<N, 17>(a, s); break;
+        case 18: e = launch_cols_v<N, 18>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -789,7 +823,8 @@ static bool variant_ok_n(int cols_v, int rows_v)
 #define OCN_FITS(v) cols_fits<N, kColsVariants[v].E, kColsVariants[v].C, kColsVariants[v].kind>()
         OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
         OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9),
-        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14), OCN_FITS(15), OCN_FITS(16)};
+        OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14), OCN_FITS(15), OCN_FITS(16),
+        OCN_FITS(17), OCN_FITS(18)};
 #undef OCN_FITS
 #define OCN_RFITS(v) \
     rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band, kRowsVariants[v].par, kRowsVariants[v].W>()

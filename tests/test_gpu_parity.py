@@ -583,7 +583,7 @@ def test_every_schedule_variant_matches_oracle(hip, N):
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
         tried = 0
-        for cv in range(17):
+        for cv in range(19):
             fused = {}  # first fused row variant's frame per row radix under this column variant
             for rv in range(9):
                 try:
