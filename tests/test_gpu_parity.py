@@ -526,20 +526,23 @@ def test_full_size_point_samples_and_parseval(hip, N):
     assert abs(e_got - e_ref) <= 1e-5 * e_ref, (e_got, e_ref)
 
 
-@pytest.mark.parametrize("N", [2048, 4096])
-def test_full_size_multiframe_launch(hip, N):
-    """Two frames in one launch take the multi-frame schedule, at these N the
-    fused row + normal-map + Jacobian walk (bands of 8 rows at N=2048, 32 at
-    N=4096) that reads Z2 rows one at a time from their two-row tiles, after
-    the LDS-staged block-synchronised column pass.  Height, Dx and Dz of both
-    frames against direct-DFT point samples; normals and Jacobian against the
+@pytest.mark.parametrize("N,F,want", [(2048, 2, (16, 8)), (4096, 2, (15, 32)), (4096, 8, (17, 32))],
+                         ids=["2048_2_frames", "4096_2_frames", "4096_8_frames"])
+def test_full_size_multiframe_launch(hip, N, F, want):
+    """F frames in one launch take the multi-frame schedule `want` = (column
+    variant, rows per band): the fused row + normal-map + Jacobian walk (bands
+    of 8 rows at N=2048, 32 at N=4096) that reads Z2 rows one at a time from
+    their two-row tiles, after the LDS-staged block-synchronised column pass
+    (k_cols_ldsb with 2 or 1 columns; from 4 frames at N=4096 its two-pass form
+    k_cols_lds2b with 2).  Height, Dx and Dz of the first two frames and the
+    last against direct-DFT point samples; normals and Jacobian against the
     oracle's stencils applied to the GPU's own height, Dx and Dz over the
     whole field, which covers every band seam (the halo rows of each band)."""
     kw = dict(N=N, L=2000.0, wind_speed=25.0, wind_dir=(math.cos(0.5), math.sin(0.5)), amplitude=4.0,
               choppiness=1.3)
     lp, op = both_params(**kw)
-    times = np.array([7.25, 7.25 + 1.0 / 60.0])
-    F, nn = len(times), N * N
+    times = 7.25 + np.arange(F) / 60.0
+    nn = N * N
     names = ("height", "dx", "dz", "normals", "jacobian")
     sizes = {"height": 4, "dx": 4, "dz": 4, "normals": 16, "jacobian": 4}
     bufs = {k: hip.malloc(F * nn * sizes[k]) for k in names}
@@ -549,19 +552,20 @@ def test_full_size_multiframe_launch(hip, N):
             oc.tilde_h0_k()
             oc.reserve_frames(F)
             sch = oc.schedule(F)
-            assert sch["rows_band"] == (8 if N == 2048 else 32) and sch["cols_variant"] in (15, 16), sch
+            assert (sch["cols_variant"], sch["rows_band"]) == want, sch
             oc.update_frames(times, pyocean.OcnFields(*[bufs[k] for k in names]))
             hip.sync()
-            got = [{k: hip.download(bufs[k] + f * nn * sizes[k], (N, N, 4) if k == "normals" else (N, N))
-                    for k in names} for f in range(F)]
+            checked = sorted({0, 1, F - 1})
+            got = {f: {k: hip.download(bufs[k] + f * nn * sizes[k], (N, N, 4) if k == "normals" else (N, N))
+                       for k in names} for f in checked}
     finally:
         for b in bufs.values():
             hip.free(b)
     h0 = oracle.h0(op)
     rng = np.random.default_rng(N + 1)
     pts = [(int(a), int(b)) for a, b in rng.integers(0, N, size=(8, 2))] + [(0, 0), (N - 1, 31), (5, 32)]
-    for f, t in enumerate(times):
-        g = got[f]
+    for f in checked:
+        g, t = got[f], times[f]
         ref = _direct_points(op, h0, t, pts)
         for i, (x, y) in enumerate(pts):
             for k, fld in enumerate(("height", "dx", "dz")):
