@@ -165,12 +165,17 @@ std::pair<int, int> default_schedule(int N, int nframes)
     // walks.  Launches whose intermediates fit the 256 MB Infinity Cache
     // beside h0 (up to 20 N=1024 frames, 4 N=2048 frames) are the fastest per
     // frame, since the row pass then reads them from there.
+    // Column strips over pairs of frames (variant 19, k_cols_lds_f2 with 4
+    // columns: one h0 load and dispersion per texel pair for two frames) at
+    // N <= 512 where they measured faster per frame over the sequence: N=512
+    // from 8 frames per launch (8: 4.59 -> 4.41 us, 32: 3.31-3.46 -> 3.25),
+    // N=256 from 16 (32: 1.09 -> 1.07), N=128 at 32 (0.73 -> 0.71), N=64 from 8.
     int cv, rv;
     switch (N) {
-    case 64: cv = batch ? 9 : 14; rv = batch ? 1 : 8; break;
-    case 128: cv = batch ? 9 : 14; rv = batch ? 2 : 8; break;
-    case 256: cv = batch ? 9 : 13; rv = 8; break;
-    case 512: cv = !batch ? 14 : (nframes <= 4 || nframes >= 32) ? 9 : 5; rv = batch ? 7 : 8; break;
+    case 64: cv = !batch ? 14 : nframes >= 8 ? 19 : 9; rv = batch ? 1 : 8; break;
+    case 128: cv = !batch ? 14 : nframes >= 32 ? 19 : 9; rv = batch ? 2 : 8; break;
+    case 256: cv = !batch ? 13 : nframes >= 16 ? 19 : 9; rv = 8; break;
+    case 512: cv = !batch ? 14 : nframes <= 4 ? 9 : 19; rv = batch ? 7 : 8; break;
     case 1024:
         cv = batch ? 9 : 14;
         rv = !batch ? 7 : nframes <= 6 ? 7 : nframes <= 9 ? 3 : nframes <= 16 ? 4 : 5;

@@ -160,6 +160,22 @@ __device__ __forceinline__ float2 choppy(float m, float2 h) { return make_float2
 // mirror t' = -t mod N.  Both share |k| (n' = -n except on the Nyquist row or
 // column, where n' = n = -N/2), hence one phase and one sincos for the pair.
 // p = h0 at t, q = h0 at t'; (nx, ny) and (qx, qy) are their centred indices.
+// pair_fold_at takes the phase fr (turns) and the choppy factor kinv of the
+// pair, so that frames of one texel pair share the dispersion evaluation.
+__device__ __forceinline__ void pair_fold_at(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
+                                             float kinv, float fr, float2& ay, float2& ax, float2& az)
+{
+    float s, c;
+    sincos_turns(fr, s, c);
+    const float2 H = evolve_amp(sc, p, s, c);
+    const float2 G = evolve_amp(sc, q, s, c);
+    const float mx = (float)nx * kinv, mz = (float)ny * kinv;
+    const float gx = (float)qx * kinv, gz = (float)qy * kinv;
+    ay = make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
+    ax = make_float2(0.5f * (mx * H.y + gx * G.y), 0.5f * (gx * G.x - mx * H.x));
+    az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
+}
+
 __device__ __forceinline__ void pair_fold(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
                                           double t_rev, float2& ay, float2& ax, float2& az)
 {
@@ -264,6 +280,102 @@ __host__ __device__ constexpr int lds_strips()
     return (N / 2) / C;
 }
 
+// Phases of cols_lds_strip (below), shared with the two-frame strip.
+// Strip 0, after the folds: column slot 0 += i * (the Nyquist column's fold),
+// its N rows spread over the whole block (strip is the same for every thread
+// of the block).  Begins with a block barrier.
+template <int N, int C, int NB, bool HT, int CL = C>
+__device__ __forceinline__ void strip_nyquist(const float4* __restrict__ h0, const SpecConst& sc, double t_rev,
+                                              float2* lds)
+{
+    auto at = [&](int x, int y) -> float4 { return HT ? h0[(size_t)x * N + y] : h0[(size_t)y * N + x]; };
+    constexpr int LEN = lds_len(N);
+    __syncthreads();
+    for (int ty = threadIdx.x; ty < N; ty += NB) {
+        const int my = (N - ty) & (N - 1);
+        float2 by, bx, bz;
+        pair_fold(sc, at(N / 2, ty), at(N / 2, my), 0, ty - N / 2, 0, my - N / 2, t_rev, by, bx, bz);
+        float2& ay = lds[0 * CL * LEN + lds_pad(ty)];
+        float2& ax = lds[1 * CL * LEN + lds_pad(ty)];
+        float2& az = lds[2 * CL * LEN + lds_pad(ty)];
+        ay = make_float2(ay.x - by.y, ay.y + by.x);
+        ax = make_float2(ax.x - bx.y, ax.y + bx.x);
+        az = make_float2(az.x - bz.y, az.y + bz.x);
+    }
+}
+
+// The 3C column transforms in place, one per T-thread group of the first NT
+// threads (Sync: WaveSync when a group lies in one wave, else BlockSync).
+template <int N, int C, class Sync, int NB>
+__device__ __forceinline__ void strip_transform(const float2* __restrict__ tw, float2* lds)
+{
+    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
+    const bool active = NB == NT || (int)threadIdx.x < NT;  // takes part in the transforms
+    if (active) {
+        const int g = threadIdx.x / T, j = threadIdx.x % T;
+        float2* buf = lds + g * LEN;
+        float2  v[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
+        fft_inverse<N, E>(v, j, buf, tw, Sync{});
+        Sync{}();
+#pragma unroll
+        for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
+    }
+}
+
+// Z1 = G_h + i G_dx and Z2 = G_dz(2p) + i G_dz(2p+1) of the strip's columns
+// and their mirrors from the transformed LDS buffers, in two-row tiles:
+// thread k stores Z1 tile m = k / C (rows 2m, 2m+1) and, for m < N/4, Z2
+// tile m (Z2 rows 2m, 2m+1, i.e. G_dz rows 4m..4m+3) of column c = k % C.
+template <int N, int C, int NB, int CL = C>
+__device__ __forceinline__ void strip_store(float2* __restrict__ z1, float2* __restrict__ z2, int strip,
+                                            const float2* lds)
+{
+    constexpr int LEN = lds_len(N);
+    constexpr int ITZ = (C * N / 2 + NB - 1) / NB;
+#pragma unroll
+    for (int it = 0; it < ITZ; ++it) {
+        const int k = threadIdx.x + it * NB;
+        if (k < C * N / 2) {
+            const int    c   = k % C, m = k / C;
+            const int    tx  = strip * C + c;
+            const float2 y0  = lds[(0 * CL + c) * LEN + lds_pad(2 * m)];
+            const float2 x0  = lds[(1 * CL + c) * LEN + lds_pad(2 * m)];
+            const float2 y1  = lds[(0 * CL + c) * LEN + lds_pad(2 * m + 1)];
+            const float2 x1  = lds[(1 * CL + c) * LEN + lds_pad(2 * m + 1)];
+            float4*      t1  = ztile<N>(z1, m);
+            const bool   z2t = m < N / 4;  // Z2 row q pairs G_dz rows 2q and 2q+1
+            float2       a{}, b{}, d{}, e{};
+            if (z2t) {
+                a = lds[(2 * CL + c) * LEN + lds_pad(4 * m)];
+                b = lds[(2 * CL + c) * LEN + lds_pad(4 * m + 1)];
+                d = lds[(2 * CL + c) * LEN + lds_pad(4 * m + 2)];
+                e = lds[(2 * CL + c) * LEN + lds_pad(4 * m + 3)];
+            }
+            float4* t2 = ztile<N>(z2, m);
+            if (tx == 0) {
+                // packed self-mirror columns: real parts are column 0's (real)
+                // transforms, imaginary parts column N/2's
+                t1[0]     = make_float4(y0.x, x0.x, y1.x, x1.x);
+                t1[N / 2] = make_float4(y0.y, x0.y, y1.y, x1.y);
+                if (z2t) {
+                    t2[0]     = make_float4(a.x, b.x, d.x, e.x);
+                    t2[N / 2] = make_float4(a.y, b.y, d.y, e.y);
+                }
+            } else {
+                const int mxc = N - tx;
+                t1[tx]        = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
+                t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
+                if (z2t) {
+                    t2[tx]        = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
+                    t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
+                }
+            }
+        }
+    }
+}
+
 // LDS-staged column transforms of one strip: the C half-plane columns
 // tx = strip*C + c, block of 3*C*T threads (T = N/16), frame time t_rev;
 // z1/z2 point at the frame's intermediate, lds holds 3*C*lds_len(N).
@@ -298,7 +410,6 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
     static_assert((N / 2) % C == 0, "strips of C columns tile the half plane");
-    const bool active = NB == NT || (int)threadIdx.x < NT;  // takes part in the transforms
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NB;
@@ -315,78 +426,61 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
             lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
         }
     }
-    if (strip == 0) {
-        // slot 0 += i * (the Nyquist column's fold), its N rows spread over the
-        // whole block (strip is the same for every thread of the block)
-        __syncthreads();
-        for (int ty = threadIdx.x; ty < N; ty += NB) {
-            const int my = (N - ty) & (N - 1);
-            float2 by, bx, bz;
-            pair_fold(sc, at(N / 2, ty), at(N / 2, my), 0, ty - N / 2, 0, my - N / 2, t_rev, by, bx, bz);
-            float2& ay = lds[0 * C * LEN + lds_pad(ty)];
-            float2& ax = lds[1 * C * LEN + lds_pad(ty)];
-            float2& az = lds[2 * C * LEN + lds_pad(ty)];
-            ay = make_float2(ay.x - by.y, ay.y + by.x);
-            ax = make_float2(ax.x - bx.y, ax.y + bx.x);
-            az = make_float2(az.x - bz.y, az.y + bz.x);
-        }
-    }
+    if (strip == 0) strip_nyquist<N, C, NB, HT>(h0, sc, t_rev, lds);
     __syncthreads();
-    if (active) {
-        const int g = threadIdx.x / T, j = threadIdx.x % T;
-        float2* buf = lds + g * LEN;
-        float2  v[E];
-#pragma unroll
-        for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
-        fft_inverse<N, E>(v, j, buf, tw, Sync{});
-        Sync{}();
-#pragma unroll
-        for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
-    }
+    strip_transform<N, C, Sync, NB>(tw, lds);
     __syncthreads();
-    // thread k stores Z1 tile m = k / C (rows 2m, 2m+1) and, for m < N/4, Z2
-    // tile m (Z2 rows 2m, 2m+1, i.e. G_dz rows 4m..4m+3) of column c = k % C
-    constexpr int ITZ = (C * N / 2 + NB - 1) / NB;
+    strip_store<N, C, NB>(z1, z2, strip, lds);
+}
+
+// cols_lds_strip for two frames (times ta, tb) of the same C columns, both
+// in LDS at once as 2C buffers per field (frame a's columns, then frame
+// b's): each texel pair's h0 values are loaded and its dispersion evaluated
+// once for both frames.  Block = 6*C*T threads (NB), the 6C transforms one
+// per T-thread group; lds holds 6*C*lds_len(N).
+template <int N, int C, class Sync, int NB = 6 * C * (N / 16)>
+__device__ __forceinline__ void cols_lds_strip_f2(const float4* __restrict__ h0, float2* __restrict__ z1a,
+                                                  float2* __restrict__ z2a, float2* __restrict__ z1b,
+                                                  float2* __restrict__ z2b, const float2* __restrict__ tw,
+                                                  const SpecConst& sc, double ta, double tb, int strip, float2* lds)
+{
+    constexpr int E = 16, T = N / E, NT = 6 * C * T, LEN = lds_len(N), CL = 2 * C;
+    constexpr int ITERS = (C * N + NB - 1) / NB;
+    static_assert(NB >= NT, "block too small for the strip");
+    static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
+    static_assert((N / 2) % C == 0, "strips of C columns tile the half plane");
 #pragma unroll
-    for (int it = 0; it < ITZ; ++it) {
+    for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NB;
-        if (k < C * N / 2) {
-            const int    c   = k % C, m = k / C;
-            const int    tx  = strip * C + c;
-            const float2 y0  = lds[(0 * C + c) * LEN + lds_pad(2 * m)];
-            const float2 x0  = lds[(1 * C + c) * LEN + lds_pad(2 * m)];
-            const float2 y1  = lds[(0 * C + c) * LEN + lds_pad(2 * m + 1)];
-            const float2 x1  = lds[(1 * C + c) * LEN + lds_pad(2 * m + 1)];
-            float4*      t1  = ztile<N>(z1, m);
-            const bool   z2t = m < N / 4;  // Z2 row q pairs G_dz rows 2q and 2q+1
-            float2       a{}, b{}, d{}, e{};
-            if (z2t) {
-                a = lds[(2 * C + c) * LEN + lds_pad(4 * m)];
-                b = lds[(2 * C + c) * LEN + lds_pad(4 * m + 1)];
-                d = lds[(2 * C + c) * LEN + lds_pad(4 * m + 2)];
-                e = lds[(2 * C + c) * LEN + lds_pad(4 * m + 3)];
-            }
-            float4* t2 = ztile<N>(z2, m);
-            if (tx == 0) {
-                // packed self-mirror columns: real parts are column 0's (real)
-                // transforms, imaginary parts column N/2's
-                t1[0]     = make_float4(y0.x, x0.x, y1.x, x1.x);
-                t1[N / 2] = make_float4(y0.y, x0.y, y1.y, x1.y);
-                if (z2t) {
-                    t2[0]     = make_float4(a.x, b.x, d.x, e.x);
-                    t2[N / 2] = make_float4(a.y, b.y, d.y, e.y);
-                }
-            } else {
-                const int mxc = N - tx;
-                t1[tx]        = make_float4(y0.x - x0.y, y0.y + x0.x, y1.x - x1.y, y1.y + x1.x);
-                t1[zpos<N>(mxc)] = make_float4(y0.x + x0.y, x0.x - y0.y, y1.x + x1.y, x1.x - y1.y);
-                if (z2t) {
-                    t2[tx]        = make_float4(a.x - b.y, a.y + b.x, d.x - e.y, d.y + e.x);
-                    t2[zpos<N>(mxc)] = make_float4(a.x + b.y, b.x - a.y, d.x + e.y, e.x - d.y);
-                }
-            }
+        if (k < C * N) {
+            const int c   = k % C, ty = k / C;
+            const int tx  = strip * C + c;  // < N/2
+            const int mxc = (N - tx) & (N - 1);
+            const int my  = (N - ty) & (N - 1);
+            const int nx = tx - N / 2, ny = ty - N / 2, qx = mxc - N / 2, qy = my - N / 2;
+            const float4 p = h0[(size_t)ty * N + tx], q = h0[(size_t)my * N + mxc];
+            float        kinv;
+            const double w = omega_fp64(sc, nx * nx + ny * ny, kinv);
+            float2       ay, ax, az;
+            pair_fold_at(sc, p, q, nx, ny, qx, qy, kinv, turns(w, ta), ay, ax, az);
+            lds[(0 * CL + c) * LEN + lds_pad(ty)] = ay;
+            lds[(1 * CL + c) * LEN + lds_pad(ty)] = ax;
+            lds[(2 * CL + c) * LEN + lds_pad(ty)] = az;
+            pair_fold_at(sc, p, q, nx, ny, qx, qy, kinv, turns(w, tb), ay, ax, az);
+            lds[(0 * CL + C + c) * LEN + lds_pad(ty)] = ay;
+            lds[(1 * CL + C + c) * LEN + lds_pad(ty)] = ax;
+            lds[(2 * CL + C + c) * LEN + lds_pad(ty)] = az;
         }
     }
+    if (strip == 0) {
+        strip_nyquist<N, C, NB, false, CL>(h0, sc, ta, lds);
+        strip_nyquist<N, C, NB, false, CL>(h0, sc, tb, lds + C * LEN);
+    }
+    __syncthreads();
+    strip_transform<N, 2 * C, Sync, NB>(tw, lds);
+    __syncthreads();
+    strip_store<N, C, NB, CL>(z1a, z2a, strip, lds);
+    strip_store<N, C, NB, CL>(z1b, z2b, strip, lds + C * LEN);
 }
 
 // The same strip in two passes through half the LDS: block = 2*C*T threads,

@@ -9,7 +9,7 @@
 namespace ocn {
 
 constexpr int kMaxFrames        = 32;  // frames per launch of the fused path
-constexpr int kNumColsVariants  = 19;  // column-pass (kernel, radix, columns/block) schedules
+constexpr int kNumColsVariants  = 21;  // column-pass (kernel, radix, columns/block) schedules
 constexpr int kNumRowsVariants  = 9;   // row-pass schedules (radix, fused band, parallel bands)
 
 // Per-ocean spectrum constants, FP32 for the kernels plus the FP64 pieces of

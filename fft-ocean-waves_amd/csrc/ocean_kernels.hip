@@ -194,6 +194,37 @@ __global__ __launch_bounds__(W * 3 * C * (N / 16)) void k_cols_lds(const float4*
                                                   tw, sc, fb.t_rev[f], strip, lds);
 }
 
+// k_cols_lds over pairs of frames: block (strip, frames 2i and 2i+1) holds
+// both frames of its C columns in LDS, loading each texel pair's h0 and
+// evaluating its dispersion once for both (cols_lds_strip_f2); an odd last
+// frame takes a block of its own.  Blocks of one strip group and consecutive
+// frame pairs are consecutive on one XCD, as in k_cols_lds.  Block = 6C*T
+// threads.
+template <int N, int C>
+__global__ __launch_bounds__(6 * C * (N / 16)) void k_cols_lds_f2(const float4* __restrict__ h0,
+                                                                  float2* __restrict__ z1, float2* __restrict__ z2,
+                                                                  const float2* __restrict__ tw, SpecConst sc,
+                                                                  FrameBatch fb)
+{
+    constexpr int T = N / 16, NB = 6 * C * T;
+    constexpr int G = strip_group<N, C>();
+    static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
+    __shared__ float2 lds[6 * C * lds_len(N)];
+    const int L     = xcd_logical(blockIdx.x, gridDim.x);
+    const int pairs = (fb.nframes + 1) / 2;
+    const int rest  = L / G;
+    const int f     = 2 * (rest % pairs);
+    const int strip = (rest / pairs) * G + L % G;
+    if (strip >= lds_strips<N, C>()) return;  // padding of the last group (whole block)
+    float2* z1a = z1 + (size_t)f * fb.z1_stride;
+    float2* z2a = z2 + (size_t)f * fb.z2_stride;
+    if (f + 1 < fb.nframes)
+        cols_lds_strip_f2<N, C, WaveSync, NB>(h0, z1a, z2a, z1a + fb.z1_stride, z2a + fb.z2_stride, tw, sc,
+                                              fb.t_rev[f], fb.t_rev[f + 1], strip, lds);
+    else
+        cols_lds_strip<N, C, WaveSync, NB>(h0, z1a, z2a, tw, sc, fb.t_rev[f], strip, lds);
+}
+
 // k_cols_lds for N >= 2048, whose N/16-thread transform groups span several
 // waves: the same strip (cols_lds_strip) with block barriers between the
 // transform stages, reading the transposed h0 (h0t), where a strip's columns
@@ -624,14 +655,16 @@ __global__ __launch_bounds__(C*(N / E)) void k_c2c_cols_inv(const float2* __rest
 // kind 2: k_cols_lds2 (the same in two passes through half the LDS);
 // kind 3: k_cols_lds with twice the threads (folds and stores split finer);
 // kind 4: k_cols_ldsb (LDS-staged, block-synchronised transforms, N >= 2048);
-// kind 5: k_cols_lds2b (the same in two passes through half the LDS, N >= 2048).
+// kind 5: k_cols_lds2b (the same in two passes through half the LDS, N >= 2048);
+// kind 6: k_cols_lds_f2 (k_cols_lds over pairs of frames).
 struct ColsVariant {
     int E, C, kind;
 };
 constexpr ColsVariant kColsVariants[kNumColsVariants] = {{16, 1, 0}, {16, 2, 0}, {16, 4, 0}, {16, 8, 0}, {8, 2, 0},
                                                          {8, 4, 0},  {8, 8, 0},  {16, 1, 1}, {16, 2, 1}, {16, 4, 1},
                                                          {16, 2, 2}, {16, 4, 2}, {16, 8, 2}, {16, 1, 3}, {16, 2, 3},
-                                                         {16, 1, 4}, {16, 2, 4}, {16, 2, 5}, {16, 4, 5}};
+                                                         {16, 1, 4}, {16, 2, 4}, {16, 2, 5}, {16, 4, 5},
+                                                         {16, 4, 6}, {16, 8, 6}};
 // Row-pass schedules: band 0 = k_rows (radix E) followed by k_normals;
 // band R > 0 = k_rows_nrm with bands of R rows (radix E), normals fused;
 // par = k_rows_nrm_par (bands of R rows whose rows are transformed at once).
@@ -649,9 +682,9 @@ constexpr bool cols_fits()
 {
     if constexpr (K == 0)
         return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
-    else if constexpr (K == 1 || K == 3)
-        return E == 16 && N / 16 <= 64 && (K == 3 ? 2 : 1) * 3 * C * (N / 16) <= 1024 &&
-               3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+    else if constexpr (K == 1 || K == 3 || K == 6)
+        return E == 16 && N / 16 <= 64 && (K == 1 ? 1 : 2) * 3 * C * (N / 16) <= 1024 &&
+               (K == 6 ? 2 : 1) * 3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
     else if constexpr (K == 4)
         return E == 16 && N / 16 > 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024;
     else if constexpr (K == 5)
@@ -689,6 +722,13 @@ static hipError_t launch_cols_v(const FrameArgs& a, hipStream_t s)
         constexpr int groups = (strips + G - 1) / G;
         hipLaunchKernelGGL((k_cols_lds<N, C, W>), dim3(groups * G * a.nframes), dim3(W * 3 * C * (N / 16)), 0, s,
                            a.h0, a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
+        return hipGetLastError();
+    } else if constexpr (K == 6) {
+        constexpr int G      = strip_group<N, C>();
+        constexpr int strips = lds_strips<N, C>();
+        constexpr int groups = (strips + G - 1) / G;
+        hipLaunchKernelGGL((k_cols_lds_f2<N, C>), dim3(groups * G * ((a.nframes + 1) / 2)), dim3(6 * C * (N / 16)), 0,
+                           s, a.h0, a.z1, a.z2, a.tw[radix_slot(16)], a.sc, a.fb);
         return hipGetLastError();
     } else if constexpr (K == 4) {
         constexpr int G      = strip_group<N, C>();
@@ -780,6 +820,8 @@ static hipError_t launch_frame_n(const FrameArgs& a, int stages, hipStream_t s)
         case 16: e = launch_cols_v<N, 16>(a, s); break;
         case 17: e = launch_cols_v<N, 17>(a, s); break;
         case 18: e = launch_cols_v<N, 18>(a, s); break;
+        case 19: e = launch_cols_v<N, 19>(a, s); break;
+        case 20: e = launch_cols_v<N, 20>(a, s); break;
         default: e = hipErrorInvalidValue;
         }
         if (e != hipSuccess) return e;
@@ -824,7 +866,7 @@ static bool variant_ok_n(int cols_v, int rows_v)
         OCN_FITS(0), OCN_FITS(1), OCN_FITS(2), OCN_FITS(3), OCN_FITS(4),
         OCN_FITS(5), OCN_FITS(6), OCN_FITS(7), OCN_FITS(8), OCN_FITS(9),
         OCN_FITS(10), OCN_FITS(11), OCN_FITS(12), OCN_FITS(13), OCN_FITS(14), OCN_FITS(15), OCN_FITS(16),
-        OCN_FITS(17), OCN_FITS(18)};
+        OCN_FITS(17), OCN_FITS(18), OCN_FITS(19), OCN_FITS(20)};
 #undef OCN_FITS
 #define OCN_RFITS(v) \
     rows_fits<N, kRowsVariants[v].E, kRowsVariants[v].band, kRowsVariants[v].par, kRowsVariants[v].W>()

@@ -15,6 +15,8 @@ from ocean_testutil import (assert_field_close, both_params, c1_kwargs, c2_kwarg
 pytestmark = pytest.mark.gpu
 
 NTHREADS = min(16, os.cpu_count() or 1)
+N_COLS_VARIANTS = 21  # column-pass schedules of this library (ocn_set_schedule numbering)
+FRAME_PAIR_VARIANTS = (19, 20)  # k_cols_lds_f2: strips of 4 and 8 columns over pairs of frames
 
 
 def _say(msg):
@@ -587,7 +589,7 @@ def test_every_schedule_variant_matches_oracle(hip, N):
     with pyocean.Ocean(lp) as oc:
         oc.tilde_h0_k()
         tried = 0
-        for cv in range(19):
+        for cv in range(N_COLS_VARIANTS):
             fused = {}  # first fused row variant's frame per row radix under this column variant
             for rv in range(9):
                 try:
@@ -614,6 +616,52 @@ def test_every_schedule_variant_matches_oracle(hip, N):
                             np.testing.assert_array_equal(got[fld], fused[key][fld],
                                                           err_msg="%s[%d,%d] vs first band height" % (fld, cv, rv))
         assert tried >= 4
+
+
+@pytest.mark.parametrize("N", [64, 128, 256, 512])
+def test_frame_pair_column_strips_match_oracle(hip, N):
+    """The column strips over pairs of frames (k_cols_lds_f2: one h0 load and
+    dispersion per texel pair for both frames) in 5-frame launches (two pairs
+    and an odd frame): every frame against the oracle, and bit-equal to the
+    one-frame strips of the same radix (variant 9), whose per-column
+    arithmetic is the same."""
+    kw = dict(c2_kwargs(), N=N, L=N * 2.0)
+    lp, op = both_params(**kw)
+    F = 5
+    times = 1.7 + np.arange(F) / 60.0
+    h0 = oracle.h0(op)
+    frames = {}
+    with pyocean.Ocean(lp) as oc:
+        oc.tilde_h0_k()
+        oc.reserve_frames(F)
+        rv = oc.schedule(F)["rows_variant"]
+        bufs = {k: hip.malloc(F * N * N * (16 if k == "normals" else 4))
+                for k in ("height", "dx", "dz", "normals", "jacobian")}
+        try:
+            fl = pyocean.OcnFields(bufs["height"], bufs["dx"], bufs["dz"], bufs["normals"], bufs["jacobian"])
+            for cv in (9,) + FRAME_PAIR_VARIANTS:
+                try:
+                    oc.set_schedule(cv, rv)
+                except pyocean.OceanError:
+                    continue
+                _say("frame pairs N=%d cols=%d rows=%d" % (N, cv, rv))
+                oc.update_frames(times, fl)
+                hip.sync()
+                frames[cv] = {k: hip.download(v, (F, N, N, 4) if k == "normals" else (F, N, N)).copy()
+                              for k, v in bufs.items()}
+        finally:
+            for v in bufs.values():
+                hip.free(v)
+    pair = [cv for cv in FRAME_PAIR_VARIANTS if cv in frames]
+    assert 9 in frames and pair, sorted(frames)
+    for f in range(F):
+        ref = oracle.frame(op, h0, times[f], nthreads=NTHREADS)
+        for cv in pair:
+            for fld in ("height", "dx", "dz", "normals", "jacobian"):
+                assert_field_close("%s[cols=%d f=%d]" % (fld, cv, f), frames[cv][fld][f], ref[fld])
+    for cv in pair:
+        for fld in frames[9]:
+            np.testing.assert_array_equal(frames[cv][fld], frames[9][fld], err_msg="%s cols=%d vs 9" % (fld, cv))
 
 
 def test_multiframe_launch_matches_oracle_per_frame(hip):
