@@ -80,8 +80,8 @@ MEASURED_TRAFFIC = {
 def kernel_names(sched):
     """Kernel names of a (column, row) schedule as ocn_get_schedule reports it."""
     cv, rv = sched["cols_variant"], sched["rows_variant"]
-    cols = ("k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_lds2b" if cv >= 17
-            else "k_cols_ldsb" if cv >= 15 else "k_cols_lds")
+    cols = ("k_cols" if cv < 7 else "k_cols_lds2" if 10 <= cv < 13 else "k_cols_lds_f2" if cv >= 19
+            else "k_cols_lds2b" if cv >= 17 else "k_cols_ldsb" if cv >= 15 else "k_cols_lds")
     rows = "k_rows" if sched["rows_band"] == 0 else "k_rows_nrm_par" if rv >= 6 else "k_rows_nrm"
     return cols, rows
 

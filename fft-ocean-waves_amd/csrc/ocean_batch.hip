@@ -32,7 +32,7 @@ constexpr int cols_lds_need()
 {
     int m = 0;
     for (int N = kBatchMinN; N <= kBatchMaxN; N *= 2) {
-        const int need = 3 * batch_cols_per_block(N) * lds_len(N);
+        const int need = 3 * batch_cols_per_block(N) * strip_len(N);
         m = m > need ? m : need;
     }
     return m;
@@ -46,7 +46,7 @@ constexpr int rows_lds_need()
     }
     return m;
 }
-constexpr int kColsLds = cols_lds_need();  // 3 * C * lds_len(N)
+constexpr int kColsLds = cols_lds_need();  // 3 * C * strip_len(N)
 constexpr int kRowsLds = rows_lds_need();  // (256 / T) * lds_len(N), T = N/8 (N = 4096: the wide launch)
 
 template <int N>

@@ -280,6 +280,18 @@ __host__ __device__ constexpr int lds_strips()
     return (N / 2) / C;
 }
 
+// Stride (float2) of the per-(field, column) LDS buffers of cols_lds_strip:
+// lds_len(N) plus a few elements.  The fold writes of 16 lanes (4 columns by
+// 4 consecutive rows) and the store reads then fall on distinct banks more
+// often: with a stride of 1 mod 32 the 32 banks of a 16-lane b64 access saw
+// up to 4 lanes per bank in the fold writes.  A bank model of the strip's
+// accesses (fold writes, transforms, store reads) gives 7424 instead of 8960
+// LDS cycles per block at N = 1024 and C = 4, and no width C in 1..8 more.
+__host__ __device__ constexpr int strip_len(int N)
+{
+    return N == 1024 ? 1091 : N == 512 ? 547 : N == 256 ? 275 : N == 128 ? 152 : N == 64 ? 76 : lds_len(N);
+}
+
 // Phases of cols_lds_strip (below), shared with the two-frame strip.
 // Strip 0, after the folds: column slot 0 += i * (the Nyquist column's fold),
 // its N rows spread over the whole block (strip is the same for every thread
@@ -289,7 +301,7 @@ __device__ __forceinline__ void strip_nyquist(const float4* __restrict__ h0, con
                                               float2* lds)
 {
     auto at = [&](int x, int y) -> float4 { return HT ? h0[(size_t)x * N + y] : h0[(size_t)y * N + x]; };
-    constexpr int LEN = lds_len(N);
+    constexpr int LEN = strip_len(N);
     __syncthreads();
     for (int ty = threadIdx.x; ty < N; ty += NB) {
         const int my = (N - ty) & (N - 1);
@@ -309,7 +321,7 @@ __device__ __forceinline__ void strip_nyquist(const float4* __restrict__ h0, con
 template <int N, int C, class Sync, int NB>
 __device__ __forceinline__ void strip_transform(const float2* __restrict__ tw, float2* lds)
 {
-    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
+    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = strip_len(N);
     const bool active = NB == NT || (int)threadIdx.x < NT;  // takes part in the transforms
     if (active) {
         const int g = threadIdx.x / T, j = threadIdx.x % T;
@@ -332,7 +344,7 @@ template <int N, int C, int NB, int CL = C>
 __device__ __forceinline__ void strip_store(float2* __restrict__ z1, float2* __restrict__ z2, int strip,
                                             const float2* lds)
 {
-    constexpr int LEN = lds_len(N);
+    constexpr int LEN = strip_len(N);
     constexpr int ITZ = (C * N / 2 + NB - 1) / NB;
 #pragma unroll
     for (int it = 0; it < ITZ; ++it) {
@@ -378,7 +390,7 @@ __device__ __forceinline__ void strip_store(float2* __restrict__ z1, float2* __r
 
 // LDS-staged column transforms of one strip: the C half-plane columns
 // tx = strip*C + c, block of 3*C*T threads (T = N/16), frame time t_rev;
-// z1/z2 point at the frame's intermediate, lds holds 3*C*lds_len(N).
+// z1/z2 point at the frame's intermediate, lds holds 3*C*strip_len(N).
 // Phase A: every thread evaluates Hermitian parts (consecutive threads take
 // consecutive columns, then rows) into per-(field, column) LDS buffers.
 // Phase B: each of the 3C transforms runs inside one T-thread group (Sync:
@@ -405,7 +417,7 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     // C); the 3C transforms take the first NT.  A block wider than NT (a
     // single frame, whose few blocks leave the SIMDs short of waves, or the
     // batch's block shared by every N) splits the folds and stores finer.
-    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = lds_len(N);
+    constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = strip_len(N);
     constexpr int ITERS = (C * N + NB - 1) / NB;
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
@@ -437,14 +449,14 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
 // in LDS at once as 2C buffers per field (frame a's columns, then frame
 // b's): each texel pair's h0 values are loaded and its dispersion evaluated
 // once for both frames.  Block = 6*C*T threads (NB), the 6C transforms one
-// per T-thread group; lds holds 6*C*lds_len(N).
+// per T-thread group; lds holds 6*C*strip_len(N).
 template <int N, int C, class Sync, int NB = 6 * C * (N / 16)>
 __device__ __forceinline__ void cols_lds_strip_f2(const float4* __restrict__ h0, float2* __restrict__ z1a,
                                                   float2* __restrict__ z2a, float2* __restrict__ z1b,
                                                   float2* __restrict__ z2b, const float2* __restrict__ tw,
                                                   const SpecConst& sc, double ta, double tb, int strip, float2* lds)
 {
-    constexpr int E = 16, T = N / E, NT = 6 * C * T, LEN = lds_len(N), CL = 2 * C;
+    constexpr int E = 16, T = N / E, NT = 6 * C * T, LEN = strip_len(N), CL = 2 * C;
     constexpr int ITERS = (C * N + NB - 1) / NB;
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");

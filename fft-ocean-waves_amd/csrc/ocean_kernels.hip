@@ -184,7 +184,7 @@ __global__ __launch_bounds__(W * 3 * C * (N / 16)) void k_cols_lds(const float4*
     constexpr int T = N / 16;
     constexpr int G = strip_group<N, C>();
     static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
-    __shared__ float2 lds[3 * C * lds_len(N)];
+    __shared__ float2 lds[3 * C * strip_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
     const int rest  = L / G;
     const int f     = rest % fb.nframes;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(6 * C * (N / 16)) void k_cols_lds_f2(const float4* 
     constexpr int T = N / 16, NB = 6 * C * T;
     constexpr int G = strip_group<N, C>();
     static_assert(T <= 64 && 64 % T == 0, "each transform group must lie inside one wave");
-    __shared__ float2 lds[6 * C * lds_len(N)];
+    __shared__ float2 lds[6 * C * strip_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
     const int pairs = (fb.nframes + 1) / 2;
     const int rest  = L / G;
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(3 * C * (N / 16)) void k_cols_ldsb(const float4* __
 {
     constexpr int G = strip_group<N, C>();
     static_assert(N / 16 > 64 && (N / 16) % 64 == 0, "transform groups of whole waves");
-    __shared__ float2 lds[3 * C * lds_len(N)];
+    __shared__ float2 lds[3 * C * strip_len(N)];
     const int L     = xcd_logical(blockIdx.x, gridDim.x);
     const int rest  = L / G;
     const int f     = rest % fb.nframes;
@@ -684,9 +684,9 @@ constexpr bool cols_fits()
         return N >= 2 * E && C * (N / E) <= 1024 && C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
     else if constexpr (K == 1 || K == 3 || K == 6)
         return E == 16 && N / 16 <= 64 && (K == 1 ? 1 : 2) * 3 * C * (N / 16) <= 1024 &&
-               (K == 6 ? 2 : 1) * 3 * C * lds_len(N) * 8 <= 160 * 1024 && C <= N / 2;
+               (K == 6 ? 2 : 1) * 3 * C * strip_len(N) * 8 <= 160 * 1024 && C <= N / 2;
     else if constexpr (K == 4)
-        return E == 16 && N / 16 > 64 && 3 * C * (N / 16) <= 1024 && 3 * C * lds_len(N) * 8 <= 160 * 1024;
+        return E == 16 && N / 16 > 64 && 3 * C * (N / 16) <= 1024 && 3 * C * strip_len(N) * 8 <= 160 * 1024;
     else if constexpr (K == 5)
         return E == 16 && N / 16 > 64 && 2 * C * (N / 16) <= 1024 && 2 * C * lds_len(N) * 8 <= 160 * 1024;
     else
