@@ -162,6 +162,19 @@ __device__ __forceinline__ float2 choppy(float m, float2 h) { return make_float2
 // p = h0 at t, q = h0 at t'; (nx, ny) and (qx, qy) are their centred indices.
 // pair_fold_at takes the phase fr (turns) and the choppy factor kinv of the
 // pair, so that frames of one texel pair share the dispersion evaluation.
+// pair_fold_sc takes its sin and cos, for pairs that also share the phase.
+__device__ __forceinline__ void pair_fold_sc(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
+                                             float kinv, float s, float c, float2& ay, float2& ax, float2& az)
+{
+    const float2 H = evolve_amp(sc, p, s, c);
+    const float2 G = evolve_amp(sc, q, s, c);
+    const float mx = (float)nx * kinv, mz = (float)ny * kinv;
+    const float gx = (float)qx * kinv, gz = (float)qy * kinv;
+    ay = make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
+    ax = make_float2(0.5f * (mx * H.y + gx * G.y), 0.5f * (gx * G.x - mx * H.x));
+    az = make_float2(0.5f * (mz * H.y + gz * G.y), 0.5f * (gz * G.x - mz * H.x));
+}
+
 __device__ __forceinline__ void pair_fold_at(const SpecConst& sc, float4 p, float4 q, int nx, int ny, int qx, int qy,
                                              float kinv, float fr, float2& ay, float2& ax, float2& az)
 {
@@ -392,7 +405,8 @@ __device__ __forceinline__ void strip_store(float2* __restrict__ z1, float2* __r
 // tx = strip*C + c, block of 3*C*T threads (T = N/16), frame time t_rev;
 // z1/z2 point at the frame's intermediate, lds holds 3*C*strip_len(N).
 // Phase A: every thread evaluates Hermitian parts (consecutive threads take
-// consecutive columns, then rows) into per-(field, column) LDS buffers.
+// consecutive columns, then rows ty with their partner rows N - ty, whose
+// texel pairs share |k| and one phase) into per-(field, column) LDS buffers.
 // Phase B: each of the 3C transforms runs inside one T-thread group (Sync:
 // WaveSync when a group lies in one wave, else BlockSync) and leaves its
 // result in natural order in the same buffer.  Phase C: every thread stores
@@ -418,24 +432,35 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
     // single frame, whose few blocks leave the SIMDs short of waves, or the
     // batch's block shared by every N) splits the folds and stores finer.
     constexpr int E = 16, T = N / E, NT = 3 * C * T, LEN = strip_len(N);
-    constexpr int ITERS = (C * N + NB - 1) / NB;
+    constexpr int ITERS = (C * (N / 2 + 1) + NB - 1) / NB;  // row-pair tasks per thread
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
     static_assert((N / 2) % C == 0, "strips of C columns tile the half plane");
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NB;
-        if (k < C * N) {
-            const int c   = k % C, ty = k / C;
+        if (k < C * (N / 2 + 1)) {
+            // rows ra and rb = -ra mod N of column c (ny -> -ny): their texel
+            // pairs share |k| and so one dispersion phase; rows 0 and N/2 are
+            // their own partners and are folded once
+            const int c   = k % C, ra = k / C;
+            const int rb  = (N - ra) & (N - 1);
             const int tx  = strip * C + c;  // < N/2
             const int mxc = (N - tx) & (N - 1);
-            const int my  = (N - ty) & (N - 1);
+            const int nx = tx - N / 2, qx = mxc - N / 2;
+            float     kinv, sn, cs;
+            sincos_turns(turns(omega_fp64(sc, nx * nx + (ra - N / 2) * (ra - N / 2), kinv), t_rev), sn, cs);
             float2 ay, ax, az;
-            pair_fold(sc, at(tx, ty), at(mxc, my), tx - N / 2, ty - N / 2, mxc - N / 2, my - N / 2, t_rev, ay, ax,
-                      az);
-            lds[(0 * C + c) * LEN + lds_pad(ty)] = ay;
-            lds[(1 * C + c) * LEN + lds_pad(ty)] = ax;
-            lds[(2 * C + c) * LEN + lds_pad(ty)] = az;
+            pair_fold_sc(sc, at(tx, ra), at(mxc, rb), nx, ra - N / 2, qx, rb - N / 2, kinv, sn, cs, ay, ax, az);
+            lds[(0 * C + c) * LEN + lds_pad(ra)] = ay;
+            lds[(1 * C + c) * LEN + lds_pad(ra)] = ax;
+            lds[(2 * C + c) * LEN + lds_pad(ra)] = az;
+            if (rb != ra) {
+                pair_fold_sc(sc, at(tx, rb), at(mxc, ra), nx, rb - N / 2, qx, ra - N / 2, kinv, sn, cs, ay, ax, az);
+                lds[(0 * C + c) * LEN + lds_pad(rb)] = ay;
+                lds[(1 * C + c) * LEN + lds_pad(rb)] = ax;
+                lds[(2 * C + c) * LEN + lds_pad(rb)] = az;
+            }
         }
     }
     if (strip == 0) strip_nyquist<N, C, NB, HT>(h0, sc, t_rev, lds);
