@@ -295,11 +295,13 @@ __host__ __device__ constexpr int lds_strips()
 
 // Stride (float2) of the per-(field, column) LDS buffers of cols_lds_strip:
 // lds_len(N) plus a few elements.  The fold writes of 16 lanes (4 columns by
-// 4 consecutive rows) and the store reads then fall on distinct banks more
-// often: with a stride of 1 mod 32 the 32 banks of a 16-lane b64 access saw
-// up to 4 lanes per bank in the fold writes.  A bank model of the strip's
-// accesses (fold writes, transforms, store reads) gives 7424 instead of 8960
-// LDS cycles per block at N = 1024 and C = 4, and no width C in 1..8 more.
+// 4 consecutive rows, then their partner rows N - ty) and the store reads
+// then fall on distinct banks more often: with a stride of 1 mod 32 the 32
+// banks of a 16-lane b64 access saw up to 4 lanes per bank in the fold
+// writes.  A bank model of the strip's accesses gives 1536 instead of 2979
+// fold-write cycles per block at N = 1024 and C = 4 (7424 instead of 8960
+// over folds, transforms and store reads with one fold per texel pair), and
+// no width C in 1..8 more.
 __host__ __device__ constexpr int strip_len(int N)
 {
     return N == 1024 ? 1091 : N == 512 ? 547 : N == 256 ? 275 : N == 128 ? 152 : N == 64 ? 76 : lds_len(N);
