@@ -44,11 +44,14 @@ def _frames(hip, oc, times, noalloc):
             hip.free(v)
 
 
-@pytest.mark.parametrize("N,F", [(1024, 6), (256, 3)], ids=["n1024_6_frames", "n256_3_frames"])
+@pytest.mark.parametrize("N,F", [(1024, 6), (512, 8), (256, 3)],
+                         ids=["n1024_6_frames", "n512_8_frames", "n256_3_frames"])
 def test_noalloc_frames_equal_update_frames(hip, N, F):
-    """Two calls in a row (N=1024 with 6 frames: launches of 4 + 2 frames in
-    the static array; then N=256 with 3 frames), each bit-equal to
-    ocn_update_frames of the same handle, schedule pinned for both paths."""
+    """Calls in a row, each on a smaller input (N=1024 with 6 frames: launches
+    of 4 + 2 frames in the static array; N=512 with 8 frames, whose schedule
+    runs the column strips over pairs of frames; then N=256 with 3 frames),
+    each bit-equal to ocn_update_frames of the same handle, schedule pinned
+    for both paths."""
     print("noalloc N=%d frames=%d" % (N, F), flush=True)
     sys.stdout.flush()
     kw = dict(c2_kwargs(), N=N, L=2.0 * N)
@@ -56,7 +59,7 @@ def test_noalloc_frames_equal_update_frames(hip, N, F):
     with pyocean.Ocean(pyocean.make_params(**kw)) as oc:
         oc.reserve_frames(32)
         oc.tilde_h0_k()
-        sch = oc.schedule(min(F, 4))
+        sch = oc.schedule(min(F, 8))  # N=1024: the 4- and 6-frame schedules agree
         oc.set_schedule(sch["cols_variant"], sch["rows_variant"])
         ref = _frames(hip, oc, times, noalloc=False)
         got = _frames(hip, oc, times, noalloc=True)
