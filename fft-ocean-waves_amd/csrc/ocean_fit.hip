@@ -337,15 +337,18 @@ struct ParamGeo {
     bool  live;
 };
 
-__device__ __forceinline__ ParamGeo param_geo(const SpecConst& sc, float iLp2, int N, int tx, int ty)
+// sqA = sqrt(A): c = sqrt(A exp(x) / |k|^4) / sqrt(2) is evaluated as
+// sqA exp(x/2) / |k|^2 / sqrt(2), with 1/|k| from one reciprocal square root
+// of the clamped |k|^2 (two transcendentals per texel instead of four).
+__device__ __forceinline__ ParamGeo param_geo(const SpecConst& sc, float iLp2, float sqA, int N, int tx, int ty)
 {
     const float kx = sc.kscale * (float)(tx - N / 2), ky = sc.kscale * (float)(ty - N / 2);
-    const float kmag = fmaxf(sqrtf(kx * kx + ky * ky), 0.00001f);
-    const float rk   = 1.0f / kmag;
+    const float k2   = fmaxf(kx * kx + ky * ky, 1e-10f);  // |k| clamped at 1e-5
+    const float rk   = __builtin_amdgcn_rsqf(k2);
     const float rk2  = rk * rk;
     const float d    = (kx * sc.wx + ky * sc.wy) * rk;
     const float dp   = (ky * sc.wx - kx * sc.wy) * rk;
-    const float c    = sqrtf(sc.A * expf(-rk2 * iLp2 - kmag * kmag * sc.l2) * (rk2 * rk2)) * 0.70710678118654752f;
+    const float c    = sqA * expf(-0.5f * (rk2 * iLp2 + k2 * sc.l2)) * rk2 * 0.70710678118654752f;
     ParamGeo    g;
     g.s     = c * fabsf(d);
     g.rk2   = rk2;
@@ -471,6 +474,7 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     const float  dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
     const float  da_scale = (float)(0.5 / A);
     const float  iLp2     = 1.0f / sc.Lp2;
+    const float  sqA      = (float)sqrt(A);
     float dV = 0.0f, dT = 0.0f, dA = 0.0f;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
@@ -488,14 +492,14 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
             if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
         }
         const bool     paired = !self && ty != 0;
-        const ParamGeo gu     = param_geo(sc, iLp2, N, tx, ty);
+        const ParamGeo gu     = param_geo(sc, iLp2, sqA, N, tx, ty);
         if (gu.live) {
             float dl = dl_ds(ga[it], gb[it], m.xi[(size_t)ty * N + tx]);
             if (paired) dl += dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]);
             param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
         }
         if (self || paired) continue;
-        const ParamGeo gm = param_geo(sc, iLp2, N, mx, my);
+        const ParamGeo gm = param_geo(sc, iLp2, sqA, N, mx, my);
         if (gm.live) param_add(gm, dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]), dv_scale, da_scale, dV, dT, dA);
     }
     dV = block_sum(dV, red);
