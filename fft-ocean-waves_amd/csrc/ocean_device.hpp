@@ -474,8 +474,9 @@ __device__ __forceinline__ void cols_lds_strip(const float4* __restrict__ h0, fl
 
 // cols_lds_strip for two frames (times ta, tb) of the same C columns, both
 // in LDS at once as 2C buffers per field (frame a's columns, then frame
-// b's): each texel pair's h0 values are loaded and its dispersion evaluated
-// once for both frames.  Block = 6*C*T threads (NB), the 6C transforms one
+// b's): each texel pair's h0 values are loaded once for both frames, and the
+// dispersion is evaluated once for the two row pairs (ty, N - ty) and both
+// frames.  Block = 6*C*T threads (NB), the 6C transforms one
 // per T-thread group; lds holds 6*C*strip_len(N).
 template <int N, int C, class Sync, int NB = 6 * C * (N / 16)>
 __device__ __forceinline__ void cols_lds_strip_f2(const float4* __restrict__ h0, float2* __restrict__ z1a,
@@ -484,31 +485,43 @@ __device__ __forceinline__ void cols_lds_strip_f2(const float4* __restrict__ h0,
                                                   const SpecConst& sc, double ta, double tb, int strip, float2* lds)
 {
     constexpr int E = 16, T = N / E, NT = 6 * C * T, LEN = strip_len(N), CL = 2 * C;
-    constexpr int ITERS = (C * N + NB - 1) / NB;
+    constexpr int ITERS = (C * (N / 2 + 1) + NB - 1) / NB;  // row-pair tasks per thread
     static_assert(NB >= NT, "block too small for the strip");
     static_assert(NB == NT || !std::is_same<Sync, BlockSync>::value, "block barriers need every thread active");
     static_assert((N / 2) % C == 0, "strips of C columns tile the half plane");
+    // frame f's folds of the texel pair at row y, all sharing the sin/cos of
+    // that frame: frame a to buffers 0..C-1 of each field, frame b to C..2C-1
+    auto fold2 = [&](float4 p, float4 q, int nx, int y, int qx, int qy, float kinv, float sa, float ca, float sb,
+                     float cb, int c) {
+        float2 ay, ax, az;
+        pair_fold_sc(sc, p, q, nx, y - N / 2, qx, qy, kinv, sa, ca, ay, ax, az);
+        lds[(0 * CL + c) * LEN + lds_pad(y)] = ay;
+        lds[(1 * CL + c) * LEN + lds_pad(y)] = ax;
+        lds[(2 * CL + c) * LEN + lds_pad(y)] = az;
+        pair_fold_sc(sc, p, q, nx, y - N / 2, qx, qy, kinv, sb, cb, ay, ax, az);
+        lds[(0 * CL + C + c) * LEN + lds_pad(y)] = ay;
+        lds[(1 * CL + C + c) * LEN + lds_pad(y)] = ax;
+        lds[(2 * CL + C + c) * LEN + lds_pad(y)] = az;
+    };
 #pragma unroll
     for (int it = 0; it < ITERS; ++it) {
         const int k = threadIdx.x + it * NB;
-        if (k < C * N) {
-            const int c   = k % C, ty = k / C;
+        if (k < C * (N / 2 + 1)) {
+            // rows ra and rb = -ra mod N share |k|: one dispersion evaluation
+            // and one sin/cos per frame for the four texels
+            const int c   = k % C, ra = k / C;
+            const int rb  = (N - ra) & (N - 1);
             const int tx  = strip * C + c;  // < N/2
             const int mxc = (N - tx) & (N - 1);
-            const int my  = (N - ty) & (N - 1);
-            const int nx = tx - N / 2, ny = ty - N / 2, qx = mxc - N / 2, qy = my - N / 2;
-            const float4 p = h0[(size_t)ty * N + tx], q = h0[(size_t)my * N + mxc];
-            float        kinv;
-            const double w = omega_fp64(sc, nx * nx + ny * ny, kinv);
-            float2       ay, ax, az;
-            pair_fold_at(sc, p, q, nx, ny, qx, qy, kinv, turns(w, ta), ay, ax, az);
-            lds[(0 * CL + c) * LEN + lds_pad(ty)] = ay;
-            lds[(1 * CL + c) * LEN + lds_pad(ty)] = ax;
-            lds[(2 * CL + c) * LEN + lds_pad(ty)] = az;
-            pair_fold_at(sc, p, q, nx, ny, qx, qy, kinv, turns(w, tb), ay, ax, az);
-            lds[(0 * CL + C + c) * LEN + lds_pad(ty)] = ay;
-            lds[(1 * CL + C + c) * LEN + lds_pad(ty)] = ax;
-            lds[(2 * CL + C + c) * LEN + lds_pad(ty)] = az;
+            const int nx = tx - N / 2, qx = mxc - N / 2;
+            float        kinv, sa, ca, sb, cb;
+            const double w = omega_fp64(sc, nx * nx + (ra - N / 2) * (ra - N / 2), kinv);
+            sincos_turns(turns(w, ta), sa, ca);
+            sincos_turns(turns(w, tb), sb, cb);
+            fold2(h0[(size_t)ra * N + tx], h0[(size_t)rb * N + mxc], nx, ra, qx, rb - N / 2, kinv, sa, ca, sb, cb, c);
+            if (rb != ra)
+                fold2(h0[(size_t)rb * N + tx], h0[(size_t)ra * N + mxc], nx, rb, qx, ra - N / 2, kinv, sa, ca, sb, cb,
+                      c);
         }
     }
     if (strip == 0) {
