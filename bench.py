@@ -308,7 +308,7 @@ def batched_fit_bench(hip, steps=200, n_oceans=256, adam_steps=60):
     pi: the Phillips spectrum depends on (k.w)^2, so w and -w are the same
     ocean), and the same for `steps` Adam steps alone.
 
-    Convergence is judged against each ocean's noise floor, the loss of its
+    Convergence is measured against each ocean's noise floor, the loss of its
     noisy targets at the hidden parameters: a fit whose loss reaches it
     explains the data as well as the truth does, whatever its parameters.
     Adam alone crosses the angle landscape well but creeps along the valley

@@ -6,8 +6,8 @@
  * six private methods of FFTOceanWaves (src/main.cpp:553-744) called from
  * init() / update() (main.cpp:199-255), with parameters in members
  * (main.cpp:1622-1646) and GL textures as outputs (main.cpp:1083-1099).  Each
- * entry point below names the reference code it replaces.  A binding that
- * drives the reference path through this ABI is shown in INTEGRATION.md.
+ * entry point below names the reference code it replaces; the reference's
+ * update() maps onto ocn_tilde_h0_k once and ocn_update per frame.
  *
  * Conventions
  *   - Every call returns OCN_OK (0) or a negative OCN_E* status; arguments are

@@ -1,8 +1,8 @@
 /*
  * ocean_oracle.c -- FP64 CPU restatement of the reference ocean path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline).  The product
- * library never links or calls this file.
+ * TEST INFRASTRUCTURE ONLY (the parity reference of the tests and the CPU
+ * baseline).  The product library never links or calls this file.
  *
  * What pins each function (tests/test_oracle.py):
  *   orc_reverse_bits / orc_bit_reversed_indices : brute-force string reversal, involution.

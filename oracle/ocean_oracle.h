@@ -2,8 +2,8 @@
  * ocean_oracle.h -- FP64 CPU restatement of the reference's Tessendorf ocean
  * compute path (diharaw/fft-ocean-waves, the src/shader compute shaders and src/main.cpp).
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP library is compared
- * against and the CPU baseline that bench.py times.  Nothing in the product
+ * TEST INFRASTRUCTURE ONLY.  This is the reference the HIP library's results
+ * are compared against and the CPU baseline that bench.py times.  Nothing in the product
  * (fft-ocean-waves_amd/) links, loads or calls it.
  *
  * Every function cites the reference file:line it restates.  Pins (which

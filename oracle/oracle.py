@@ -1,7 +1,7 @@
 """ctypes wrapper around the FP64 CPU oracle (oracle/liborc.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
-the checker, and by bench.py's cpu_baseline leg.  The product library never
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its frame, and by bench.py's cpu_baseline leg.  The product library never
 imports it.  Each wrapper names the reference shader/host code it restates
 (see ocean_oracle.c for the file:line citations and for what pins each one).
 """
