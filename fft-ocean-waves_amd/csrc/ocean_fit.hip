@@ -177,6 +177,7 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
     constexpr int IT = C * N / NT;  // texel pairs (t, t') per thread
     constexpr int S  = 2 * C;       // transforms per frame pair
     static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
+    static_assert((C * N / 2) % NT == 0, "whole two-row tiles per thread");
     const SpecConst sc = fit_spec(m, *m.vars);
     const size_t    nn = (size_t)N * N;
 
@@ -250,17 +251,22 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         for (int q = 0; q < 2; ++q) {
             if (q >= np) break;
             float2* zp = m.z + (size_t)(p0 + q) * nn;
+            // two-row tiles (zidx): thread k stores rows 2t, 2t+1 of column c
+            // with one 16-byte store, C columns side by side
 #pragma unroll
-            for (int it = 0; it < IT; ++it) {
+            for (int it = 0; it < IT / 2; ++it) {
                 const int k = threadIdx.x + it * NT;
-                const int c = k % C, y = k / C;
+                const int c = k % C, t = k / C;
                 const int tx = strip * C + c;
                 if (tx <= N / 2) {
                     const int    mxc = (N - tx) & (N - 1);
-                    const float2 g1  = lds[(q * S + c) * LEN + lds_pad(y)];
-                    const float2 g2  = lds[(q * S + C + c) * LEN + lds_pad(y)];
-                    zp[(size_t)y * N + tx] = make_float2(g1.x - g2.y, g1.y + g2.x);
-                    if (mxc != tx) zp[(size_t)y * N + zpos<N>(mxc)] = make_float2(g1.x + g2.y, g2.x - g1.y);
+                    const float2 a1  = lds[(q * S + c) * LEN + lds_pad(2 * t)];
+                    const float2 a2  = lds[(q * S + C + c) * LEN + lds_pad(2 * t)];
+                    const float2 b1  = lds[(q * S + c) * LEN + lds_pad(2 * t + 1)];
+                    const float2 b2  = lds[(q * S + C + c) * LEN + lds_pad(2 * t + 1)];
+                    float4*      zt  = ztile<N>(zp, t);
+                    zt[tx] = make_float4(a1.x - a2.y, a1.y + a2.x, b1.x - b2.y, b1.y + b2.x);
+                    if (mxc != tx) zt[zpos<N>(mxc)] = make_float4(a1.x + a2.y, a2.x - a1.y, b1.x + b2.y, b2.x - b1.y);
                 }
             }
         }
@@ -283,9 +289,10 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
     float2*      buf = lds + g * LEN;
 
     float2 v[E];
-    const float2* src = m.z + (size_t)p * nn + (size_t)y * N;
+    // row y of the pair's Z, in two-row tiles (zidx)
+    const float2* src = m.z + (size_t)p * nn + zidx<N>(y, 0);
 #pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[zpos_elem<N, E>(j, r)];
+    for (int r = 0; r < E; ++r) v[r] = src[2 * zpos_elem<N, E>(j, r)];
     const int    k1 = 2 * p, k2 = 2 * p + 1;
     const bool   two = k2 < m.K;
     const float* t1 = m.targets + (size_t)k1 * nn + (size_t)y * N;
@@ -312,9 +319,9 @@ __device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adj
     }
     if (adjoint) {
         fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
-        float2* dst = m.y + (size_t)p * nn + (size_t)y * N;
+        float2* dst = m.y + (size_t)p * nn + zidx<N>(y, 0);  // Y in two-row tiles like Z
 #pragma unroll
-        for (int r = 0; r < E; ++r) dst[zpos_elem<N, E>(j, r)] = make_float2(v[r].x, -v[r].y);
+        for (int r = 0; r < E; ++r) dst[2 * zpos_elem<N, E>(j, r)] = make_float2(v[r].x, -v[r].y);
     }
     lsum = block_sum(lsum, red);
     if (threadIdx.x == 0) m.loss_part[local] = lsum;
@@ -386,6 +393,7 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     constexpr int S  = 2 * C;           // transforms per frame pair: the strip's columns and their mirrors
     constexpr int IT = C * N / NT;      // texel pairs per thread
     static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
+    static_assert((C * N / 2) % NT == 0, "whole two-row tiles per thread");
     const FitVars   vars = *m.vars;
     const SpecConst sc = fit_spec(m, vars);
     const size_t    nn = (size_t)N * N;
@@ -415,16 +423,18 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
         for (int q = 0; q < 2; ++q) {
             if (q >= np) break;
             const float2* yp = m.y + (size_t)(p0 + q) * nn;
-            // load the strip's columns (slots 0..C-1) and their mirrors (slots C..2C-1), conjugated
-            constexpr int ITL = 2 * C * N / NT;
+            // load the strip's columns (slots 0..C-1) and their mirrors (slots
+            // C..2C-1), conjugated, two rows (one tile of Y) per 16-byte load
+            constexpr int ITL = C * N / NT;
 #pragma unroll
             for (int it = 0; it < ITL; ++it) {
                 const int kk = threadIdx.x + it * NT;
-                const int slot = kk % S, ty = kk / S;
+                const int slot = kk % S, t = kk / S;
                 const int txc = strip * C + slot % C;
                 const int col = slot < C ? txc : (N - txc) & (N - 1);
-                const float2 a = txc <= N / 2 ? yp[(size_t)ty * N + zpos<N>(col)] : make_float2(0.0f, 0.0f);
-                lds[(q * S + slot) * LEN + lds_pad(ty)] = make_float2(a.x, -a.y);
+                const float4 a = txc <= N / 2 ? ztile<N>(yp, t)[zpos<N>(col)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                lds[(q * S + slot) * LEN + lds_pad(2 * t)]     = make_float2(a.x, -a.y);
+                lds[(q * S + slot) * LEN + lds_pad(2 * t + 1)] = make_float2(a.z, -a.w);
             }
         }
         __syncthreads();
