@@ -166,6 +166,18 @@ __device__ __forceinline__ T block_sum(T x, T* red)
     return t;
 }
 
+// The fit's unit noise xi is stored in column strips: texel (x, y) at
+// ((p / C) N + y) C + p % C with p = zpos(x) and C = fit_cols_per_block(N), so
+// that a strip's C columns and, through zpos, its C mirror columns are
+// contiguous runs down the rows.  The Philox counter stays the texel's
+// row-major index, so the values are those of a row-major draw.
+__host__ __device__ __forceinline__ size_t fit_xi_index(int N, int x, int y)
+{
+    const int C = fit_cols_per_block(N);
+    const int p = x <= N / 2 ? x : 3 * N / 2 - x;
+    return ((size_t)(p / C) * N + y) * C + p % C;
+}
+
 // ---------------------------------------------------------------- forward columns
 
 // Column transforms run two frame pairs at a time: 2 pairs x 2C columns of T
@@ -198,7 +210,7 @@ __device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, floa
         // its own.
         const float  s  = phillips_amp(sc, txc, ty);
         const float  sm = (txc != 0 && ty != 0) ? s : phillips_amp(sc, mxc, my);
-        const float4 a = m.xi[(size_t)ty * N + txc], b = m.xi[(size_t)my * N + mxc];
+        const float4 a = m.xi[fit_xi_index(N, txc, ty)], b = m.xi[fit_xi_index(N, mxc, my)];
         hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         if constexpr (UNIFORM) {
@@ -504,13 +516,13 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
         const bool     paired = !self && ty != 0;
         const ParamGeo gu     = param_geo(sc, iLp2, sqA, N, tx, ty);
         if (gu.live) {
-            float dl = dl_ds(ga[it], gb[it], m.xi[(size_t)ty * N + tx]);
-            if (paired) dl += dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]);
+            float dl = dl_ds(ga[it], gb[it], m.xi[fit_xi_index(N, tx, ty)]);
+            if (paired) dl += dl_ds(gam[it], gbm[it], m.xi[fit_xi_index(N, mx, my)]);
             param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
         }
         if (self || paired) continue;
         const ParamGeo gm = param_geo(sc, iLp2, sqA, N, mx, my);
-        if (gm.live) param_add(gm, dl_ds(gam[it], gbm[it], m.xi[(size_t)my * N + mx]), dv_scale, da_scale, dV, dT, dA);
+        if (gm.live) param_add(gm, dl_ds(gam[it], gbm[it], m.xi[fit_xi_index(N, mx, my)]), dv_scale, da_scale, dV, dT, dA);
     }
     dV = block_sum(dV, red);
     dT = block_sum(dT, red);
@@ -542,7 +554,7 @@ __global__ __launch_bounds__(256) void k_fit_xi(float4* __restrict__ xi, int N, 
     if (tx >= N) return;
     const uint32_t idx = (uint32_t)ty * (uint32_t)N + (uint32_t)tx;
     const uint4    r   = philox4x32_10(make_uint4(idx, ocean_id, cascade_id, 0u), make_uint2(key0, key1));
-    xi[idx] = box_muller(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
+    xi[fit_xi_index(N, tx, ty)] = box_muller(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
 }
 
 __global__ __launch_bounds__(kFitColsThreads) void k_fit_cols_fwd(const FitMember* __restrict__ members,
