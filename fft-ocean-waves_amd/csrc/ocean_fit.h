@@ -12,6 +12,7 @@ constexpr int kFitMinN       = 64;
 constexpr int kFitMaxN       = 1024;
 constexpr int kFitColsWidth  = 1024;  // C * N: columns per column block times N
 constexpr int kFitColsThreads = kFitColsWidth / 4;  // 2 frame pairs x 2C column transforms of N/16 threads
+constexpr int kFitColsFewItems = 256;  // adjoint column launches below this many items: one block per CU, two waves per SIMD
 constexpr int kFitRowsThreads = 256;  // (256 / T) rows of T = N/8 threads
 constexpr int kFitReduceThreads = 256;
 

@@ -582,10 +582,15 @@ __global__ __launch_bounds__(kFitRowsThreads) void k_fit_rows(const FitMember* _
 #undef OCN_CALL
 }
 
-// Capped at three waves per SIMD (a few registers spill): 0.80 -> 0.68 ms per
-// step of a 256-ocean N=256 fit against the uncapped two waves.
-__global__ __launch_bounds__(kFitColsThreads, 3) void k_fit_cols_bwd(const FitMember* __restrict__ members,
-                                                                  const int2* __restrict__ items, int store_grad_h0)
+// WAVES = 3: capped at three waves per SIMD (a few registers spill): 0.80 ->
+// 0.68 ms per step of a 256-ocean N=256 fit against the uncapped two waves.
+// WAVES = 2 (no spills) for launches of fewer items than kFitColsFewItems,
+// whose blocks do not share CUs: single N=512 fit (129 items) forward+backward
+// 54.0 -> 51.9 us, where a 256-ocean step would lose 14%.
+template <int WAVES>
+__global__ __launch_bounds__(kFitColsThreads, WAVES) void k_fit_cols_bwd(const FitMember* __restrict__ members,
+                                                                      const int2* __restrict__ items,
+                                                                      int store_grad_h0)
 {
     __shared__ float2 lds[kColsLds];
     __shared__ float  red[kFitColsThreads / 64];
@@ -775,9 +780,14 @@ hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_
     hipLaunchKernelGGL(k_fit_rows, dim3(n_rows), dim3(kFitRowsThreads), 0, s, members, rows_items, adjoint);
     // an optimizer step needs the parameter gradient only; dL/dh0 (16 bytes
     // per texel) is written by ocn_fit_backward
-    if (adjoint)
-        hipLaunchKernelGGL(k_fit_cols_bwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
-                           mode == kFitGrad ? 1 : 0);
+    if (adjoint) {
+        if (n_cols < kFitColsFewItems)
+            hipLaunchKernelGGL(k_fit_cols_bwd<2>, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
+                               mode == kFitGrad ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_fit_cols_bwd<3>, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
+                               mode == kFitGrad ? 1 : 0);
+    }
     hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam, qn);
     return hipGetLastError();
 }
