@@ -598,6 +598,10 @@ struct ocn_fit {
     int                                   n_rows    = 0;
     char*                                 pool      = nullptr;
     std::map<int, std::array<float2*, 2>> tw;  // per N: radix-8 and radix-16 Stockham twiddles
+    // Members whose targets may have changed since their spectra T^ were
+    // made: set by ocn_fit_targets (which hands out the writable buffer) and
+    // ocn_fit_upload_targets, cleared when the next fit call remakes T^.
+    mutable std::vector<char>             targets_dirty;
 };
 
 namespace {
@@ -633,6 +637,11 @@ int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, const ocn::QnCfg& qn
 {
     if (!f) return fail(OCN_EINVAL, "fit is NULL");
     hipError_t e = hipSetDevice(f->device);
+    for (size_t i = 0; e == hipSuccess && i < f->members.size(); ++i)
+        if (f->targets_dirty[i]) {
+            e = ocn::launch_fit_targets(f->members[i], (hipStream_t)stream);
+            if (e == hipSuccess) f->targets_dirty[i] = 0;
+        }
     if (e == hipSuccess)
         e = ocn::launch_fit(f->d_members, (int)f->members.size(), f->d_cols, f->n_cols, f->d_rows, f->n_rows, mode,
                             adam, qn, (hipStream_t)stream);
@@ -1597,14 +1606,16 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     hipError_t e  = hipGetDevice(&f->device);
     if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
     // Per member, 256-byte aligned: xi 16, Z and Y 8 P each, targets 4 K, dL/dh0 16 bytes per texel, partials,
-    // and with equally spaced times the phase table (16 bytes per texel pair of the half plane).
+    // the targets' spectra T^ (8 bytes per frame and texel pair of the half plane) and with equally spaced
+    // times the phase table (16 bytes per texel pair of the half plane).
     auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
     auto   phase_bytes = [&](int N) { return uniform_times ? al(sizeof(float4) * ocn::fit_phase_entries(N)) : 0; };
     auto   bytes = [&](int N) {
         const size_t nn = (size_t)N * N;
         return al(nn * 16) + 2 * al(nn * 8 * P) + al(nn * 4 * K) + al(nn * 16) +
-               al(sizeof(float) * ocn::fit_rows_items_per_member(N, P)) +
-               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N)) + phase_bytes(N);
+               al(sizeof(float) * ocn::fit_cols_items_per_member(N)) +
+               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N)) +
+               al(sizeof(float2) * K * ocn::fit_phase_entries(N)) + phase_bytes(N);
     };
     size_t total = 0;
     for (const ocn_params& p : f->params) total += bytes(p.N);
@@ -1644,10 +1655,11 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.y         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
             m.targets   = reinterpret_cast<const float*>(q), q += al(nn * 4 * K);
             m.grad_h0   = reinterpret_cast<float4*>(q), q += al(nn * 16);
-            m.n_loss_part = ocn::fit_rows_items_per_member(p.N, P);
+            m.n_loss_part = ocn::fit_cols_items_per_member(p.N);  // one loss partial per column item
             m.n_grad_part = ocn::fit_cols_items_per_member(p.N);
             m.loss_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * m.n_loss_part);
             m.grad_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * 3 * m.n_grad_part);
+            m.that      = reinterpret_cast<float2*>(q), q += al(sizeof(float2) * K * ocn::fit_phase_entries(p.N));
             m.phase     = uniform_times ? reinterpret_cast<const float4*>(q) : nullptr, q += phase_bytes(p.N);
             m.tw8       = f->tw[p.N][0];
             m.tw16      = f->tw[p.N][1];
@@ -1663,6 +1675,7 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
         }
         f->n_cols = (int)cols.size();
         f->n_rows = (int)rows.size();
+        f->targets_dirty.assign(count, 1);
         if (!(rc = upload_vector(&f->d_members, f->members, "fit members")) &&
             !(rc = upload_vector(&f->d_cols, cols, "fit column items")) &&
             !(rc = upload_vector(&f->d_rows, rows, "fit row items"))) {
@@ -1701,6 +1714,7 @@ int ocn_fit_targets(const ocn_fit* f, int member, float** targets)
     if (rc) return rc;
     if (!targets) return fail(OCN_EINVAL, "targets is NULL");
     *targets = const_cast<float*>(f->members[member].targets);
+    f->targets_dirty[member] = 1;  // the caller may write them through this pointer
     return OCN_OK;
 }
 
@@ -1715,6 +1729,7 @@ int ocn_fit_upload_targets(ocn_fit* f, int member, const float* targets, void* s
     if (e == hipSuccess)
         e = hipMemcpyAsync(const_cast<float*>(m.targets), targets, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
     if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+    if (e == hipSuccess) f->targets_dirty[member] = 1;
     return e == hipSuccess ? OCN_OK : hip_fail(e, "hipMemcpyAsync(fit targets)");
 }
 

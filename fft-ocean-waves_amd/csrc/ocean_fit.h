@@ -71,6 +71,12 @@ struct FitMember {
     float2* z;              // P * N*N: column-transformed spectra (two frames per complex field)
     float2* y;              // P * N*N: row-transformed residuals
     const float* targets;   // K * N*N
+    // Per target frame k, the sign-shifted 2-D DFT of the targets,
+    // T^_k(n) = sum_x T_k(x) e^{-2 pi i n.x / N} over centred wave numbers n,
+    // at the half-plane texels of the column items: entry
+    // k * fit_phase_entries(N) + (strip * N + ty) * C + c (the phase table's
+    // order).  Written by launch_fit_targets whenever the targets change.
+    float2* that;
     float4* grad_h0;        // N*N: dL/dh0 as (dRe h0k, dIm h0k, dRe h0mk, dIm h0mk)
     float*  loss_part;      // one partial sum per row item
     float*  grad_part;      // three partial sums per backward column item
@@ -90,6 +96,8 @@ inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_
 enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2, kFitQn = 3 };
 
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
+// T^ of every target frame of member m (uses m.z as scratch).
+hipError_t launch_fit_targets(const FitMember& m, hipStream_t s);
 // Fills m.phase (a member with equally spaced times) from its constants.
 hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
 // cols_items: int2 (member, strip); rows_items: int2 (member, local row item)

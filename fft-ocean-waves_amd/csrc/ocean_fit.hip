@@ -534,6 +534,146 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     }
 }
 
+// ---------------------------------------------------------------- the step in the spectral domain
+
+// Loss and gradient of one column item without transforms.  h_k is real with
+// (-1)^(x+y)/N^2 times the inverse DFT of its Hermitian-folded spectrum A_k
+// over the texel grid, i.e. h_k = IDFT(A_k)/N^2 over centred wave numbers n.
+// With T^_k the DFT of the targets over the same n (Parseval):
+//   sum_x (h_k - T_k)^2 = sum_n |A_k(n) - T^_k(n)|^2 / N^2,
+// so each texel pair (u, -u) adds 2 |D|^2 / N^2 with D = A_k(u) - T^_k(u)
+// (|D(-u)| = |D(u)|: both A_k and T^_k are Hermitian), or |D|^2 / N^2 for the
+// columns 0 and N/2, whose rows hold u and -u as texels of their own.  The
+// gradient of L = sum / (K N^2) with respect to the spectrum H(u) of the
+// texel is 2 D / (K N^4), and conj of it for -u: the pull-back of the
+// adjoint column pass (pull_back_pair) and its parameter terms follow as
+// there.  Loss partials hold sum_x (h_k - T_k)^2 like the row pass's.
+template <int N, bool UNIFORM>
+__device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int mode, float* red)
+{
+    constexpr int C = fit_cols_per_block(N), NT = kFitColsThreads, IT = C * N / NT;
+    static_assert((C * N) % NT == 0, "whole texel pairs per thread");
+    const FitVars   vars = *m.vars;
+    const SpecConst sc   = fit_spec(m, vars);
+    const bool      grad = mode != kFitLoss;
+    const bool      store_grad_h0 = mode == kFitGrad;
+    const size_t    ne   = fit_phase_entries(N);
+    const size_t    base = (size_t)strip * (N * C);
+    const float     inv_nn = 1.0f / ((float)N * (float)N);
+    const float     gsc  = 2.0f * m.inv_knn * inv_nn;
+    const double    V = vars.V, A = vars.A;
+    const float     dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
+    const float     da_scale = (float)(0.5 / A);
+    const float     iLp2     = 1.0f / sc.Lp2;
+    const float     sqA      = (float)sqrt(A);
+    float lsum = 0.0f, dV = 0.0f, dT = 0.0f, dA = 0.0f;
+    for (int it = 0; it < IT; ++it) {
+        const int k  = threadIdx.x + it * NT;
+        const int c  = k % C, ty = k / C;
+        const int tx = strip * C + c;
+        if (tx > N / 2) continue;  // past the half plane (the last strip)
+        const int  mx = (N - tx) & (N - 1), my = (N - ty) & (N - 1);
+        const bool self = tx == 0 || tx == N / 2;
+        // the amplitude is even in k; the mirror's differs on the Nyquist row or column only
+        const float  s  = phillips_amp(sc, tx, ty);
+        const float  sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
+        const float4 a = m.xi[fit_xi_index(N, tx, ty)], b = m.xi[fit_xi_index(N, mx, my)];
+        const float4 hp = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
+        const float4 hq = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
+        PhaseWalk walk;
+        double    w = 0.0;
+        if constexpr (UNIFORM) {
+            walk.load(m.phase[base + k]);
+        } else {
+            const int nx = tx - N / 2, ny = ty - N / 2;
+            float     kinv;
+            w = omega_fp64(sc, nx * nx + ny * ny, kinv);
+        }
+        const float wt = self ? inv_nn : 2.0f * inv_nn;
+        float2 ga{}, gb{}, gam{}, gbm{};
+        for (int f = 0; f < m.K; ++f) {
+            const float2 e  = UNIFORM ? walk.next() : phase_at(w, m.t_rev[f]);
+            const float2 Af = fold_height(sc, hp, hq, e);
+            const float2 T  = m.that[(size_t)f * ne + base + k];
+            const float2 D  = make_float2(Af.x - T.x, Af.y - T.y);
+            lsum = fmaf(wt, fmaf(D.x, D.x, D.y * D.y), lsum);
+            if (grad) pull_back_pair(sc, make_float2(gsc * D.x, gsc * D.y), e, ga, gb, gam, gbm);
+        }
+        if (!grad) continue;
+        if (store_grad_h0) {
+            m.grad_h0[(size_t)ty * N + tx] = make_float4(ga.x, ga.y, gb.x, gb.y);
+            if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam.x, gam.y, gbm.x, gbm.y);
+        }
+        // -u lies in the same column for tx = 0, N/2 (a u of its own); off the
+        // Nyquist row its centred index is u's negated and shares u's factors
+        const bool     paired = !self && ty != 0;
+        const ParamGeo gu     = param_geo(sc, iLp2, sqA, N, tx, ty);
+        if (gu.live) {
+            float dl = dl_ds(ga, gb, a);
+            if (paired) dl += dl_ds(gam, gbm, b);
+            param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
+        }
+        if (self || paired) continue;
+        const ParamGeo gm = param_geo(sc, iLp2, sqA, N, mx, my);
+        if (gm.live) param_add(gm, dl_ds(gam, gbm, b), dv_scale, da_scale, dV, dT, dA);
+    }
+    lsum = block_sum(lsum, red);
+    if (threadIdx.x == 0) m.loss_part[strip] = lsum;
+    if (grad) {
+        dV = block_sum(dV, red);
+        dT = block_sum(dT, red);
+        dA = block_sum(dA, red);
+        if (threadIdx.x == 0) {
+            m.grad_part[3 * strip + 0] = dV;
+            m.grad_part[3 * strip + 1] = dT;
+            m.grad_part[3 * strip + 2] = dA;
+        }
+    }
+}
+
+// T^ of one target frame, rows: R(t, y) = sum_x (-1)^(x+y) T(x, y) e^{-2 pi i t x / N}
+// (the conjugate of the inverse transform of the real input), one row per
+// block of N/8 threads; R in the member's Z buffer.
+template <int N>
+__device__ __forceinline__ void fit_targets_row(const float* __restrict__ tgt, float2* __restrict__ R,
+                                                const float2* __restrict__ tw8, float2* lds)
+{
+    constexpr int E = 8, T = N / E;
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    const int y = blockIdx.x, j = threadIdx.x;
+    float2    v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int x = j + r * T;
+        const float t = tgt[(size_t)y * N + x];
+        v[r] = make_float2(((x + y) & 1) ? -t : t, 0.0f);
+    }
+    fft_inverse<N, E>(v, j, lds, tw8, Sync{});
+#pragma unroll
+    for (int r = 0; r < E; ++r) R[(size_t)y * N + j + r * T] = make_float2(v[r].x, -v[r].y);
+}
+
+// ... and columns t = 0..N/2 of R: T^(t, s) = sum_y R(t, y) e^{-2 pi i s y / N},
+// stored at the half-plane entries (strip t / C, c = t % C, row s).
+template <int N>
+__device__ __forceinline__ void fit_targets_col(const float2* __restrict__ R, float2* __restrict__ that,
+                                                const float2* __restrict__ tw8, float2* lds)
+{
+    constexpr int E = 8, T = N / E, C = fit_cols_per_block(N);
+    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
+    const int t = blockIdx.x, j = threadIdx.x;  // t <= N/2
+    float2    v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const float2 q = R[(size_t)(j + r * T) * N + t];
+        v[r] = make_float2(q.x, -q.y);
+    }
+    fft_inverse<N, E>(v, j, lds, tw8, Sync{});
+    const size_t col = (size_t)(t / C) * (N * C) + t % C;
+#pragma unroll
+    for (int r = 0; r < E; ++r) that[col + (size_t)(j + r * T) * C] = make_float2(v[r].x, -v[r].y);
+}
+
 }  // namespace
 
 #define OCN_FIT_SWITCH(N_, CALL)                 \
@@ -545,6 +685,35 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
     case 1024: CALL(1024); break;                \
     default: break;                              \
     }
+
+__global__ __launch_bounds__(kFitColsThreads) void k_fit_spectral(const FitMember* __restrict__ members,
+                                                                  const int2* __restrict__ items, int mode)
+{
+    __shared__ float red[kFitColsThreads / 64];
+    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
+    const FitMember& m  = members[it.x];
+#define OCN_CALL(n) \
+    if (m.uniform) fit_spectral<n, true>(m, it.y, mode, red); \
+    else fit_spectral<n, false>(m, it.y, mode, red)
+    OCN_FIT_SWITCH(m.N, OCN_CALL)
+#undef OCN_CALL
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_fit_targets_rows(const float* __restrict__ tgt, float2* __restrict__ R,
+                                                            const float2* __restrict__ tw8)
+{
+    __shared__ float2 lds[lds_len(N)];
+    fit_targets_row<N>(tgt, R, tw8, lds);
+}
+
+template <int N>
+__global__ __launch_bounds__(N / 8) void k_fit_targets_cols(const float2* __restrict__ R, float2* __restrict__ that,
+                                                            const float2* __restrict__ tw8)
+{
+    __shared__ float2 lds[lds_len(N)];
+    fit_targets_col<N>(R, that, tw8, lds);
+}
 
 __global__ __launch_bounds__(256) void k_fit_xi(float4* __restrict__ xi, int N, uint32_t key0, uint32_t key1,
                                                 uint32_t ocean_id, uint32_t cascade_id)
@@ -762,6 +931,25 @@ hipError_t launch_fit_phase(const FitMember& m, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int N>
+static hipError_t launch_fit_targets_n(const FitMember& m, hipStream_t s)
+{
+    const size_t nn = (size_t)N * N, ne = fit_phase_entries(N);
+    for (int f = 0; f < m.K; ++f) {
+        hipLaunchKernelGGL(k_fit_targets_rows<N>, dim3(N), dim3(N / 8), 0, s, m.targets + f * nn, m.z, m.tw8);
+        hipLaunchKernelGGL(k_fit_targets_cols<N>, dim3(N / 2 + 1), dim3(N / 8), 0, s, m.z, m.that + f * ne, m.tw8);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fit_targets(const FitMember& m, hipStream_t s)
+{
+#define OCN_CALL(n) return launch_fit_targets_n<n>(m, s)
+    OCN_FIT_SWITCH(m.N, OCN_CALL)
+#undef OCN_CALL
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s)
 {
     dim3 grid((m.N + 255) / 256, m.N);
@@ -775,19 +963,8 @@ hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_
                       hipStream_t s)
 {
     if (n_members < 1 || n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
-    const int adjoint = mode != kFitLoss;
-    hipLaunchKernelGGL(k_fit_cols_fwd, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items);
-    hipLaunchKernelGGL(k_fit_rows, dim3(n_rows), dim3(kFitRowsThreads), 0, s, members, rows_items, adjoint);
-    // an optimizer step needs the parameter gradient only; dL/dh0 (16 bytes
-    // per texel) is written by ocn_fit_backward
-    if (adjoint) {
-        if (n_cols < kFitColsFewItems)
-            hipLaunchKernelGGL(k_fit_cols_bwd<2>, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
-                               mode == kFitGrad ? 1 : 0);
-        else
-            hipLaunchKernelGGL(k_fit_cols_bwd<3>, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items,
-                               mode == kFitGrad ? 1 : 0);
-    }
+    // one launch over the column items: loss and gradient in the spectral domain
+    hipLaunchKernelGGL(k_fit_spectral, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items, mode);
     hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam, qn);
     return hipGetLastError();
 }
