@@ -593,9 +593,7 @@ struct ocn_fit {
     ocn::FitMember*                       d_members = nullptr;
     ocn::FitVars*                         d_vars    = nullptr;
     int2*                                 d_cols    = nullptr;
-    int2*                                 d_rows    = nullptr;
     int                                   n_cols    = 0;
-    int                                   n_rows    = 0;
     char*                                 pool      = nullptr;
     std::map<int, std::array<float2*, 2>> tw;  // per N: radix-8 and radix-16 Stockham twiddles
     // Members whose targets may have changed since their spectra T^ were
@@ -608,7 +606,7 @@ namespace {
 
 void fit_free(ocn_fit* f)
 {
-    void* ptrs[] = {f->d_members, f->d_vars, f->d_cols, f->d_rows, f->pool};
+    void* ptrs[] = {f->d_members, f->d_vars, f->d_cols, f->pool};
     for (void* q : ptrs)
         if (q) (void)hipFree(q);
     for (auto& kv : f->tw)
@@ -643,8 +641,8 @@ int fit_run(ocn_fit* f, int mode, const ocn::AdamCfg& adam, const ocn::QnCfg& qn
             if (e == hipSuccess) f->targets_dirty[i] = 0;
         }
     if (e == hipSuccess)
-        e = ocn::launch_fit(f->d_members, (int)f->members.size(), f->d_cols, f->n_cols, f->d_rows, f->n_rows, mode,
-                            adam, qn, (hipStream_t)stream);
+        e = ocn::launch_fit(f->d_members, (int)f->members.size(), f->d_cols, f->n_cols, mode, adam, qn,
+                            (hipStream_t)stream);
     return e == hipSuccess ? OCN_OK : hip_fail(e, "fit launch");
 }
 
@@ -1601,18 +1599,17 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     if (!f) return fail(OCN_ENOMEM, "host allocation of the fit failed");
     f->K = K;
     f->params.assign(params, params + count);
-    const int  P  = (K + 1) / 2;
     int        rc = OCN_OK;
     hipError_t e  = hipGetDevice(&f->device);
     if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
-    // Per member, 256-byte aligned: xi 16, Z and Y 8 P each, targets 4 K, dL/dh0 16 bytes per texel, partials,
+    // Per member, 256-byte aligned: xi 16, scratch 8, targets 4 K, dL/dh0 16 bytes per texel, partials,
     // the targets' spectra T^ (8 bytes per frame and texel pair of the half plane) and with equally spaced
     // times the phase table (16 bytes per texel pair of the half plane).
     auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
     auto   phase_bytes = [&](int N) { return uniform_times ? al(sizeof(float4) * ocn::fit_phase_entries(N)) : 0; };
     auto   bytes = [&](int N) {
         const size_t nn = (size_t)N * N;
-        return al(nn * 16) + 2 * al(nn * 8 * P) + al(nn * 4 * K) + al(nn * 16) +
+        return al(nn * 16) + al(nn * 8) + al(nn * 4 * K) + al(nn * 16) +
                al(sizeof(float) * ocn::fit_cols_items_per_member(N)) +
                al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N)) +
                al(sizeof(float2) * K * ocn::fit_phase_entries(N)) + phase_bytes(N);
@@ -1631,14 +1628,13 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     if (!rc) {
         std::vector<ocn::FitVars> vars;
         char*                     q = f->pool;
-        std::vector<int2>         cols, rows;
+        std::vector<int2>         cols;
         for (int i = 0; i < count; ++i) {
             const ocn_params& p  = f->params[i];
             const size_t      nn = (size_t)p.N * p.N;
             ocn::FitMember    m{};
             m.N         = p.N;
             m.K         = K;
-            m.P         = P;
             m.conj_mode = p.conj_mode;
             m.kscale    = (float)(2.0 * kPi / p.L);
             m.l2        = (float)(p.suppression * p.suppression);
@@ -1651,8 +1647,7 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.uniform = uniform_times ? 1 : 0;
             m.dt_rev  = uniform_dt / (2.0 * kPi);
             m.xi        = reinterpret_cast<const float4*>(q), q += al(nn * 16);
-            m.z         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
-            m.y         = reinterpret_cast<float2*>(q), q += al(nn * 8 * P);
+            m.scratch   = reinterpret_cast<float2*>(q), q += al(nn * 8);
             m.targets   = reinterpret_cast<const float*>(q), q += al(nn * 4 * K);
             m.grad_h0   = reinterpret_cast<float4*>(q), q += al(nn * 16);
             m.n_loss_part = ocn::fit_cols_items_per_member(p.N);  // one loss partial per column item
@@ -1671,14 +1666,11 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             f->members.push_back(m);
             vars.push_back(initial_vars(p));
             for (int k = 0; k < m.n_grad_part; ++k) cols.push_back(make_int2(i, k));
-            for (int k = 0; k < m.n_loss_part; ++k) rows.push_back(make_int2(i, k));
         }
         f->n_cols = (int)cols.size();
-        f->n_rows = (int)rows.size();
         f->targets_dirty.assign(count, 1);
         if (!(rc = upload_vector(&f->d_members, f->members, "fit members")) &&
-            !(rc = upload_vector(&f->d_cols, cols, "fit column items")) &&
-            !(rc = upload_vector(&f->d_rows, rows, "fit row items"))) {
+            !(rc = upload_vector(&f->d_cols, cols, "fit column items"))) {
             e = hipMemcpy(f->d_vars, vars.data(), sizeof(ocn::FitVars) * vars.size(), hipMemcpyHostToDevice);
             if (e != hipSuccess) rc = hip_fail(e, "hipMemcpy(fit variables)");
         }

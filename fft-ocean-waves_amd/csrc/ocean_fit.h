@@ -11,14 +11,12 @@ constexpr int kFitMaxK       = 16;    // frames (times) per fit
 constexpr int kFitMinN       = 64;
 constexpr int kFitMaxN       = 1024;
 constexpr int kFitColsWidth  = 1024;  // C * N: columns per column block times N
-constexpr int kFitColsThreads = kFitColsWidth / 4;  // 2 frame pairs x 2C column transforms of N/16 threads
-constexpr int kFitColsFewItems = 256;  // adjoint column launches below this many items: one block per CU, two waves per SIMD
-constexpr int kFitRowsThreads = 256;  // (256 / T) rows of T = N/8 threads
+constexpr int kFitColsThreads = kFitColsWidth / 4;  // threads of a column item: 4 texel pairs each
 constexpr int kFitReduceThreads = 256;
 
 // Fitted variables of one member, in device memory: written by the host
 // (set) and by the reduction kernel (loss, gradient, Adam state), read by the
-// column kernels of the next fit step.
+// step kernel of the next fit step.
 struct FitVars {
     double V, theta, A;    // wind speed, wind angle (direction (cos, sin)), Phillips amplitude
     double m[3], v[3];     // Adam moments of (V, theta, A)
@@ -48,7 +46,7 @@ struct QnCfg {
 
 // Device-side descriptor of one fit member (one ocean, K frames).
 struct FitMember {
-    int     N, K, P;        // P = ceil(K/2) frame pairs
+    int     N, K;
     int     conj_mode;
     float   kscale;         // 2 pi / L
     float   l2;             // suppression^2
@@ -68,8 +66,7 @@ struct FitMember {
     const float4* phase;
     const float2* tw8;      // Stockham twiddles (N, 8) and (N, 16)
     const float2* tw16;
-    float2* z;              // P * N*N: column-transformed spectra (two frames per complex field)
-    float2* y;              // P * N*N: row-transformed residuals
+    float2* scratch;        // N*N: the row transforms of a target frame (launch_fit_targets)
     const float* targets;   // K * N*N
     // Per target frame k, the sign-shifted 2-D DFT of the targets,
     // T^_k(n) = sum_x T_k(x) e^{-2 pi i n.x / N} over centred wave numbers n,
@@ -78,8 +75,8 @@ struct FitMember {
     // order).  Written by launch_fit_targets whenever the targets change.
     float2* that;
     float4* grad_h0;        // N*N: dL/dh0 as (dRe h0k, dIm h0k, dRe h0mk, dIm h0mk)
-    float*  loss_part;      // one partial sum per row item
-    float*  grad_part;      // three partial sums per backward column item
+    float*  loss_part;      // one partial sum per column item
+    float*  grad_part;      // three partial sums per column item
     int     n_loss_part, n_grad_part;
     FitVars* vars;
     uint32_t key0, key1, ocean_id, cascade_id;  // Philox noise of xi
@@ -89,20 +86,17 @@ __host__ __device__ constexpr int fit_cols_per_block(int N) { return kFitColsWid
 __host__ __device__ inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
 // float4 entries of a member's phase table
 __host__ __device__ inline size_t fit_phase_entries(int N) { return (size_t)fit_cols_items_per_member(N) * N * fit_cols_per_block(N); }
-inline int fit_rows_groups_per_block(int N) { return kFitRowsThreads / (N / 8); }
-inline int fit_rows_items_per_member(int N, int P) { return N / fit_rows_groups_per_block(N) * P; }
 
 // Modes of the fit launch sequence.
 enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2, kFitQn = 3 };
 
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
-// T^ of every target frame of member m (uses m.z as scratch).
+// T^ of every target frame of member m (m.scratch holds the row transforms).
 hipError_t launch_fit_targets(const FitMember& m, hipStream_t s);
 // Fills m.phase (a member with equally spaced times) from its constants.
 hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
-// cols_items: int2 (member, strip); rows_items: int2 (member, local row item)
-hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
-                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, const QnCfg& qn,
-                      hipStream_t s);
+// cols_items: int2 (member, strip): one block each for the step kernel
+hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols, int mode,
+                      const AdamCfg& adam, const QnCfg& qn, hipStream_t s);
 
 }  // namespace ocn

@@ -5,26 +5,25 @@
 // path (tilde_h0_k_cs.glsl, tilde_h0_t_cs.glsl, butterfly_cs.glsl,
 // inversion_cs.glsl), whose maths the forward part here repeats.
 //
-// Loss L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2.  h_k is real, so two
-// frames share one complex 2-D transform (Hermitian fold: Re IDFT(F) =
-// IDFT((F + conj F(-k))/2)), and the adjoint of h -> (-1)^(x+y) Re IDFT / N^2
-// is the forward DFT of the signed residual, again two frames per complex
-// transform.  Nothing per frame is stored between steps: h0 is recomputed
-// from the unit noise xi and the current variables in both column passes.
+// Loss L = 1/(K N^2) sum_k sum_x (h_k(x) - T_k(x))^2.  h_k is real: with the
+// inversion step's (-1)^(x+y) it is the inverse DFT, over centred wave numbers
+// n, of the Hermitian part A_k = (H_k(n) + conj H_k(-n))/2 of its spectrum,
+// divided by N^2.  By Parseval the loss is then 1/(K N^4) sum_k sum_n
+// |A_k(n) - T^_k(n)|^2 with T^_k the DFT of the targets over the same n, and
+// its gradient with respect to each texel's spectrum is pointwise.  So a fit
+// step evaluates the spectrum of every texel pair and frame, as the forward
+// path does, and needs no transform: T^ is made once per change of the
+// targets.  Nothing per frame is stored between steps: h0 is recomputed from
+// the unit noise xi and the current variables.
 //
-// One fit step, P = ceil(K/2) frame pairs:
-//   k_fit_cols_fwd  item (member, strip): h0 = s(V, theta, A) xi for the strip,
-//                   per pair the folded spectra of both frames, column IFFTs,
-//                   Z_p = G_k1 + i G_k2 (columns tx and N - tx; rows of Z and Y
-//                   in the zpos column order)
-//   k_fit_rows      item (member, rows x pair): row IFFT -> h_k1, h_k2,
-//                   residual against the targets, loss partial; in gradient
-//                   mode the forward row DFT of the signed residual pair -> Y_p
-//   k_fit_cols_bwd  item (member, strip): per pair the forward column DFT of
-//                   Y_p for columns tx and N - tx, split into the two frames'
-//                   spectra G_k(t), accumulation of dL/dh0 over frames; then
-//                   dL/ds per texel and the three parameter partials
-//   k_fit_reduce    one block per member: loss, gradient, optional Adam step
+//   k_fit_targets_rows/cols  per member and target frame, when the targets
+//                   changed: the 2-D DFT T^ at the half-plane texels
+//   k_fit_spectral  item (member, strip of C columns): h0 = s(V, theta, A) xi,
+//                   per frame the folded spectrum, its residual against T^,
+//                   the loss partial; in gradient mode the pull-back to dL/dh0
+//                   and dL/ds per texel and the three parameter partials
+//   k_fit_reduce    one block per member: loss, gradient, optional Adam or
+//                   quasi-Newton step
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -36,28 +35,6 @@
 namespace ocn {
 
 namespace {
-
-// LDS of one block: the largest need over the fitted sizes N = 64..1024
-constexpr int cols_lds_need()
-{
-    int m = 0;
-    for (int N = kFitMinN; N <= kFitMaxN; N *= 2) {
-        const int need = 4 * fit_cols_per_block(N) * lds_len(N);  // 2 pairs x 2C transforms
-        m = m > need ? m : need;
-    }
-    return m;
-}
-constexpr int rows_lds_need()
-{
-    int m = 0;
-    for (int N = kFitMinN; N <= kFitMaxN; N *= 2) {
-        const int need = kFitRowsThreads / (N / 8) * lds_len(N);
-        m = m > need ? m : need;
-    }
-    return m;
-}
-constexpr int kColsLds = cols_lds_need();
-constexpr int kRowsLds = rows_lds_need();
 
 // Spectrum constants of the member at its current variables.
 __device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars& v)
@@ -178,168 +155,7 @@ __host__ __device__ __forceinline__ size_t fit_xi_index(int N, int x, int y)
     return ((size_t)(p / C) * N + y) * C + p % C;
 }
 
-// ---------------------------------------------------------------- forward columns
-
-// Column transforms run two frame pairs at a time: 2 pairs x 2C columns of T
-// threads fill the 256-thread block.
-template <int N, bool UNIFORM>
-__device__ __forceinline__ void fit_cols_fwd(const FitMember& m, int strip, float2* lds)
-{
-    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
-    constexpr int IT = C * N / NT;  // texel pairs (t, t') per thread
-    constexpr int S  = 2 * C;       // transforms per frame pair
-    static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
-    static_assert((C * N / 2) % NT == 0, "whole two-row tiles per thread");
-    const SpecConst sc = fit_spec(m, *m.vars);
-    const size_t    nn = (size_t)N * N;
-
-    float4    hp[IT], hq[IT];
-    double    w[UNIFORM ? 1 : IT];    // per-frame phases: omega of each texel pair
-    PhaseWalk walk[UNIFORM ? IT : 1]; // equally spaced frames: the phase recurrence
-    const float4* ph = UNIFORM ? m.phase + (size_t)strip * (N * C) : nullptr;
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int k = threadIdx.x + it * NT;
-        const int c = k % C, ty = k / C;
-        const int txc = min(strip * C + c, N / 2);
-        const int mxc = (N - txc) & (N - 1), my = (N - ty) & (N - 1);
-        // The amplitude is even in k, and the mirror's centred index is the
-        // texel's negated (bit for bit the same amplitude) except on the Nyquist
-        // row or column (ty = 0 or txc = 0), where it is (nx, -N/2) or
-        // (-N/2, ny): there the wind factor d^2 differs and it is evaluated on
-        // its own.
-        const float  s  = phillips_amp(sc, txc, ty);
-        const float  sm = (txc != 0 && ty != 0) ? s : phillips_amp(sc, mxc, my);
-        const float4 a = m.xi[fit_xi_index(N, txc, ty)], b = m.xi[fit_xi_index(N, mxc, my)];
-        hp[it] = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
-        hq[it] = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
-        if constexpr (UNIFORM) {
-            walk[it].load(ph[k]);
-        } else {
-            const int nx = txc - N / 2, ny = ty - N / 2;
-            float     kinv;
-            w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);  // |k| is shared by t and t'
-        }
-    }
-    for (int p0 = 0; p0 < m.P; p0 += 2) {
-        const int np = min(2, m.P - p0);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            if (q >= np) break;
-            const int    p   = p0 + q;
-            const bool   two = 2 * p + 1 < m.K;
-            const double tr1 = m.t_rev[2 * p], tr2 = m.t_rev[two ? 2 * p + 1 : m.K - 1];
-#pragma unroll
-            for (int it = 0; it < IT; ++it) {
-                const int k = threadIdx.x + it * NT;
-                const int c = k % C, ty = k / C;
-                // frames in order 2p, 2p + 1; a lone last frame is folded twice
-                float2 e1, e2;
-                if constexpr (UNIFORM) {
-                    e1 = walk[it].next();
-                    e2 = two ? walk[it].next() : e1;
-                } else {
-                    e1 = phase_at(w[it], tr1);
-                    e2 = phase_at(w[it], tr2);
-                }
-                lds[(q * S + c) * LEN + lds_pad(ty)]     = fold_height(sc, hp[it], hq[it], e1);
-                lds[(q * S + C + c) * LEN + lds_pad(ty)] = fold_height(sc, hp[it], hq[it], e2);
-            }
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < np * S * T) {  // whole waves: np * 2C transforms of T threads
-            const int g = threadIdx.x / T, j = threadIdx.x % T;
-            float2*   buf = lds + g * LEN;
-            float2    v[E];
-#pragma unroll
-            for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
-            fft_inverse<N, E>(v, j, buf, m.tw16, WaveSync{});
-            WaveSync{}();
-#pragma unroll
-            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            if (q >= np) break;
-            float2* zp = m.z + (size_t)(p0 + q) * nn;
-            // two-row tiles (zidx): thread k stores rows 2t, 2t+1 of column c
-            // with one 16-byte store, C columns side by side
-#pragma unroll
-            for (int it = 0; it < IT / 2; ++it) {
-                const int k = threadIdx.x + it * NT;
-                const int c = k % C, t = k / C;
-                const int tx = strip * C + c;
-                if (tx <= N / 2) {
-                    const int    mxc = (N - tx) & (N - 1);
-                    const float2 a1  = lds[(q * S + c) * LEN + lds_pad(2 * t)];
-                    const float2 a2  = lds[(q * S + C + c) * LEN + lds_pad(2 * t)];
-                    const float2 b1  = lds[(q * S + c) * LEN + lds_pad(2 * t + 1)];
-                    const float2 b2  = lds[(q * S + C + c) * LEN + lds_pad(2 * t + 1)];
-                    float4*      zt  = ztile<N>(zp, t);
-                    zt[tx] = make_float4(a1.x - a2.y, a1.y + a2.x, b1.x - b2.y, b1.y + b2.x);
-                    if (mxc != tx) zt[zpos<N>(mxc)] = make_float4(a1.x + a2.y, a2.x - a1.y, b1.x + b2.y, b2.x - b1.y);
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------- rows: heights, residual, adjoint rows
-
-template <int N>
-__device__ __forceinline__ void fit_rows(const FitMember& m, int local, bool adjoint, float2* lds, float* red)
-{
-    constexpr int E = 8, T = N / E, GPB = kFitRowsThreads / T, LEN = lds_len(N);
-    static_assert(T % 2 == 0 && GPB >= 1, "row block shape");
-    using Sync = typename std::conditional<(T <= 64), WaveSync, BlockSync>::type;
-    const int    p = local % m.P, rb = local / m.P;
-    const int    g = threadIdx.x / T, j = threadIdx.x % T;
-    const int    y = rb * GPB + g;
-    const size_t nn = (size_t)N * N;
-    float2*      buf = lds + g * LEN;
-
-    float2 v[E];
-    // row y of the pair's Z, in two-row tiles (zidx)
-    const float2* src = m.z + (size_t)p * nn + zidx<N>(y, 0);
-#pragma unroll
-    for (int r = 0; r < E; ++r) v[r] = src[2 * zpos_elem<N, E>(j, r)];
-    const int    k1 = 2 * p, k2 = 2 * p + 1;
-    const bool   two = k2 < m.K;
-    const float* t1 = m.targets + (size_t)k1 * nn + (size_t)y * N;
-    const float* t2 = m.targets + (size_t)(two ? k2 : k1) * nn + (size_t)y * N;
-    float        ta[E], tb[E];  // the targets, loaded before the transform so their reads overlap it
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-        ta[r] = t1[j + r * T];
-        tb[r] = t2[j + r * T];
-    }
-    fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
-
-    // (-1)^(x+y) / N^2; x = j + r*T with T even
-    const float sg = ((j + y) & 1) ? -1.0f / ((float)N * (float)N) : 1.0f / ((float)N * (float)N);
-    // adjoint input (-1)^(x+y) r / N^2 with r = 2 (h - target) / (K N^2)
-    const float  qs = sg * 2.0f * m.inv_knn;
-    float lsum = 0.0f;
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-        const float d1 = sg * v[r].x - ta[r];
-        const float d2 = two ? sg * v[r].y - tb[r] : 0.0f;
-        lsum = fmaf(d1, d1, fmaf(d2, d2, lsum));
-        v[r] = make_float2(qs * d1, -(qs * d2));  // conj of q1 + i q2: the forward DFT as conj(IDFT(conj))
-    }
-    if (adjoint) {
-        fft_inverse<N, E>(v, j, buf, m.tw8, Sync{});
-        float2* dst = m.y + (size_t)p * nn + zidx<N>(y, 0);  // Y in two-row tiles like Z
-#pragma unroll
-        for (int r = 0; r < E; ++r) dst[2 * zpos_elem<N, E>(j, r)] = make_float2(v[r].x, -v[r].y);
-    }
-    lsum = block_sum(lsum, red);
-    if (threadIdx.x == 0) m.loss_part[local] = lsum;
-}
-
-// ---------------------------------------------------------------- adjoint columns + parameter gradient
+// ---------------------------------------------------------------- parameter gradient factors
 
 // The parameter-gradient factors of one texel: dL/d(V, theta, A) gets
 // dL/ds * ds/d(V, theta, A) with s = sqrt(P/2) unclamped, nothing where the
@@ -391,149 +207,6 @@ __device__ __forceinline__ void param_add(const ParamGeo& g, float dl, float dv_
     dA = fmaf(w, da_scale, dA);
 }
 
-// Each thread owns texel pairs (u, -u): u in strip column tx = strip*C + c
-// (tx <= N/2) at row ty, -u in column N - tx at row N - ty (mod N).  The two
-// share |k| and so the phase, and G_k(-u) = conj G_k(u) (h_k is real), so
-// one phase and one sincos per frame serve both.  Self-mirror columns (tx = 0,
-// N/2) map -u into the same column, where it is a u of another row: there
-// only u is accumulated.
-template <int N, bool UNIFORM>
-__device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool store_grad_h0, float2* lds,
-                                             float* red)
-{
-    constexpr int E = 16, T = N / E, C = fit_cols_per_block(N), NT = kFitColsThreads, LEN = lds_len(N);
-    constexpr int S  = 2 * C;           // transforms per frame pair: the strip's columns and their mirrors
-    constexpr int IT = C * N / NT;      // texel pairs per thread
-    static_assert(2 * S * T == NT && (C * N) % NT == 0 && T <= 64, "column block shape");
-    static_assert((C * N / 2) % NT == 0, "whole two-row tiles per thread");
-    const FitVars   vars = *m.vars;
-    const SpecConst sc = fit_spec(m, vars);
-    const size_t    nn = (size_t)N * N;
-
-    float2    ga[IT], gb[IT], gam[IT], gbm[IT];  // u, then -u
-    double    w[UNIFORM ? 1 : IT];
-    PhaseWalk walk[UNIFORM ? IT : 1];
-    const float4* ph = UNIFORM ? m.phase + (size_t)strip * (N * C) : nullptr;
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        ga[it] = gb[it] = gam[it] = gbm[it] = make_float2(0.0f, 0.0f);
-        const int k = threadIdx.x + it * NT;
-        if constexpr (UNIFORM) {
-            walk[it].load(ph[k]);
-        } else {
-            const int c = k % C, ty = k / C;
-            const int tx = min(strip * C + c, N / 2);
-            const int nx = tx - N / 2, ny = ty - N / 2;
-            float     kinv;
-            w[it] = omega_fp64(sc, nx * nx + ny * ny, kinv);
-        }
-    }
-
-    for (int p0 = 0; p0 < m.P; p0 += 2) {
-        const int np = min(2, m.P - p0);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            if (q >= np) break;
-            const float2* yp = m.y + (size_t)(p0 + q) * nn;
-            // load the strip's columns (slots 0..C-1) and their mirrors (slots
-            // C..2C-1), conjugated, two rows (one tile of Y) per 16-byte load
-            constexpr int ITL = C * N / NT;
-#pragma unroll
-            for (int it = 0; it < ITL; ++it) {
-                const int kk = threadIdx.x + it * NT;
-                const int slot = kk % S, t = kk / S;
-                const int txc = strip * C + slot % C;
-                const int col = slot < C ? txc : (N - txc) & (N - 1);
-                const float4 a = txc <= N / 2 ? ztile<N>(yp, t)[zpos<N>(col)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                lds[(q * S + slot) * LEN + lds_pad(2 * t)]     = make_float2(a.x, -a.y);
-                lds[(q * S + slot) * LEN + lds_pad(2 * t + 1)] = make_float2(a.z, -a.w);
-            }
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < np * S * T) {  // whole waves: np * 2C transforms of T threads
-            const int g = threadIdx.x / T, j = threadIdx.x % T;
-            float2*   buf = lds + g * LEN;
-            float2    v[E];
-#pragma unroll
-            for (int r = 0; r < E; ++r) v[r] = buf[lds_pad(j + r * T)];
-            fft_inverse<N, E>(v, j, buf, m.tw16, WaveSync{});
-            WaveSync{}();
-#pragma unroll
-            for (int r = 0; r < E; ++r) buf[lds_pad(j + r * T)] = v[r];
-        }
-        __syncthreads();
-#pragma unroll 1
-        for (int q = 0; q < 2; ++q) {
-            if (q >= np) break;
-            const int    k1 = 2 * (p0 + q), k2 = k1 + 1;
-            const bool   two = k2 < m.K;
-            const double tr1 = m.t_rev[k1], tr2 = m.t_rev[two ? k2 : k1];
-#pragma unroll
-            for (int it = 0; it < IT; ++it) {
-                const int k = threadIdx.x + it * NT;
-                const int c = k % C, ty = k / C, my = (N - ty) & (N - 1);
-                // F = conj of the stored transform; G1 = (F(u) + conj F(-u))/2, G2 = (F(u) - conj F(-u))/(2i)
-                const float2 fu = lds[(q * S + c) * LEN + lds_pad(ty)];
-                const float2 fm = lds[(q * S + C + c) * LEN + lds_pad(my)];
-                const float2 G1 = make_float2(0.5f * (fu.x + fm.x), 0.5f * (fm.y - fu.y));
-                const float2 G2 = make_float2(-0.5f * (fu.y + fm.y), -0.5f * (fu.x - fm.x));
-                float2 e1;
-                if constexpr (UNIFORM) e1 = walk[it].next();
-                else e1 = phase_at(w[it], tr1);
-                pull_back_pair(sc, G1, e1, ga[it], gb[it], gam[it], gbm[it]);
-                if (two) {
-                    float2 e2;
-                    if constexpr (UNIFORM) e2 = walk[it].next();
-                    else e2 = phase_at(w[it], tr2);
-                    pull_back_pair(sc, G2, e2, ga[it], gb[it], gam[it], gbm[it]);
-                }
-            }
-        }
-        __syncthreads();
-    }
-
-    const double V = vars.V, A = vars.A;
-    const float  dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
-    const float  da_scale = (float)(0.5 / A);
-    const float  iLp2     = 1.0f / sc.Lp2;
-    const float  sqA      = (float)sqrt(A);
-    float dV = 0.0f, dT = 0.0f, dA = 0.0f;
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-        const int k = threadIdx.x + it * NT;
-        const int c = k % C, ty = k / C;
-        const int tx = strip * C + c;
-        if (tx > N / 2) continue;  // past the half plane (the last strip)
-        // -u lies in the same column for tx = 0, N/2 (a u of its own); off the
-        // Nyquist row (ty != 0) its centred index is u's negated and shares u's
-        // factors, on it (my = 0) it has factors of its own
-        const bool self = tx == 0 || tx == N / 2;
-        const int  mx = N - tx, my = (N - ty) & (N - 1);
-        if (store_grad_h0) {
-            m.grad_h0[(size_t)ty * N + tx] = make_float4(ga[it].x, ga[it].y, gb[it].x, gb[it].y);
-            if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam[it].x, gam[it].y, gbm[it].x, gbm[it].y);
-        }
-        const bool     paired = !self && ty != 0;
-        const ParamGeo gu     = param_geo(sc, iLp2, sqA, N, tx, ty);
-        if (gu.live) {
-            float dl = dl_ds(ga[it], gb[it], m.xi[fit_xi_index(N, tx, ty)]);
-            if (paired) dl += dl_ds(gam[it], gbm[it], m.xi[fit_xi_index(N, mx, my)]);
-            param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
-        }
-        if (self || paired) continue;
-        const ParamGeo gm = param_geo(sc, iLp2, sqA, N, mx, my);
-        if (gm.live) param_add(gm, dl_ds(gam[it], gbm[it], m.xi[fit_xi_index(N, mx, my)]), dv_scale, da_scale, dV, dT, dA);
-    }
-    dV = block_sum(dV, red);
-    dT = block_sum(dT, red);
-    dA = block_sum(dA, red);
-    if (threadIdx.x == 0) {
-        m.grad_part[3 * strip + 0] = dV;
-        m.grad_part[3 * strip + 1] = dT;
-        m.grad_part[3 * strip + 2] = dA;
-    }
-}
-
 // ---------------------------------------------------------------- the step in the spectral domain
 
 // Loss and gradient of one column item without transforms.  h_k is real with
@@ -545,9 +218,9 @@ __device__ __forceinline__ void fit_cols_bwd(const FitMember& m, int strip, bool
 // (|D(-u)| = |D(u)|: both A_k and T^_k are Hermitian), or |D|^2 / N^2 for the
 // columns 0 and N/2, whose rows hold u and -u as texels of their own.  The
 // gradient of L = sum / (K N^2) with respect to the spectrum H(u) of the
-// texel is 2 D / (K N^4), and conj of it for -u: the pull-back of the
-// adjoint column pass (pull_back_pair) and its parameter terms follow as
-// there.  Loss partials hold sum_x (h_k - T_k)^2 like the row pass's.
+// texel is 2 D / (K N^4), and conj of it for -u: pull_back_pair takes it to
+// dL/dh0, then to dL/ds and the parameter partials.  Loss partials hold
+// sum_x (h_k - T_k)^2 (k_fit_reduce divides by K N^2).
 template <int N, bool UNIFORM>
 __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int mode, float* red)
 {
@@ -726,52 +399,6 @@ __global__ __launch_bounds__(256) void k_fit_xi(float4* __restrict__ xi, int N, 
     xi[fit_xi_index(N, tx, ty)] = box_muller(u01(r.x), u01(r.y), u01(r.z), u01(r.w));
 }
 
-__global__ __launch_bounds__(kFitColsThreads) void k_fit_cols_fwd(const FitMember* __restrict__ members,
-                                                                  const int2* __restrict__ items)
-{
-    __shared__ float2 lds[kColsLds];
-    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
-    const FitMember& m  = members[it.x];
-#define OCN_CALL(n) \
-    if (m.uniform) fit_cols_fwd<n, true>(m, it.y, lds); \
-    else fit_cols_fwd<n, false>(m, it.y, lds)
-    OCN_FIT_SWITCH(m.N, OCN_CALL)
-#undef OCN_CALL
-}
-
-__global__ __launch_bounds__(kFitRowsThreads) void k_fit_rows(const FitMember* __restrict__ members,
-                                                              const int2* __restrict__ items, int adjoint)
-{
-    __shared__ float2 lds[kRowsLds];
-    __shared__ float  red[kFitRowsThreads / 64];
-    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
-    const FitMember& m  = members[it.x];
-#define OCN_CALL(n) fit_rows<n>(m, it.y, adjoint != 0, lds, red)
-    OCN_FIT_SWITCH(m.N, OCN_CALL)
-#undef OCN_CALL
-}
-
-// WAVES = 3: capped at three waves per SIMD (a few registers spill): 0.80 ->
-// 0.68 ms per step of a 256-ocean N=256 fit against the uncapped two waves.
-// WAVES = 2 (no spills) for launches of fewer items than kFitColsFewItems,
-// whose blocks do not share CUs: single N=512 fit (129 items) forward+backward
-// 54.0 -> 51.9 us, where a 256-ocean step would lose 14%.
-template <int WAVES>
-__global__ __launch_bounds__(kFitColsThreads, WAVES) void k_fit_cols_bwd(const FitMember* __restrict__ members,
-                                                                      const int2* __restrict__ items,
-                                                                      int store_grad_h0)
-{
-    __shared__ float2 lds[kColsLds];
-    __shared__ float  red[kFitColsThreads / 64];
-    const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
-    const FitMember& m  = members[it.x];
-#define OCN_CALL(n) \
-    if (m.uniform) fit_cols_bwd<n, true>(m, it.y, store_grad_h0 != 0, lds, red); \
-    else fit_cols_bwd<n, false>(m, it.y, store_grad_h0 != 0, lds, red)
-    OCN_FIT_SWITCH(m.N, OCN_CALL)
-#undef OCN_CALL
-}
-
 // One quasi-Newton step of one member (a single thread, FP64), after the loss
 // L and gradient g at the current variables x were reduced.  BFGS on the
 // inverse Hessian with a trust region (Nocedal & Wright, Numerical
@@ -901,8 +528,8 @@ __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMembe
 }
 
 // The phase table of a member with equally spaced times: per column item and
-// texel pair the start and step of the recurrence the column kernels take
-// (the same omega_fp64 and PhaseWalk::start they would evaluate).
+// texel pair the start and step of the recurrence the step kernel takes
+// (the same omega_fp64 and PhaseWalk::start it would evaluate).
 __global__ __launch_bounds__(256) void k_fit_phase(FitMember m)
 {
     const int    C = fit_cols_per_block(m.N), N = m.N;
@@ -936,8 +563,8 @@ static hipError_t launch_fit_targets_n(const FitMember& m, hipStream_t s)
 {
     const size_t nn = (size_t)N * N, ne = fit_phase_entries(N);
     for (int f = 0; f < m.K; ++f) {
-        hipLaunchKernelGGL(k_fit_targets_rows<N>, dim3(N), dim3(N / 8), 0, s, m.targets + f * nn, m.z, m.tw8);
-        hipLaunchKernelGGL(k_fit_targets_cols<N>, dim3(N / 2 + 1), dim3(N / 8), 0, s, m.z, m.that + f * ne, m.tw8);
+        hipLaunchKernelGGL(k_fit_targets_rows<N>, dim3(N), dim3(N / 8), 0, s, m.targets + f * nn, m.scratch, m.tw8);
+        hipLaunchKernelGGL(k_fit_targets_cols<N>, dim3(N / 2 + 1), dim3(N / 8), 0, s, m.scratch, m.that + f * ne, m.tw8);
     }
     return hipGetLastError();
 }
@@ -958,11 +585,10 @@ hipError_t launch_fit_xi(const FitMember& m, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols,
-                      const int2* rows_items, int n_rows, int mode, const AdamCfg& adam, const QnCfg& qn,
-                      hipStream_t s)
+hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols, int mode,
+                      const AdamCfg& adam, const QnCfg& qn, hipStream_t s)
 {
-    if (n_members < 1 || n_cols < 1 || n_rows < 1) return hipErrorInvalidValue;
+    if (n_members < 1 || n_cols < 1) return hipErrorInvalidValue;
     // one launch over the column items: loss and gradient in the spectral domain
     hipLaunchKernelGGL(k_fit_spectral, dim3(n_cols), dim3(kFitColsThreads), 0, s, members, cols_items, mode);
     hipLaunchKernelGGL(k_fit_reduce, dim3(n_members), dim3(kFitReduceThreads), 0, s, members, mode, adam, qn);
