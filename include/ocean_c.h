@@ -409,14 +409,20 @@ int ocn_batch_copy_fields_to_host(ocn_batch* b, int member, float* height, float
  * h_k the height that ocn_update writes at times[k] for the current variables.
  * Gradients are dL/d(V, theta, A) and dL/dh0 (float4 per texel: d/dRe and
  * d/dIm of h0k, then of h0mk).  Each call recomputes the spectrum from the
- * noise; no per-frame field is kept between calls.  Calls on one fit must not
- * overlap in time. */
+ * noise; no per-frame field is kept between calls except the targets' 2-D
+ * DFT, which the first fit call after ocn_fit_targets or
+ * ocn_fit_upload_targets remakes (on that call's stream).  Calls on one fit
+ * must not overlap in time. */
 
 int ocn_fit_create(const ocn_params* members, int count, int K, const double* times, ocn_fit** out);
 int ocn_fit_destroy(ocn_fit* f);
 
 /* Device buffer of one member's targets: K * N * N floats, frame-major, [y][x]
- * rows as the height texture (zero after creation). */
+ * rows as the height texture (zero after creation).  Each call marks the
+ * member's targets as changed: write them through the pointer, then make the
+ * next fit call.  A caller that writes through a pointer it obtained earlier
+ * calls ocn_fit_targets again before its next fit call, or the fit keeps
+ * using the targets it last transformed. */
 int ocn_fit_targets(const ocn_fit* f, int member, float** targets);
 
 /* Copies K * N * N host floats into one member's targets, ordered on `stream`

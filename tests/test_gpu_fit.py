@@ -119,6 +119,33 @@ def test_fit_members_equal_single_fits(hip):
         np.testing.assert_array_equal(together[i][3], gh)
 
 
+def test_fit_sees_changed_targets(hip):
+    """The loss follows the targets through every way of changing them: a
+    second upload_targets, and a write through targets_ptr after asking for
+    the pointer again (the targets' 2-D DFT is remade on the next fit call)."""
+    N, L, times = 128, 250.0, [0.5, 1.0, 1.5]
+    kw = _kw(N, L, *START)
+    tg1 = _targets(N, L, times, key=11)
+    tg2 = _targets(N, L, times, key=22, sigma=0.2)
+    ref1 = _oracle_grads(kw, times, tg1)[0]
+    ref2 = _oracle_grads(kw, times, tg2)[0]
+    losses = []
+    with pyocean.Fit([pyocean.make_params(**kw)], times) as f:
+        for step in ("upload 1", "upload 2", "pointer 1", "pointer 2"):
+            _say("fit targets: %s" % step)
+            tg = tg1 if step.endswith("1") else tg2
+            if step.startswith("upload"):
+                f.upload_targets(0, tg)
+            else:
+                hip.sync()
+                hip.upload(f.targets_ptr(0), tg)
+            f.forward()
+            losses.append(f.get(0)[2])
+    _say("  losses %s, oracle %.6e %.6e" % (losses, ref1, ref2))
+    for loss, ref in zip(losses, (ref1, ref2, ref1, ref2)):
+        assert abs(loss - ref) <= 1e-4 * ref
+
+
 def _orc_from_lib(p):
     return oracle.make_params(N=p.N, L=p.L, wind_speed=p.wind_speed, wind_dir=(p.wind_dir_x, p.wind_dir_y),
                               amplitude=p.amplitude, suppression=p.suppression, choppiness=p.choppiness,
