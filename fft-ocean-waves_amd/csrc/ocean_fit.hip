@@ -247,10 +247,15 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
         if (tx > N / 2) continue;  // past the half plane (the last strip)
         const int  mx = (N - tx) & (N - 1), my = (N - ty) & (N - 1);
         const bool self = tx == 0 || tx == N / 2;
+        // this pair's loads first: its noise and the first four frames' T^
+        const float4 a = m.xi[fit_xi_index(N, tx, ty)], b = m.xi[fit_xi_index(N, mx, my)];
+        float2       Tq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            Tq[q] = q < m.K ? m.that[(size_t)q * ne + base + k] : make_float2(0.0f, 0.0f);
         // the amplitude is even in k; the mirror's differs on the Nyquist row or column only
         const float  s  = phillips_amp(sc, tx, ty);
         const float  sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
-        const float4 a = m.xi[fit_xi_index(N, tx, ty)], b = m.xi[fit_xi_index(N, mx, my)];
         const float4 hp = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         const float4 hq = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         PhaseWalk walk;
@@ -264,13 +269,22 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
         }
         const float wt = self ? inv_nn : 2.0f * inv_nn;
         float2 ga{}, gb{}, gam{}, gbm{};
-        for (int f = 0; f < m.K; ++f) {
-            const float2 e  = UNIFORM ? walk.next() : phase_at(w, m.t_rev[f]);
-            const float2 Af = fold_height(sc, hp, hq, e);
-            const float2 T  = m.that[(size_t)f * ne + base + k];
-            const float2 D  = make_float2(Af.x - T.x, Af.y - T.y);
-            lsum = fmaf(wt, fmaf(D.x, D.x, D.y * D.y), lsum);
-            if (grad) pull_back_pair(sc, make_float2(gsc * D.x, gsc * D.y), e, ga, gb, gam, gbm);
+        // frames in chunks of four: a chunk's T^ loads are issued before its arithmetic
+        for (int f0 = 0; f0 < m.K; f0 += 4) {
+            if (f0 > 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    Tq[q] = f0 + q < m.K ? m.that[(size_t)(f0 + q) * ne + base + k] : make_float2(0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (f0 + q >= m.K) break;
+                const float2 e  = UNIFORM ? walk.next() : phase_at(w, m.t_rev[f0 + q]);
+                const float2 Af = fold_height(sc, hp, hq, e);
+                const float2 D  = make_float2(Af.x - Tq[q].x, Af.y - Tq[q].y);
+                lsum = fmaf(wt, fmaf(D.x, D.x, D.y * D.y), lsum);
+                if (grad) pull_back_pair(sc, make_float2(gsc * D.x, gsc * D.y), e, ga, gb, gam, gbm);
+            }
         }
         if (!grad) continue;
         if (store_grad_h0) {
