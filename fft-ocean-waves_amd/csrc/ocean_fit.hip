@@ -240,19 +240,34 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
     const float     iLp2     = 1.0f / sc.Lp2;
     const float     sqA      = (float)sqrt(A);
     float lsum = 0.0f, dV = 0.0f, dT = 0.0f, dA = 0.0f;
+    // A pair's loads (its noise, phase-table entry and the first four frames'
+    // T^) are issued one pair ahead, during the previous pair's arithmetic.
+    // Every index stays inside the arrays for the columns past the half plane
+    // too (tx < N in the last strip); those pairs are loaded and skipped.
+    float4 na, nb, nph{};
+    float2 nT[4];
+    auto issue = [&](int it) {
+        const int kk = threadIdx.x + it * NT;
+        const int cc = kk % C, yy = kk / C, xx = strip * C + cc;
+        na = m.xi[fit_xi_index(N, xx, yy)];
+        nb = m.xi[fit_xi_index(N, (N - xx) & (N - 1), (N - yy) & (N - 1))];
+        if constexpr (UNIFORM) nph = m.phase[base + kk];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            nT[q] = q < m.K ? m.that[(size_t)q * ne + base + kk] : make_float2(0.0f, 0.0f);
+    };
+    issue(0);
+#pragma unroll
     for (int it = 0; it < IT; ++it) {
+        const float4 a = na, b = nb, ph = nph;
+        float2       Tq[4] = {nT[0], nT[1], nT[2], nT[3]};
+        if (it + 1 < IT) issue(it + 1);
         const int k  = threadIdx.x + it * NT;
         const int c  = k % C, ty = k / C;
         const int tx = strip * C + c;
         if (tx > N / 2) continue;  // past the half plane (the last strip)
         const int  mx = (N - tx) & (N - 1), my = (N - ty) & (N - 1);
         const bool self = tx == 0 || tx == N / 2;
-        // this pair's loads first: its noise and the first four frames' T^
-        const float4 a = m.xi[fit_xi_index(N, tx, ty)], b = m.xi[fit_xi_index(N, mx, my)];
-        float2       Tq[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            Tq[q] = q < m.K ? m.that[(size_t)q * ne + base + k] : make_float2(0.0f, 0.0f);
         // the amplitude is even in k; the mirror's differs on the Nyquist row or column only
         const float  s  = phillips_amp(sc, tx, ty);
         const float  sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
@@ -261,7 +276,7 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
         PhaseWalk walk;
         double    w = 0.0;
         if constexpr (UNIFORM) {
-            walk.load(m.phase[base + k]);
+            walk.load(ph);
         } else {
             const int nx = tx - N / 2, ny = ty - N / 2;
             float     kinv;
