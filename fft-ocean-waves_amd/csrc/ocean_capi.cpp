@@ -1605,13 +1605,21 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     // Per member, 256-byte aligned: xi 16, scratch 8, targets 4 K, dL/dh0 16 bytes per texel, partials,
     // the targets' spectra T^ (8 bytes per frame and texel pair of the half plane) and with equally spaced
     // times the phase table (16 bytes per texel pair of the half plane).
+    // Small fits split each column strip's rows over up to four blocks so the
+    // step launch has at least two blocks per CU of the 256.
+    int row_parts = 1;
+    {
+        long items = 0;
+        for (const ocn_params& p : f->params) items += ocn::fit_cols_items_per_member(p.N);
+        while (row_parts < ocn::kFitMaxRowParts && items * row_parts < 512) row_parts *= 2;
+    }
     auto   al    = [](size_t x) { return (x + 255) & ~size_t(255); };
     auto   phase_bytes = [&](int N) { return uniform_times ? al(sizeof(float4) * ocn::fit_phase_entries(N)) : 0; };
     auto   bytes = [&](int N) {
         const size_t nn = (size_t)N * N;
         return al(nn * 16) + al(nn * 8) + al(nn * 4 * K) + al(nn * 16) +
-               al(sizeof(float) * ocn::fit_cols_items_per_member(N)) +
-               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N)) +
+               al(sizeof(float) * ocn::fit_cols_items_per_member(N) * row_parts) +
+               al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N) * row_parts) +
                al(sizeof(float2) * K * ocn::fit_phase_entries(N)) + phase_bytes(N);
     };
     size_t total = 0;
@@ -1650,8 +1658,9 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.scratch   = reinterpret_cast<float2*>(q), q += al(nn * 8);
             m.targets   = reinterpret_cast<const float*>(q), q += al(nn * 4 * K);
             m.grad_h0   = reinterpret_cast<float4*>(q), q += al(nn * 16);
-            m.n_loss_part = ocn::fit_cols_items_per_member(p.N);  // one loss partial per column item
-            m.n_grad_part = ocn::fit_cols_items_per_member(p.N);
+            m.row_parts   = row_parts;
+            m.n_loss_part = ocn::fit_cols_items_per_member(p.N) * row_parts;  // one partial per step block
+            m.n_grad_part = m.n_loss_part;
             m.loss_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * m.n_loss_part);
             m.grad_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * 3 * m.n_grad_part);
             m.that      = reinterpret_cast<float2*>(q), q += al(sizeof(float2) * K * ocn::fit_phase_entries(p.N));

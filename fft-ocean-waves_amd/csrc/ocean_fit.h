@@ -57,7 +57,7 @@ struct FitMember {
     double  t_rev[kFitMaxK];  // t_k / (2 pi)
     double  dt_rev;         // (t_{k+1} - t_k) / (2 pi) when `uniform`
     int     uniform;        // 1: the times are equally spaced (phases by recurrence)
-    int     pad0_;
+    int     row_parts;      // blocks per column strip (1, 2 or 4), each over a contiguous part of its texel pairs
     const float4* xi;       // unit-amplitude noise, N*N
     // Equally spaced times: per column item and texel pair of its strip, the
     // dispersion phase e^{i w t_0} and its step e^{i w dt} as (cos, sin,
@@ -75,8 +75,8 @@ struct FitMember {
     // order).  Written by launch_fit_targets whenever the targets change.
     float2* that;
     float4* grad_h0;        // N*N: dL/dh0 as (dRe h0k, dIm h0k, dRe h0mk, dIm h0mk)
-    float*  loss_part;      // one partial sum per column item
-    float*  grad_part;      // three partial sums per column item
+    float*  loss_part;      // one partial sum per column item and row part
+    float*  grad_part;      // three partial sums per column item and row part
     int     n_loss_part, n_grad_part;
     FitVars* vars;
     uint32_t key0, key1, ocean_id, cascade_id;  // Philox noise of xi
@@ -84,6 +84,7 @@ struct FitMember {
 
 __host__ __device__ constexpr int fit_cols_per_block(int N) { return kFitColsWidth / N; }
 __host__ __device__ inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
+constexpr int kFitMaxRowParts = 4;  // a step block covers at least one texel pair per thread
 // float4 entries of a member's phase table
 __host__ __device__ inline size_t fit_phase_entries(int N) { return (size_t)fit_cols_items_per_member(N) * N * fit_cols_per_block(N); }
 
@@ -95,7 +96,7 @@ hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
 hipError_t launch_fit_targets(const FitMember& m, hipStream_t s);
 // Fills m.phase (a member with equally spaced times) from its constants.
 hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
-// cols_items: int2 (member, strip): one block each for the step kernel
+// cols_items: int2 (member, strip * row_parts + part): one block each for the step kernel
 hipError_t launch_fit(const FitMember* members, int n_members, const int2* cols_items, int n_cols, int mode,
                       const AdamCfg& adam, const QnCfg& qn, hipStream_t s);
 

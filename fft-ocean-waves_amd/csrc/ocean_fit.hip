@@ -209,8 +209,8 @@ __device__ __forceinline__ void param_add(const ParamGeo& g, float dl, float dv_
 
 // ---------------------------------------------------------------- the step in the spectral domain
 
-// Loss and gradient of one column item without transforms.  h_k is real with
-// (-1)^(x+y)/N^2 times the inverse DFT of its Hermitian-folded spectrum A_k
+// Loss and gradient of one column item, or of one of R equal parts of its
+// texel pairs, without transforms.  h_k is real with (-1)^(x+y)/N^2 times the inverse DFT of its Hermitian-folded spectrum A_k
 // over the texel grid, i.e. h_k = IDFT(A_k)/N^2 over centred wave numbers n.
 // With T^_k the DFT of the targets over the same n (Parseval):
 //   sum_x (h_k - T_k)^2 = sum_n |A_k(n) - T^_k(n)|^2 / N^2,
@@ -221,11 +221,13 @@ __device__ __forceinline__ void param_add(const ParamGeo& g, float dl, float dv_
 // texel is 2 D / (K N^4), and conj of it for -u: pull_back_pair takes it to
 // dL/dh0, then to dL/ds and the parameter partials.  Loss partials hold
 // sum_x (h_k - T_k)^2 (k_fit_reduce divides by K N^2).
-template <int N, bool UNIFORM>
-__device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int mode, float* red)
+template <int N, bool UNIFORM, int R>
+__device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int mode, float* red)
 {
-    constexpr int C = fit_cols_per_block(N), NT = kFitColsThreads, IT = C * N / NT;
-    static_assert((C * N) % NT == 0, "whole texel pairs per thread");
+    constexpr int C = fit_cols_per_block(N), NT = kFitColsThreads, IT = C * N / NT / R;
+    static_assert((C * N) % (NT * R) == 0, "whole texel pairs per thread");
+    // block `item` covers part `item % R` of the strip's C * N texel pairs
+    const int strip = item / R, it0 = (item % R) * IT * NT;
     const FitVars   vars = *m.vars;
     const SpecConst sc   = fit_spec(m, vars);
     const bool      grad = mode != kFitLoss;
@@ -247,7 +249,7 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
     float4 na, nb, nph{};
     float2 nT[4];
     auto issue = [&](int it) {
-        const int kk = threadIdx.x + it * NT;
+        const int kk = it0 + threadIdx.x + it * NT;
         const int cc = kk % C, yy = kk / C, xx = strip * C + cc;
         na = m.xi[fit_xi_index(N, xx, yy)];
         nb = m.xi[fit_xi_index(N, (N - xx) & (N - 1), (N - yy) & (N - 1))];
@@ -262,7 +264,7 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
         const float4 a = na, b = nb, ph = nph;
         float2       Tq[4] = {nT[0], nT[1], nT[2], nT[3]};
         if (it + 1 < IT) issue(it + 1);
-        const int k  = threadIdx.x + it * NT;
+        const int k  = it0 + threadIdx.x + it * NT;
         const int c  = k % C, ty = k / C;
         const int tx = strip * C + c;
         if (tx > N / 2) continue;  // past the half plane (the last strip)
@@ -320,15 +322,15 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int strip, int 
         if (gm.live) param_add(gm, dl_ds(gam, gbm, b), dv_scale, da_scale, dV, dT, dA);
     }
     lsum = block_sum(lsum, red);
-    if (threadIdx.x == 0) m.loss_part[strip] = lsum;
+    if (threadIdx.x == 0) m.loss_part[item] = lsum;
     if (grad) {
         dV = block_sum(dV, red);
         dT = block_sum(dT, red);
         dA = block_sum(dA, red);
         if (threadIdx.x == 0) {
-            m.grad_part[3 * strip + 0] = dV;
-            m.grad_part[3 * strip + 1] = dT;
-            m.grad_part[3 * strip + 2] = dA;
+            m.grad_part[3 * item + 0] = dV;
+            m.grad_part[3 * item + 1] = dT;
+            m.grad_part[3 * item + 2] = dA;
         }
     }
 }
@@ -378,6 +380,13 @@ __device__ __forceinline__ void fit_targets_col(const float2* __restrict__ R, fl
 
 }  // namespace
 
+template <int N, int R>
+__device__ __forceinline__ void fit_spectral_parts(const FitMember& m, int item, int mode, float* red)
+{
+    if (m.uniform) fit_spectral<N, true, R>(m, item, mode, red);
+    else fit_spectral<N, false, R>(m, item, mode, red);
+}
+
 #define OCN_FIT_SWITCH(N_, CALL)                 \
     switch (N_) {                                \
     case 64: CALL(64); break;                    \
@@ -395,8 +404,9 @@ __global__ __launch_bounds__(kFitColsThreads) void k_fit_spectral(const FitMembe
     const int2       it = items[xcd_logical(blockIdx.x, gridDim.x)];
     const FitMember& m  = members[it.x];
 #define OCN_CALL(n) \
-    if (m.uniform) fit_spectral<n, true>(m, it.y, mode, red); \
-    else fit_spectral<n, false>(m, it.y, mode, red)
+    if (m.row_parts == 4) fit_spectral_parts<n, 4>(m, it.y, mode, red); \
+    else if (m.row_parts == 2) fit_spectral_parts<n, 2>(m, it.y, mode, red); \
+    else fit_spectral_parts<n, 1>(m, it.y, mode, red)
     OCN_FIT_SWITCH(m.N, OCN_CALL)
 #undef OCN_CALL
 }

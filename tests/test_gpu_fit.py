@@ -95,10 +95,34 @@ def test_fit_gradients_match_oracle_adjoint(hip, name, N, L, times, conj, start)
     assert np.abs(err).max() <= GRAD_RTOL * np.abs(gh_ref).max()
 
 
-def test_fit_members_equal_single_fits(hip):
-    """Members of different N in one fit give exactly the single-member results."""
+# A fit's step splits each column strip over 1, 2 or 4 blocks, chosen by the
+# fit's total size, so a member's loss and parameter gradients (sums over its
+# texels) are summed in another order in a batch than alone, and the kernel
+# instances of the three splits round dL/dh0 differently in the last bits:
+# a member agrees with the same member fitted alone to float rounding.
+SUM_RTOL = 1e-6
+
+
+def _assert_same_member(together, g, loss, gh, msg):
+    assert abs(together[2] - loss) <= SUM_RTOL * abs(loss), msg
+    np.testing.assert_allclose(together[1], g, rtol=10 * SUM_RTOL, atol=SUM_RTOL * np.abs(g).max(), err_msg=msg)
+    np.testing.assert_allclose(together[3], gh, rtol=10 * SUM_RTOL, atol=SUM_RTOL * np.abs(gh).max(), err_msg=msg)
+
+
+MEMBER_SETS = {
+    # 45 column strips: four blocks per strip in the fit and alone
+    "mixed_n": [(64, 100.0, (12.0, 0.3, 2.0), 0), (256, 500.0, (25.0, -1.0, 4.0), 1), (128, 250.0, (8.0, 2.5, 3.0), 0)],
+    # 258 strips: two blocks per strip in the fit, four alone
+    "two_n512": [(512, 1000.0, (18.0, 0.4, 2.5), 0), (512, 1000.0, (24.0, 2.0, 3.5), 1)],
+}
+
+
+@pytest.mark.parametrize("set_name", sorted(MEMBER_SETS))
+def test_fit_members_equal_single_fits(hip, set_name):
+    """Members in one fit give the single-member results to float rounding."""
     times = [0.5, 1.0, 1.5]
-    specs = [(64, 100.0, (12.0, 0.3, 2.0), 0), (256, 500.0, (25.0, -1.0, 4.0), 1), (128, 250.0, (8.0, 2.5, 3.0), 0)]
+    specs = MEMBER_SETS[set_name]
+    _say("fit members %s" % set_name)
     members, tgs = [], []
     for i, (N, L, v, conj) in enumerate(specs):
         members.append(pyocean.make_params(**_kw(N, L, *v, conj=conj, ocean_id=i, seed=42 + i)))
@@ -114,9 +138,7 @@ def test_fit_members_equal_single_fits(hip):
             f1.backward()
             x, g, loss = f1.get(0)
             gh = f1.grad_h0(0)
-        np.testing.assert_array_equal(together[i][1], g)
-        assert together[i][2] == loss
-        np.testing.assert_array_equal(together[i][3], gh)
+        _assert_same_member(together[i], g, loss, gh, "member %d" % i)
 
 
 def test_fit_sees_changed_targets(hip):
@@ -162,7 +184,8 @@ def test_c5_batched_fit_at_size(hip):
     sides of every comparison get the same targets).  One forward+backward of
     all 256 members in one set of launches: members 0, 128 and 255 against
     the FP64 oracle adjoint (loss, dL/d(V, theta, A), dL/dh0 at rel <= 1e-3),
-    and every member bit-equal to a fit of that member alone."""
+    and every member equal to a fit of that member alone to float
+    rounding."""
     times = [0.5, 1.0, 1.5, 2.0]
     M = 256
     hidden, starts = pyocean.fit_batch_params(M)
@@ -209,9 +232,7 @@ def test_c5_batched_fit_at_size(hip):
             f1.backward()
             x, g, loss = f1.get(0)
             gh = f1.grad_h0(0)
-        np.testing.assert_array_equal(together[o][1], g, err_msg="member %d" % o)
-        assert together[o][2] == loss, o
-        np.testing.assert_array_equal(together[o][3], gh, err_msg="member %d" % o)
+        _assert_same_member(together[o], g, loss, gh, "member %d" % o)
 
 
 def test_fit_adam_recovers_hidden_variables(hip):
