@@ -411,8 +411,9 @@ int ocn_batch_copy_fields_to_host(ocn_batch* b, int member, float* height, float
  * d/dIm of h0k, then of h0mk).  Each call recomputes the spectrum from the
  * noise; no per-frame field is kept between calls except the targets' 2-D
  * DFT, which the first fit call after ocn_fit_targets or
- * ocn_fit_upload_targets remakes (on that call's stream).  Calls on one fit
- * must not overlap in time. */
+ * ocn_fit_upload_targets remakes (on that call's stream).  A member's results
+ * depend on the other members of its fit only through float rounding (the
+ * order of its sums).  Calls on one fit must not overlap in time. */
 
 int ocn_fit_create(const ocn_params* members, int count, int K, const double* times, ocn_fit** out);
 int ocn_fit_destroy(ocn_fit* f);
