@@ -18,7 +18,8 @@
 //
 //   k_fit_targets_rows/cols  per member and target frame, when the targets
 //                   changed: the 2-D DFT T^ at the half-plane texels
-//   k_fit_spectral  item (member, strip of C columns): h0 = s(V, theta, A) xi,
+//   k_fit_spectral  item (member, strip of C columns or a row part of it):
+//                   h0 = s(V, theta, A) xi,
 //                   per frame the folded spectrum, its residual against T^,
 //                   the loss partial; in gradient mode the pull-back to dL/dh0
 //                   and dL/ds per texel and the three parameter partials
@@ -529,41 +530,58 @@ __device__ void qn_step(FitVars& v, double L, const double g[3], const QnCfg& cf
 __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMember* __restrict__ members, int mode,
                                                                   AdamCfg cfg, QnCfg qcfg)
 {
-    __shared__ double red[kFitReduceThreads / 64];
+    constexpr int     NW = kFitReduceThreads / 64;
+    __shared__ double red[4 * NW];
     const FitMember&  m = members[blockIdx.x];
-    double l = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int i = threadIdx.x; i < m.n_loss_part; i += blockDim.x) l += (double)m.loss_part[i];
+    // thread 0 loads the member's variables before the partial sums, so the
+    // two loads' latencies overlap
+    FitVars v;
+    if (threadIdx.x == 0) v = *m.vars;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};  // loss, dL/dV, dL/dtheta, dL/dA
+    for (int i = threadIdx.x; i < m.n_loss_part; i += blockDim.x) t[0] += (double)m.loss_part[i];
     if (mode != kFitLoss)
         for (int i = threadIdx.x; i < m.n_grad_part; i += blockDim.x) {
-            g0 += (double)m.grad_part[3 * i + 0];
-            g1 += (double)m.grad_part[3 * i + 1];
-            g2 += (double)m.grad_part[3 * i + 2];
+            t[1] += (double)m.grad_part[3 * i + 0];
+            t[2] += (double)m.grad_part[3 * i + 1];
+            t[3] += (double)m.grad_part[3 * i + 2];
         }
-    l = block_sum(l, red);
-    g0 = block_sum(g0, red);
-    g1 = block_sum(g1, red);
-    g2 = block_sum(g2, red);
+    // the four sums in one pass: across each wave, then over the waves in order
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] += __shfl_xor(t[j], o);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[4 * (threadIdx.x / 64) + j] = t[j];
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    FitVars& v = *m.vars;
-    v.loss = l / ((double)m.K * (double)m.N * (double)m.N);
-    if (mode == kFitLoss) return;
-    const double g[3] = {g0, g1, g2};
-    for (int i = 0; i < 3; ++i) v.grad[i] = g[i];
-    if (mode == kFitQn) {
-        qn_step(v, l / ((double)m.K * (double)m.N * (double)m.N), g, qcfg);
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < NW; ++w)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) sum[j] += red[4 * w + j];
+    const double L = sum[0] / ((double)m.K * (double)m.N * (double)m.N);
+    if (mode == kFitLoss) {
+        m.vars->loss = L;
         return;
     }
-    if (mode != kFitAdam) return;
-    v.step += 1;
-    const double b1t = 1.0 - pow(cfg.beta1, (double)v.step), b2t = 1.0 - pow(cfg.beta2, (double)v.step);
-    double* x[3] = {&v.V, &v.theta, &v.A};
-    for (int i = 0; i < 3; ++i) {
-        v.m[i] = cfg.beta1 * v.m[i] + (1.0 - cfg.beta1) * g[i];
-        v.v[i] = cfg.beta2 * v.v[i] + (1.0 - cfg.beta2) * g[i] * g[i];
-        *x[i] -= cfg.lr[i] * (v.m[i] / b1t) / (sqrt(v.v[i] / b2t) + cfg.eps);
+    const double g[3] = {sum[1], sum[2], sum[3]};
+    v.loss = L;
+    for (int i = 0; i < 3; ++i) v.grad[i] = g[i];
+    if (mode == kFitQn) {
+        qn_step(v, L, g, qcfg);
+    } else if (mode == kFitAdam) {
+        v.step += 1;
+        const double b1t = 1.0 - pow(cfg.beta1, (double)v.step), b2t = 1.0 - pow(cfg.beta2, (double)v.step);
+        double* x[3] = {&v.V, &v.theta, &v.A};
+        for (int i = 0; i < 3; ++i) {
+            v.m[i] = cfg.beta1 * v.m[i] + (1.0 - cfg.beta1) * g[i];
+            v.v[i] = cfg.beta2 * v.v[i] + (1.0 - cfg.beta2) * g[i] * g[i];
+            *x[i] -= cfg.lr[i] * (v.m[i] / b1t) / (sqrt(v.v[i] / b2t) + cfg.eps);
+        }
+        v.V = fmax(v.V, 1e-3);
+        v.A = fmax(v.A, 1e-6);
     }
-    v.V = fmax(v.V, 1e-3);
-    v.A = fmax(v.A, 1e-6);
+    *m.vars = v;
 }
 
 // The phase table of a member with equally spaced times: per column item and
