@@ -271,9 +271,28 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
         if (tx > N / 2) continue;  // past the half plane (the last strip)
         const int  mx = (N - tx) & (N - 1), my = (N - ty) & (N - 1);
         const bool self = tx == 0 || tx == N / 2;
-        // the amplitude is even in k; the mirror's differs on the Nyquist row or column only
-        const float  s  = phillips_amp(sc, tx, ty);
-        const float  sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
+        // -u lies in the same column for tx = 0, N/2 (a u of its own); off the
+        // Nyquist row its centred index is u's negated and shares u's factors
+        const bool paired = !self && ty != 0;
+        // The amplitude is even in k; the mirror's differs on the Nyquist row or
+        // column only.  In gradient mode it is taken from the parameter-gradient
+        // factors (the same amplitude to rounding), which the terms after the
+        // frames reuse.
+        ParamGeo gu{}, gm{};
+        float    s, sm;
+        if (grad) {
+            gu = param_geo(sc, iLp2, sqA, N, tx, ty);
+            s  = fminf(gu.s, 4000.0f);
+            if (!self && !paired) {
+                gm = param_geo(sc, iLp2, sqA, N, mx, my);
+                sm = fminf(gm.s, 4000.0f);
+            } else {
+                sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
+            }
+        } else {
+            s  = phillips_amp(sc, tx, ty);
+            sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
+        }
         const float4 hp = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
         const float4 hq = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         PhaseWalk walk;
@@ -309,17 +328,12 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
             m.grad_h0[(size_t)ty * N + tx] = make_float4(ga.x, ga.y, gb.x, gb.y);
             if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam.x, gam.y, gbm.x, gbm.y);
         }
-        // -u lies in the same column for tx = 0, N/2 (a u of its own); off the
-        // Nyquist row its centred index is u's negated and shares u's factors
-        const bool     paired = !self && ty != 0;
-        const ParamGeo gu     = param_geo(sc, iLp2, sqA, N, tx, ty);
         if (gu.live) {
             float dl = dl_ds(ga, gb, a);
             if (paired) dl += dl_ds(gam, gbm, b);
             param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
         }
         if (self || paired) continue;
-        const ParamGeo gm = param_geo(sc, iLp2, sqA, N, mx, my);
         if (gm.live) param_add(gm, dl_ds(gam, gbm, b), dv_scale, da_scale, dV, dT, dA);
     }
     lsum = block_sum(lsum, red);
