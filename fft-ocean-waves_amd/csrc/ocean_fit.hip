@@ -93,43 +93,6 @@ __device__ __forceinline__ float2 phase_at(double w, double t_rev)
     return make_float2(c, s);
 }
 
-// Hermitian part (H(t) + conj H(t'))/2 of the height spectrum at one time;
-// p = h0 at t, q = h0 at its mirror t' (which shares t's |k| and so its phase
-// e^{i phi} = (e.x, e.y)).
-__device__ __forceinline__ float2 fold_height(const SpecConst& sc, float4 p, float4 q, float2 e)
-{
-    const float2 H = evolve_amp(sc, p, e.y, e.x);
-    const float2 G = evolve_amp(sc, q, e.y, e.x);
-    return make_float2(0.5f * (H.x + G.x), 0.5f * (H.y - G.y));
-}
-
-// dL/dh0 += the pull-back of the frame spectrum gradient G through
-// H = h0k e^{i phi} + C(h0mk) e^{-i phi}: ga += e^{-i phi} G, gb += e^{+i phi} G
-// (identity C) or e^{-i phi} conj(G) (conjugation), e = e^{i phi} of the texel;
-// done for a texel u (ga, gb) and its mirror -u (gam, gbm), which share the
-// phase and whose gradients are conjugate: G(-u) = conj G(u).
-__device__ __forceinline__ void pull_back_pair(const SpecConst& sc, float2 G, float2 e, float2& ga, float2& gb,
-                                               float2& gam, float2& gbm)
-{
-    const float c = e.x, s = e.y;
-    const float cx = c * G.x, cy = c * G.y, sx = s * G.x, sy = s * G.y;
-    ga.x += cx + sy;  // e^{-i phi} G
-    ga.y += cy - sx;
-    gam.x += cx - sy;  // e^{-i phi} conj G
-    gam.y -= cy + sx;
-    if (sc.conj_mode) {
-        gb.x += cx - sy;  // e^{-i phi} conj G
-        gb.y -= sx + cy;
-        gbm.x += cx + sy;  // e^{-i phi} G
-        gbm.y += cy - sx;
-    } else {
-        gb.x += cx - sy;  // e^{+i phi} G
-        gb.y += sx + cy;
-        gbm.x += cx + sy;  // e^{+i phi} conj G
-        gbm.y += sx - cy;
-    }
-}
-
 template <class T>
 __device__ __forceinline__ T block_sum(T x, T* red)
 {
@@ -219,7 +182,7 @@ __device__ __forceinline__ void param_add(const ParamGeo& g, float dl, float dv_
 // (|D(-u)| = |D(u)|: both A_k and T^_k are Hermitian), or |D|^2 / N^2 for the
 // columns 0 and N/2, whose rows hold u and -u as texels of their own.  The
 // gradient of L = sum / (K N^2) with respect to the spectrum H(u) of the
-// texel is 2 D / (K N^4), and conj of it for -u: pull_back_pair takes it to
+// texel is 2 D / (K N^4), and conj of it for -u: the pull-back takes it to
 // dL/dh0, then to dL/ds and the parameter partials.  Loss partials hold
 // sum_x (h_k - T_k)^2 (k_fit_reduce divides by K N^2).
 template <int N, bool UNIFORM, int R>
@@ -305,7 +268,19 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
             w = omega_fp64(sc, nx * nx + ny * ny, kinv);
         }
         const float wt = self ? inv_nn : 2.0f * inv_nn;
-        float2 ga{}, gb{}, gam{}, gbm{};
+        // With H = h0k e + C(h0mk) conj(e) at u and G the same at -u, the fold
+        // (H + conj G)/2 is P e + Q conj(e) = U cos + V sin with P, Q formed once
+        // per pair: 4 FMAs per frame.
+        const float  bpi = sc.conj_mode ? -hp.w : hp.w, bqi = sc.conj_mode ? -hq.w : hq.w;
+        const float2 P   = make_float2(0.5f * (hp.x + hq.z), 0.5f * (hp.y - bqi));
+        const float2 Q   = make_float2(0.5f * (hp.z + hq.x), 0.5f * (bpi - hq.y));
+        const float2 U   = make_float2(P.x + Q.x, P.y + Q.y), Vs = make_float2(Q.y - P.y, P.x - Q.x);
+        // The pull-back of G = gsc D through H = h0k e + C(h0mk) conj(e): dL/dh0k
+        // (ga) gathers conj(e) G; at -u, where the gradient is conj G, dL/dh0k
+        // (gam) gathers conj(e) conj(G).  dL/dh0mk gathers e G at u and e conj(G)
+        // at -u for C = identity, conj(e) conj(G) and conj(e) G for C = conj.
+        // All four follow from S1 = sum conj(e) D and S2 = sum e D.
+        float2 S1{}, S2{};
         // frames in chunks of four: a chunk's T^ loads are issued before its arithmetic
         for (int f0 = 0; f0 < m.K; f0 += 4) {
             if (f0 > 0) {
@@ -317,13 +292,22 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
             for (int q = 0; q < 4; ++q) {
                 if (f0 + q >= m.K) break;
                 const float2 e  = UNIFORM ? walk.next() : phase_at(w, m.t_rev[f0 + q]);
-                const float2 Af = fold_height(sc, hp, hq, e);
-                const float2 D  = make_float2(Af.x - Tq[q].x, Af.y - Tq[q].y);
+                const float2 D  = make_float2(fmaf(U.x, e.x, fmaf(Vs.x, e.y, -Tq[q].x)),
+                                              fmaf(U.y, e.x, fmaf(Vs.y, e.y, -Tq[q].y)));
                 lsum = fmaf(wt, fmaf(D.x, D.x, D.y * D.y), lsum);
-                if (grad) pull_back_pair(sc, make_float2(gsc * D.x, gsc * D.y), e, ga, gb, gam, gbm);
+                if (grad) {
+                    S1.x = fmaf(e.x, D.x, fmaf(e.y, D.y, S1.x));
+                    S1.y = fmaf(e.x, D.y, fmaf(-e.y, D.x, S1.y));
+                    S2.x = fmaf(e.x, D.x, fmaf(-e.y, D.y, S2.x));
+                    S2.y = fmaf(e.x, D.y, fmaf(e.y, D.x, S2.y));
+                }
             }
         }
         if (!grad) continue;
+        const float2 ga  = make_float2(gsc * S1.x, gsc * S1.y);
+        const float2 gam = make_float2(gsc * S2.x, -gsc * S2.y);
+        const float2 gb  = sc.conj_mode ? gam : make_float2(gsc * S2.x, gsc * S2.y);
+        const float2 gbm = sc.conj_mode ? ga : make_float2(gsc * S1.x, -gsc * S1.y);
         if (store_grad_h0) {
             m.grad_h0[(size_t)ty * N + tx] = make_float4(ga.x, ga.y, gb.x, gb.y);
             if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam.x, gam.y, gbm.x, gbm.y);
