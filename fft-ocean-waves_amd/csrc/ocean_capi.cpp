@@ -1620,7 +1620,8 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
         return al(nn * 16) + al(nn * 8) + al(nn * 4 * K) + al(nn * 16) +
                al(sizeof(float) * ocn::fit_cols_items_per_member(N) * row_parts) +
                al(sizeof(float) * 3 * ocn::fit_cols_items_per_member(N) * row_parts) +
-               al(sizeof(float2) * K * ocn::fit_phase_entries(N)) + phase_bytes(N);
+               al(sizeof(float2) * K * ocn::fit_phase_entries(N)) + phase_bytes(N) +
+               al(sizeof(float4) * ocn::fit_phase_entries(N)) + al(2 * sizeof(float4) * ocn::fit_edge_pairs(N));
     };
     size_t total = 0;
     for (const ocn_params& p : f->params) total += bytes(p.N);
@@ -1665,6 +1666,8 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
             m.grad_part = reinterpret_cast<float*>(q), q += al(sizeof(float) * 3 * m.n_grad_part);
             m.that      = reinterpret_cast<float2*>(q), q += al(sizeof(float2) * K * ocn::fit_phase_entries(p.N));
             m.phase     = uniform_times ? reinterpret_cast<const float4*>(q) : nullptr, q += phase_bytes(p.N);
+            m.pq        = reinterpret_cast<const float4*>(q), q += al(sizeof(float4) * ocn::fit_phase_entries(p.N));
+            m.edge      = reinterpret_cast<const float4*>(q), q += al(2 * sizeof(float4) * ocn::fit_edge_pairs(p.N));
             m.tw8       = f->tw[p.N][0];
             m.tw16      = f->tw[p.N][1];
             m.vars      = f->d_vars + i;
@@ -1685,6 +1688,8 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
         }
         for (int i = 0; !rc && i < count; ++i) {
             if ((e = ocn::launch_fit_xi(f->members[i], nullptr)) != hipSuccess) rc = hip_fail(e, "fit noise launch");
+            else if ((e = ocn::launch_fit_pq(f->members[i], nullptr)) != hipSuccess)
+                rc = hip_fail(e, "fit noise coefficients launch");
             else if (uniform_times && (e = ocn::launch_fit_phase(f->members[i], nullptr)) != hipSuccess)
                 rc = hip_fail(e, "fit phase launch");
         }

@@ -59,6 +59,12 @@ struct FitMember {
     int     uniform;        // 1: the times are equally spaced (phases by recurrence)
     int     row_parts;      // blocks per column strip (1, 2 or 4), each over a contiguous part of its texel pairs
     const float4* xi;       // unit-amplitude noise, N*N
+    // Per half-plane texel pair, in the phase table's order: the noise
+    // coefficients (P~, Q~) of the fold (launch_fit_pq); and, for the pairs on
+    // row 0 and in the columns 0 and N/2, both texels' unit noise, two float4
+    // per pair at fit_edge_index.
+    const float4* pq;
+    const float4* edge;
     // Equally spaced times: per column item and texel pair of its strip, the
     // dispersion phase e^{i w t_0} and its step e^{i w dt} as (cos, sin,
     // cos, sin), written once at creation (launch_fit_phase); entry
@@ -85,6 +91,13 @@ struct FitMember {
 __host__ __device__ constexpr int fit_cols_per_block(int N) { return kFitColsWidth / N; }
 __host__ __device__ inline int fit_cols_items_per_member(int N) { return (N / 2) / fit_cols_per_block(N) + 1; }
 constexpr int kFitMaxRowParts = 4;  // a step block covers at least one texel pair per thread
+// half-plane texel pairs on row 0 and in the columns 0 and N/2, and the index
+// of one (tx <= N/2; ty == 0 or tx == 0 or tx == N/2)
+__host__ __device__ constexpr int fit_edge_pairs(int N) { return N / 2 + 2 * N - 1; }
+__host__ __device__ inline int fit_edge_index(int N, int tx, int ty)
+{
+    return ty == 0 ? tx : tx == 0 ? N / 2 + ty : N / 2 + N - 1 + ty;
+}
 // float4 entries of a member's phase table
 __host__ __device__ inline size_t fit_phase_entries(int N) { return (size_t)fit_cols_items_per_member(N) * N * fit_cols_per_block(N); }
 
@@ -94,6 +107,8 @@ enum FitMode { kFitLoss = 0, kFitGrad = 1, kFitAdam = 2, kFitQn = 3 };
 hipError_t launch_fit_xi(const FitMember& m, hipStream_t s);
 // T^ of every target frame of member m (m.scratch holds the row transforms).
 hipError_t launch_fit_targets(const FitMember& m, hipStream_t s);
+// Fills m.pq and m.edge from m.xi.
+hipError_t launch_fit_pq(const FitMember& m, hipStream_t s);
 // Fills m.phase (a member with equally spaced times) from its constants.
 hipError_t launch_fit_phase(const FitMember& m, hipStream_t s);
 // cols_items: int2 (member, strip * row_parts + part): one block each for the step kernel

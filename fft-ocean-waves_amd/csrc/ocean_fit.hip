@@ -206,17 +206,25 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
     const float     iLp2     = 1.0f / sc.Lp2;
     const float     sqA      = (float)sqrt(A);
     float lsum = 0.0f, dV = 0.0f, dT = 0.0f, dA = 0.0f;
-    // A pair's loads (its noise, phase-table entry and the first four frames'
-    // T^) are issued one pair ahead, during the previous pair's arithmetic.
-    // Every index stays inside the arrays for the columns past the half plane
-    // too (tx < N in the last strip); those pairs are loaded and skipped.
-    float4 na, nb, nph{};
+    // A pair's loads (its noise coefficients, phase-table entry and the first
+    // four frames' T^) are issued one pair ahead, during the previous pair's
+    // arithmetic.  A pair off the edges takes its noise as the coefficients
+    // (P~, Q~) of its entry in m.pq; a pair on row 0 or in the columns 0 and
+    // N/2 takes both texels' unit noise from m.edge.  Every index stays inside
+    // the arrays for the columns past the half plane too (the last strip,
+    // tx < N, read from m.pq); those pairs are loaded and skipped.
+    float4 na, nb{}, nph{};
     float2 nT[4];
     auto issue = [&](int it) {
         const int kk = it0 + threadIdx.x + it * NT;
         const int cc = kk % C, yy = kk / C, xx = strip * C + cc;
-        na = m.xi[fit_xi_index(N, xx, yy)];
-        nb = m.xi[fit_xi_index(N, (N - xx) & (N - 1), (N - yy) & (N - 1))];
+        if (xx <= N / 2 && (xx == 0 || xx == N / 2 || yy == 0)) {
+            const int ei = fit_edge_index(N, xx, yy);
+            na = m.edge[2 * ei];
+            nb = m.edge[2 * ei + 1];
+        } else {
+            na = m.pq[base + kk];
+        }
         if constexpr (UNIFORM) nph = m.phase[base + kk];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -256,8 +264,6 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
             s  = phillips_amp(sc, tx, ty);
             sm = (tx != 0 && ty != 0) ? s : phillips_amp(sc, mx, my);
         }
-        const float4 hp = make_float4(s * a.x, s * a.y, s * a.z, s * a.w);
-        const float4 hq = make_float4(sm * b.x, sm * b.y, sm * b.z, sm * b.w);
         PhaseWalk walk;
         double    w = 0.0;
         if constexpr (UNIFORM) {
@@ -270,11 +276,19 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
         const float wt = self ? inv_nn : 2.0f * inv_nn;
         // With H = h0k e + C(h0mk) conj(e) at u and G the same at -u, the fold
         // (H + conj G)/2 is P e + Q conj(e) = U cos + V sin with P, Q formed once
-        // per pair: 4 FMAs per frame.
-        const float  bpi = sc.conj_mode ? -hp.w : hp.w, bqi = sc.conj_mode ? -hq.w : hq.w;
-        const float2 P   = make_float2(0.5f * (hp.x + hq.z), 0.5f * (hp.y - bqi));
-        const float2 Q   = make_float2(0.5f * (hp.z + hq.x), 0.5f * (bpi - hq.y));
-        const float2 U   = make_float2(P.x + Q.x, P.y + Q.y), Vs = make_float2(Q.y - P.y, P.x - Q.x);
+        // per pair: 4 FMAs per frame.  Off the edges s = sm and P = s P~,
+        // Q = s Q~ with the noise coefficients of the pair (k_fit_pq).
+        const bool edge = !paired;
+        float2     P, Q;
+        if (edge) {
+            const float bpi = sc.conj_mode ? -a.w : a.w, bqi = sc.conj_mode ? -b.w : b.w;
+            P = make_float2(0.5f * (s * a.x + sm * b.z), 0.5f * (s * a.y - sm * bqi));
+            Q = make_float2(0.5f * (s * a.z + sm * b.x), 0.5f * (s * bpi - sm * b.y));
+        } else {
+            P = make_float2(s * a.x, s * a.y);
+            Q = make_float2(s * a.z, s * a.w);
+        }
+        const float2 U = make_float2(P.x + Q.x, P.y + Q.y), Vs = make_float2(Q.y - P.y, P.x - Q.x);
         // The pull-back of G = gsc D through H = h0k e + C(h0mk) conj(e): dL/dh0k
         // (ga) gathers conj(e) G; at -u, where the gradient is conj G, dL/dh0k
         // (gam) gathers conj(e) conj(G).  dL/dh0mk gathers e G at u and e conj(G)
@@ -312,9 +326,11 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
             m.grad_h0[(size_t)ty * N + tx] = make_float4(ga.x, ga.y, gb.x, gb.y);
             if (!self) m.grad_h0[(size_t)my * N + mx] = make_float4(gam.x, gam.y, gbm.x, gbm.y);
         }
+        // dL/ds: the texels' gradients against their unit noise; for a pair off
+        // the edges, ga.a + gb.a' + gam.b + gbm.b' = 2 gsc (S1.P~ + S2.Q~)
         if (gu.live) {
-            float dl = dl_ds(ga, gb, a);
-            if (paired) dl += dl_ds(gam, gbm, b);
+            const float dl = edge ? dl_ds(ga, gb, a)
+                                  : 2.0f * gsc * (S1.x * a.x + S1.y * a.y + S2.x * a.z + S2.y * a.w);
             param_add(gu, dl, dv_scale, da_scale, dV, dT, dA);
         }
         if (self || paired) continue;
@@ -610,6 +626,43 @@ hipError_t launch_fit_phase(const FitMember& m, hipStream_t s)
     if (!m.uniform || !m.phase) return hipErrorInvalidValue;
     const size_t n = fit_phase_entries(m.N);
     hipLaunchKernelGGL(k_fit_phase, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m);
+    return hipGetLastError();
+}
+
+// The noise coefficients of each half-plane texel pair, in the phase table's
+// order: P~ = (a + conj(C b'))/2 and Q~ = (C a' + conj b)/2 with a = (xi.x, xi.y),
+// a' = (xi.z, xi.w) of the texel and b, b' of its mirror, C the member's
+// conjugation of h0mk; and, for the pairs on row 0 and in the columns 0 and
+// N/2, both texels' unit noise (fit_edge_index).  Made once at creation.
+__global__ __launch_bounds__(256) void k_fit_pq(FitMember m)
+{
+    const int    C = fit_cols_per_block(m.N), N = m.N;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= fit_phase_entries(N)) return;
+    const int strip = (int)(e / ((size_t)N * C)), k = (int)(e % ((size_t)N * C));
+    const int c = k % C, ty = k / C, tx = strip * C + c;
+    float4*   pq = const_cast<float4*>(m.pq);
+    if (tx > N / 2) {
+        pq[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const int    mx = (N - tx) & (N - 1), my = (N - ty) & (N - 1);
+    const float4 a = m.xi[fit_xi_index(N, tx, ty)], b = m.xi[fit_xi_index(N, mx, my)];
+    const float  aw = m.conj_mode ? -a.w : a.w, bw = m.conj_mode ? -b.w : b.w;
+    pq[e] = make_float4(0.5f * (a.x + b.z), 0.5f * (a.y - bw), 0.5f * (a.z + b.x), 0.5f * (aw - b.y));
+    if (tx == 0 || tx == N / 2 || ty == 0) {
+        float4*   edge = const_cast<float4*>(m.edge);
+        const int ei   = fit_edge_index(N, tx, ty);
+        edge[2 * ei]     = a;
+        edge[2 * ei + 1] = b;
+    }
+}
+
+hipError_t launch_fit_pq(const FitMember& m, hipStream_t s)
+{
+    if (!m.pq || !m.edge) return hipErrorInvalidValue;
+    const size_t n = fit_phase_entries(m.N);
+    hipLaunchKernelGGL(k_fit_pq, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m);
     return hipGetLastError();
 }
 
