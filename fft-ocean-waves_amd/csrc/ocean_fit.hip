@@ -16,6 +16,8 @@
 // targets.  Nothing per frame is stored between steps: h0 is recomputed from
 // the unit noise xi and the current variables.
 //
+//   k_fit_xi, k_fit_pq, k_fit_phase  at creation: the unit noise, the fold's
+//                   noise coefficients per texel pair, the phase table
 //   k_fit_targets_rows/cols  per member and target frame, when the targets
 //                   changed: the 2-D DFT T^ at the half-plane texels
 //   k_fit_spectral  item (member, strip of C columns or a row part of it):
