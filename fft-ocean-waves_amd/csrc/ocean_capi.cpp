@@ -628,6 +628,7 @@ ocn::FitVars initial_vars(const ocn_params& p)
     v.V     = p.wind_speed;
     v.theta = std::atan2(p.wind_dir_y, p.wind_dir_x);
     v.A     = p.amplitude;
+    ocn::fit_vars_derive(v, p.gravity);
     return v;
 }
 
@@ -1760,6 +1761,7 @@ int ocn_fit_set_vars(ocn_fit* f, int member, const double vars[3])
     v.V     = vars[0];
     v.theta = vars[1];
     v.A     = vars[2];
+    ocn::fit_vars_derive(v, f->params[member].gravity);
     hipError_t e = hipSetDevice(f->device);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipMemcpy(f->members[member].vars, &v, sizeof(v), hipMemcpyHostToDevice);

@@ -30,7 +30,25 @@ struct FitVars {
     double qz[3], qg[3], qL, qH[6], qr;
     int    qk;             // BFGS updates taken
     int    pad_;
+    // Single-precision constants of the step kernel derived from (V, theta, A)
+    // by fit_vars_derive, which every writer of the variables calls:
+    // Lp^2 = (V^2/g)^2 and its reciprocal, A, the wind direction, the factors
+    // of dL/dV and dL/dA (2 g^2 / V^5, 1 / (2A)) and sqrt(A)
+    float  Lp2, iLp2, Af, wx, wy, dv_scale, da_scale, sqA;
 };
+
+__host__ __device__ inline void fit_vars_derive(FitVars& v, double g)
+{
+    const double Lp = v.V * v.V / g;
+    v.Lp2      = (float)(Lp * Lp);
+    v.iLp2     = 1.0f / v.Lp2;
+    v.Af       = (float)v.A;
+    v.wx       = (float)cos(v.theta);
+    v.wy       = (float)sin(v.theta);
+    v.dv_scale = (float)(2.0 * g * g / (v.V * v.V * v.V * v.V * v.V));
+    v.da_scale = (float)(0.5 / v.A);
+    v.sqA      = (float)sqrt(v.A);
+}
 
 struct AdamCfg {
     double lr[3];

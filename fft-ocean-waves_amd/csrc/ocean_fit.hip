@@ -45,12 +45,11 @@ __device__ __forceinline__ SpecConst fit_spec(const FitMember& m, const FitVars&
     SpecConst sc{};
     sc.N = m.N;
     sc.kscale = m.kscale;
-    const double Lp = v.V * v.V / m.g_d;
-    sc.Lp2 = (float)(Lp * Lp);
+    sc.Lp2 = v.Lp2;
     sc.l2 = m.l2;
-    sc.A = (float)v.A;
-    sc.wx = (float)cos(v.theta);
-    sc.wy = (float)sin(v.theta);
+    sc.A = v.Af;
+    sc.wx = v.wx;
+    sc.wy = v.wy;
     sc.g = m.g;
     sc.conj_mode = m.conj_mode;
     sc.k2_d = m.k2_d;
@@ -202,11 +201,8 @@ __device__ __forceinline__ void fit_spectral(const FitMember& m, int item, int m
     const size_t    base = (size_t)strip * (N * C);
     const float     inv_nn = 1.0f / ((float)N * (float)N);
     const float     gsc  = 2.0f * m.inv_knn * inv_nn;
-    const double    V = vars.V, A = vars.A;
-    const float     dv_scale = (float)(2.0 * m.g2_d / (V * V * V * V * V));
-    const float     da_scale = (float)(0.5 / A);
-    const float     iLp2     = 1.0f / sc.Lp2;
-    const float     sqA      = (float)sqrt(A);
+    const float     dv_scale = vars.dv_scale, da_scale = vars.da_scale;
+    const float     iLp2 = vars.iLp2, sqA = vars.sqA;
     float lsum = 0.0f, dV = 0.0f, dT = 0.0f, dA = 0.0f;
     // A pair's loads (its noise coefficients, phase-table entry and the first
     // four frames' T^) are issued one pair ahead, during the previous pair's
@@ -597,6 +593,7 @@ __global__ __launch_bounds__(kFitReduceThreads) void k_fit_reduce(const FitMembe
         v.V = fmax(v.V, 1e-3);
         v.A = fmax(v.A, 1e-6);
     }
+    fit_vars_derive(v, m.g_d);
     *m.vars = v;
 }
 
