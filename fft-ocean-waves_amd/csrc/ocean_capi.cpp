@@ -1604,8 +1604,9 @@ int ocn_fit_create(const ocn_params* params, int count, int K, const double* tim
     hipError_t e  = hipGetDevice(&f->device);
     if (e != hipSuccess) rc = hip_fail(e, "hipGetDevice");
     // Per member, 256-byte aligned: xi 16, scratch 8, targets 4 K, dL/dh0 16 bytes per texel, partials,
-    // the targets' spectra T^ (8 bytes per frame and texel pair of the half plane) and with equally spaced
-    // times the phase table (16 bytes per texel pair of the half plane).
+    // the targets' spectra T^ (8 bytes per frame and texel pair of the half plane), with equally spaced
+    // times the phase table (16 bytes per texel pair of the half plane), the noise coefficients (16 bytes
+    // per texel pair of the half plane) and the unit noise of the edge pairs (32 bytes each).
     // Small fits split each column strip's rows over up to four blocks so the
     // step launch has at least two blocks per CU of the 256.
     int row_parts = 1;
